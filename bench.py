@@ -1,0 +1,194 @@
+"""Benchmark: SpMV GB/s as % of HBM peak, and PageRank time to 1e-10 L1.
+
+Workload (one GPU): rank (0,0)'s block of a 4x4 2D distribution of
+fill_cqmat(2^24, 2^24, c=16, q=0.1, seed 42) -- 4,194,304 x 4,194,304,
+62,075,592 entries -- with values 1 - u01(42, i) and x_j = u01(1, j).
+A step is one FP64 CSR SpMV of that block with the inputs resident in HBM.
+
+Also reported (extra fields): the other formats on the same block, and
+PageRank on fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) in FP64, FP32 and
+BF16 value storage and value-free FP64, each stopped by the sum-normalised
+bound and checked against the FP64 fixed point.
+
+    python bench.py [--steps K] [--warmup W] [--no-pagerank] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SpMV GB/s as % of HBM peak; PageRank s-to-1e-10 L1 (mixed vs FP64 GPU)"
+N = 1 << 24
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_kernel(torch, fn, steps, warmup):
+    """Average device time per call of fn() over `steps` calls, measured with
+    HIP events on torch's current stream (the stream the kernels run on)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{name}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def cpu_baseline(tb_csr, x, bytes_alg):
+    """The reference's own CSR Ax (oracle/_ref, -O3 -fopenmp) on the same block."""
+    from oracle.oracle import Reference
+    if not Reference.available():
+        return None
+    a = tb_csr.arrays()
+    info = tb_csr.info()
+    m = Reference.from_arrays("CSR", info["ln_row"], info["ln_col"], a["values"], a["rowptr"], a["colidx"])
+    r = np.zeros(info["ln_row"])
+    Reference.Ax_repeat(m, r, x, 1)
+    reps = 10
+    t = time.perf_counter()
+    Reference.Ax_repeat(m, r, x, reps)
+    dt = (time.perf_counter() - t) / reps
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(bytes_alg / dt / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "reference",
+            "sample": f"{reps} x MatrixCSR::Ax on the full block (reference sources, g++ -O3 -fopenmp), "
+                      f"{dt * 1e3:.1f} ms each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-pagerank", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-formats", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the 2D block "
+              "distribution is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import tbsla_amd as tb
+    from oracle.oracle import Oracle
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("generating the CSR block")
+    t = time.time()
+    csr = tb.MatrixCSR().fill_cqmat(N, N, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+    log(f"generated in {time.time() - t:.1f}s")
+    info = csr.info()
+    x_host = Oracle.uniform_vector(1, info["ln_col"])
+    x = torch.from_numpy(x_host).to(dev)
+    y = torch.zeros(info["ln_row"], dtype=torch.float64, device=dev)
+    csr.prepare("f64")
+    mi = csr.mirror_info("f64")
+    ms = time_kernel(torch, lambda: csr.Ax_device(x, y), args.steps, args.warmup)
+    gbs = mi["algorithmic_bytes"] / (ms * 1e-3) / 1e9
+    log(f"CSR {ms * 1e3:.1f} us/launch {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
+    # one parity probe on a slice (the full check lives in the tests)
+    a = csr.arrays()
+    e = a["rowptr"][4096]
+    yo = Oracle.from_arrays("CSR", 4096, info["ln_col"], a["values"][:e], a["rowptr"][:4097],
+                            a["colidx"][:e]).spmv(x_host)
+    parity = bool(np.array_equal(y[:4096].cpu().numpy(), yo))
+    del a
+
+    out = {"metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "CSR SpMV, block (0,0) of a 4x4 grid over fill_cqmat(2^24,2^24,16,0.1,42)",
+                      "n_row": info["ln_row"], "n_col": info["ln_col"], "nnz": info["n_values"],
+                      "values": "1-u01(42,i)", "x": "u01(1,j)", "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
+                      "parity_first_4096_rows_bit_exact": parity},
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("csr_tile_kernel_cqmat2^24_block"),
+                        "kernel": "csr_tile_kernel<double,double,double>",
+                        "algorithmic_bytes_per_launch": mi["algorithmic_bytes"]}}
+
+    if not args.no_formats:
+        fm = {}
+        for fmt in ["ELL", "SCOO", "COO"]:
+            m = tb.FORMATS[fmt]().fill_cqmat(N, N, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+            mi2 = m.mirror_info("f64")
+            nx = N if fmt == "COO" else m.get_ln_col()
+            xx = torch.from_numpy(Oracle.uniform_vector(1, nx)).to(dev)
+            yy = torch.zeros(m.get_n_row(), dtype=torch.float64, device=dev)
+            ms2 = time_kernel(torch, lambda: m.Ax_device(xx, yy), max(10, args.steps // 4), 3)
+            fm[fmt] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
+                       "bytes": mi2["algorithmic_bytes"]}
+            log(f"{fmt} {ms2 * 1e3:.1f} us {fm[fmt]['gbs']} GB/s")
+            del m, xx, yy
+            torch.cuda.empty_cache()
+        m = tb.MatrixDENSE().fill_cqmat(32768, 32768, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+        mi2 = m.mirror_info("f64")
+        xx = torch.from_numpy(Oracle.uniform_vector(1, 8192)).to(dev)
+        yy = torch.zeros(32768, dtype=torch.float64, device=dev)
+        ms2 = time_kernel(torch, lambda: m.Ax_device(xx, yy), 20, 3)
+        fm["DENSE"] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
+                       "bytes": mi2["algorithmic_bytes"]}
+        log(f"DENSE {ms2 * 1e3:.1f} us {fm['DENSE']['gbs']} GB/s")
+        out["formats"] = fm
+
+    if not args.no_cpu:
+        log("timing the reference CPU path")
+        out["cpu_baseline"] = cpu_baseline(csr, x_host, mi["algorithmic_bytes"])
+    del csr, x, y
+    torch.cuda.empty_cache()
+
+    if not args.no_pagerank:
+        log("generating the PageRank graph")
+        g = tb.MatrixCSR().fill_cqmat_stochastic(N, N, 8, 0.5, 7)
+        n = g.get_n_col()
+        pstar = torch.zeros(n, dtype=torch.float64, device=dev)
+        g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
+        pr = {}
+        for prec in ["f64", "f32", "bf16", "pattern"]:
+            o = torch.zeros(n, dtype=torch.float64, device=dev)
+            g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")  # warm
+            rep = g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")
+            err = float((o - pstar).abs().sum().item())
+            pr[prec] = {"seconds": round(rep["seconds"], 6), "iterations": rep["iterations"],
+                        "anchors": rep["anchors"], "l1_vs_fp64_fixed_point": err}
+            log(f"pagerank {prec}: {rep['seconds'] * 1e3:.2f} ms, {rep['iterations']} it, "
+                f"{rep['anchors']} anchors, L1 {err:.2e}")
+        base = pr["f64"]["seconds"]
+        for prec in pr:
+            pr[prec]["speedup_vs_fp64"] = round(base / pr[prec]["seconds"], 3)
+        out["pagerank"] = {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7), beta 0.85",
+                           "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr}
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,112 @@
+/*
+ * tbsla_hip.h -- C ABI of libtbsla_hip.so for host runtimes and bindings.
+ *
+ * Plain pointers, sizes and integer status codes (0 = success, < 0 = error;
+ * tbsla_last_error() returns the message of the calling thread's last
+ * failure).  A tbsla_matrix wraps one tbsla::cpp::Matrix<F> object (host
+ * arrays, bit-identical to the reference generators) plus its lazily built
+ * HBM mirror.  `stream` arguments are hipStream_t values (NULL = default).
+ *
+ * Reference interfaces each group replaces:
+ *   create/destroy/fill_* /normalize   Matrix.hpp:37-41, MatrixCSR.cpp:234-435
+ *   convert / from_arrays              MatrixCSR.hpp:14-15 (X(const MatrixCOO&), array ctors)
+ *   read / write                       MatrixCSR.cpp:179-232 (binary format)
+ *   spmv / Ax / a_axpx                 Matrix.hpp:12-14, MatrixCSR.cpp:123-140, Matrix.cpp:6-11
+ *   page_rank / personalized           Matrix.hpp:16-17, Matrix.cpp:13-114
+ *   tbsla_hip_*                        device-resident forms of the same products
+ *                                      (no reference counterpart: the reference is CPU-only)
+ */
+#ifndef TBSLA_HIP_H
+#define TBSLA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tbsla_format { TBSLA_COO = 0, TBSLA_SCOO = 1, TBSLA_CSR = 2, TBSLA_ELL = 3, TBSLA_DENSE = 4 };
+enum tbsla_dtype { TBSLA_F64 = 0, TBSLA_F32 = 1, TBSLA_BF16 = 2, TBSLA_PATTERN = 3 };
+enum tbsla_status { TBSLA_OK = 0, TBSLA_EINVAL = -1, TBSLA_EHIP = -2, TBSLA_EIO = -3, TBSLA_EOTHER = -4 };
+
+typedef struct tbsla_matrix tbsla_matrix;
+
+int tbsla_abi_version(void);
+const char *tbsla_last_error(void);
+
+/* ---- host matrix (reference semantics) ---- */
+tbsla_matrix *tbsla_matrix_create(int format);
+void tbsla_matrix_destroy(tbsla_matrix *m);
+int tbsla_matrix_fill_cdiag(tbsla_matrix *m, int n_row, int n_col, int cdiag, int pr, int pc, int NR, int NC);
+/* value_seed = 0: reference values (1.0); otherwise value = 1 - u01(value_seed, i),
+ * i the entry's global generation index (project overlay, no reference counterpart) */
+int tbsla_matrix_fill_cqmat(tbsla_matrix *m, int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr,
+                            int pc, int NR, int NC, uint64_t value_seed);
+int tbsla_matrix_fill_cqmat_stochastic(tbsla_matrix *m, int n_row, int n_col, int c, double q,
+                                       unsigned int seed_mult, int pr, int pc, int NR, int NC);
+int tbsla_matrix_normalize_columns(tbsla_matrix *m);
+tbsla_matrix *tbsla_matrix_convert(const tbsla_matrix *coo, int format);
+tbsla_matrix *tbsla_matrix_from_arrays(int format, int n_row, int n_col, const double *values, size_t n_values,
+                                       const int *a, size_t na, const int *b, size_t nb);
+/* out[16]: format n_row n_col f_row f_col ln_row ln_col pr pc NR NC nnz max_col
+ *          n_values n_idx0 n_idx1 */
+int tbsla_matrix_info(const tbsla_matrix *m, long *out16);
+/* which: 0 values, 1 rowptr (CSR) / row (COO, SCOO), 2 colidx (CSR) / col (COO, SCOO) / columns (ELL) */
+int tbsla_matrix_array(const tbsla_matrix *m, int which, const void **data, size_t *n);
+int tbsla_matrix_write(tbsla_matrix *m, const char *path); /* appends, like C_CPP_Matrix<F>_write */
+int tbsla_matrix_read(tbsla_matrix *m, const char *path, long seek);
+int tbsla_matrix_count_stochastic(const tbsla_matrix *m);
+
+/* ---- host-vector products on the GPU ---- */
+int tbsla_matrix_spmv(tbsla_matrix *m, const double *x, size_t nx, int vect_incr, double *y, size_t ny);
+int tbsla_matrix_Ax(tbsla_matrix *m, double *r, size_t nr, const double *x, size_t nx, int vect_incr);
+int tbsla_matrix_a_axpx(tbsla_matrix *m, const double *x, size_t nx, double *y, size_t ny);
+int tbsla_matrix_page_rank(tbsla_matrix *m, double epsilon, double beta, int max_iteration, double *out, size_t n,
+                           int *iterations);
+int tbsla_matrix_personalized_page_rank(tbsla_matrix *m, double epsilon, double beta, int max_iteration,
+                                        const int *nodes, int n_nodes, double *out, size_t n, int *iterations);
+
+/* ---- device-resident products ---- */
+/* Build the HBM mirror with values of `value_dtype` (idempotent). */
+int tbsla_hip_prepare(tbsla_matrix *m, int value_dtype);
+/* out[8]: rows, entries, x_need, y_lo, y_hi, tiles, mirror bytes, algorithmic bytes
+ *         (for x/y elements of the mirror's natural type) */
+int tbsla_hip_mirror_info(tbsla_matrix *m, int value_dtype, long *out8);
+/* Ax semantics on device vectors (y assigned for CSR/ELL, accumulated for
+ * COO/SCOO/DENSE).  Supported: values F64 with x,y F64; values F32/BF16 with
+ * x,y F32.  Sizes in elements; rejected when too short for the matrix. */
+int tbsla_hip_Ax(tbsla_matrix *m, int value_dtype, const void *x, size_t nx, int x_dtype, void *y, size_t ny,
+                 int y_dtype, int vect_incr, void *stream);
+/* y[0:n) = value (a kernel, safe inside stream capture) */
+int tbsla_hip_fill(void *y, int y_dtype, size_t n, double value, void *stream);
+
+typedef struct {
+  double epsilon;
+  double beta;
+  int max_iteration;
+  int precision;     /* tbsla_dtype: F64, F32, BF16 (values), PATTERN (value-free) */
+  int rule;          /* 0: reference residual < epsilon; 1: sum-normalised bound */
+  int check_every;   /* iterations between convergence reads */
+  double anchor_tau; /* 0: default */
+  const int *nodes;  /* personalised teleport set, or NULL */
+  int n_nodes;
+} tbsla_pagerank_params;
+
+typedef struct {
+  int iterations;
+  int anchors;
+  double last_error;
+  double bound;
+  double seconds;
+} tbsla_pagerank_report;
+
+/* Sum-normalised PageRank vector (n_col doubles) into device memory. */
+int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, void *stream,
+                        tbsla_pagerank_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TBSLA_HIP_H */

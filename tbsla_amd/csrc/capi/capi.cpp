@@ -1,0 +1,521 @@
+// C ABIs of libtbsla_hip.so: the reference's C_CPP_* interface
+// (src/cpp/interface/c/matrix/*.cpp) and the tbsla_* / tbsla_hip_* interface
+// declared in include/tbsla_hip.h.
+#include <tbsla/cpp/MatrixCOO.hpp>
+#include <tbsla/cpp/MatrixCSR.hpp>
+#include <tbsla/cpp/MatrixDENSE.hpp>
+#include <tbsla/cpp/MatrixELL.hpp>
+#include <tbsla/cpp/MatrixSCOO.hpp>
+
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <numeric>
+#include <string>
+
+#include "../../../include/tbsla_cpp_c.h"
+#include "../../../include/tbsla_hip.h"
+#include "../device/kernels.hpp"
+#include "../device/mirror.hpp"
+#include "../device/pagerank.hpp"
+#include "../device/products.hpp"
+#include "../host/fill_common.hpp"
+
+using namespace tbsla;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return TBSLA_OK;
+  } catch (const hip::HipError &e) {
+    return fail(TBSLA_EHIP, e.what());
+  } catch (const std::invalid_argument &e) {
+    return fail(TBSLA_EINVAL, e.what());
+  } catch (const std::exception &e) {
+    return fail(TBSLA_EOTHER, e.what());
+  }
+}
+
+cpp::Matrix *make_matrix(int format) {
+  switch (format) {
+    case TBSLA_COO: return new cpp::MatrixCOO();
+    case TBSLA_SCOO: return new cpp::MatrixSCOO();
+    case TBSLA_CSR: return new cpp::MatrixCSR();
+    case TBSLA_ELL: return new cpp::MatrixELL();
+    case TBSLA_DENSE: return new cpp::MatrixDENSE();
+  }
+  return nullptr;
+}
+
+hip::Dtype dtype_of(int d) {
+  if (d < 0 || d > 3) throw std::invalid_argument("tbsla: unknown dtype " + std::to_string(d));
+  return static_cast<hip::Dtype>(d);
+}
+
+}  // namespace
+
+struct tbsla_matrix {
+  std::unique_ptr<cpp::Matrix> m;
+};
+
+// ===================================================================== C_CPP
+
+namespace {
+std::vector<double> *vec(C_CPP_Vector_t *v) { return static_cast<std::vector<double> *>(v->obj); }
+}  // namespace
+
+extern "C" {
+
+// Vector.cpp:13-22
+C_CPP_Vector_t *C_CPP_Vector_create(void) {
+  C_CPP_Vector_t *v = static_cast<C_CPP_Vector_t *>(std::malloc(sizeof(C_CPP_Vector_t)));
+  v->obj = new std::vector<double>();
+  return v;
+}
+// Vector.cpp:24-29
+void C_CPP_Vector_destroy(C_CPP_Vector_t *v) {
+  if (v == NULL) return;
+  delete vec(v);
+  std::free(v);
+}
+// Vector.cpp:31-40
+void C_CPP_Vector_print(C_CPP_Vector_t *v) {
+  if (v == NULL) {
+    std::printf("C_CPP_Vector_print : input vector is NULL\n");
+    return;
+  }
+  host::streamvector<double>(std::cout, "v", *vec(v));
+  std::cout << std::endl << std::flush;
+}
+// Vector.cpp:42-50: takes the object, the previous one is not freed
+void C_CPP_Vector_set(C_CPP_Vector_t *v, void *obj) {
+  if (v == NULL) return;
+  v->obj = obj;
+}
+// Vector.cpp:52-61
+void C_CPP_Vector_copy(C_CPP_Vector_t *v, void *obj) {
+  if (v == NULL) return;
+  const auto *src = static_cast<std::vector<double> *>(obj);
+  vec(v)->assign(src->begin(), src->end());
+}
+// Vector.cpp:63-71
+void C_CPP_Vector_fill(C_CPP_Vector_t *v, int n, int s) {
+  if (v == NULL) return;
+  auto *o = vec(v);
+  o->clear();
+  o->resize(static_cast<std::size_t>(n));
+  std::iota(o->begin(), o->end(), s);
+}
+// Vector.cpp:73-88
+bool C_CPP_Vector_read(C_CPP_Vector_t *v, char *filename, int seek) {
+  if (v == NULL) return false;
+  std::ifstream is(std::string(filename), std::ifstream::binary);
+  if (!is.is_open()) return false;
+  is.seekg(seek, is.beg);
+  host::read_array(is, *vec(v));
+  return true;
+}
+// Vector.cpp:90-103 (appends)
+bool C_CPP_Vector_write(C_CPP_Vector_t *v, char *filename) {
+  if (v == NULL) return false;
+  std::ofstream os(std::string(filename), std::ofstream::binary | std::ofstream::app);
+  if (!os.is_open()) return false;
+  host::write_array(os, *vec(v));
+  return true;
+}
+// Vector.cpp:105-114
+bool C_CPP_Vector_add(C_CPP_Vector_t *v1, C_CPP_Vector_t *v2) {
+  if (v1 == NULL || v2 == NULL) return false;
+  auto *a = vec(v1), *b = vec(v2);
+  if (b->size() < a->size()) return false;
+  std::transform(a->begin(), a->end(), b->begin(), a->begin(), std::plus<double>());
+  return true;
+}
+// Vector.cpp:116-125
+bool C_CPP_Vector_add_incr(C_CPP_Vector_t *v1, C_CPP_Vector_t *v2, int incr) {
+  if (v1 == NULL || v2 == NULL) return false;
+  auto *a = vec(v1), *b = vec(v2);
+  if (incr < 0 || b->size() < a->size() + static_cast<std::size_t>(incr)) return false;
+  std::transform(a->begin(), a->end(), b->begin() + incr, a->begin(), std::plus<double>());
+  return true;
+}
+// Vector.cpp:127-136
+bool C_CPP_Vector_gather(C_CPP_Vector_t *v1, C_CPP_Vector_t *v2) {
+  if (v1 == NULL || v2 == NULL) return false;
+  auto *a = vec(v1), *b = vec(v2);
+  a->insert(a->end(), b->begin(), b->end());
+  return true;
+}
+
+}  // extern "C"
+
+// interface/c/matrix/MatrixCSR.cpp:4-93, the same for every format
+#define TBSLA_C_MATRIX_DEF(F)                                                                                  \
+  struct C_CPP_Matrix##F {                                                                                     \
+    void *obj;                                                                                                 \
+  };                                                                                                           \
+  extern "C" C_CPP_Matrix##F##_t *C_CPP_Matrix##F##_create(void) {                                             \
+    C_CPP_Matrix##F##_t *m = static_cast<C_CPP_Matrix##F##_t *>(std::malloc(sizeof(C_CPP_Matrix##F##_t)));     \
+    m->obj = new cpp::Matrix##F();                                                                             \
+    return m;                                                                                                  \
+  }                                                                                                            \
+  extern "C" void C_CPP_Matrix##F##_destroy(C_CPP_Matrix##F##_t *m) {                                          \
+    if (m == NULL) return;                                                                                     \
+    delete static_cast<cpp::Matrix##F *>(m->obj);                                                             \
+    std::free(m);                                                                                              \
+  }                                                                                                            \
+  extern "C" void C_CPP_Matrix##F##_fill_cdiag(C_CPP_Matrix##F##_t *m, int n_row, int n_col, int cdiag, int pr,  \
+                                               int pc, int NR, int NC) {                                       \
+    if (m == NULL) return;                                                                                     \
+    static_cast<cpp::Matrix##F *>(m->obj)->fill_cdiag(n_row, n_col, cdiag, pr, pc, NR, NC);                    \
+  }                                                                                                            \
+  extern "C" void C_CPP_Matrix##F##_fill_cqmat(C_CPP_Matrix##F##_t *m, int n_row, int n_col, int c, double q,    \
+                                               unsigned int seed_mult, int pr, int pc, int NR, int NC) {       \
+    if (m == NULL) return;                                                                                     \
+    static_cast<cpp::Matrix##F *>(m->obj)->fill_cqmat(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC);          \
+  }                                                                                                            \
+  extern "C" bool C_CPP_Matrix##F##_read(C_CPP_Matrix##F##_t *m, char *filename, int seek) {                   \
+    if (m == NULL) return false;                                                                               \
+    std::ifstream is(std::string(filename), std::ifstream::binary);                                            \
+    if (!is.is_open()) return false;                                                                           \
+    is.seekg(seek, is.beg);                                                                                    \
+    static_cast<cpp::Matrix##F *>(m->obj)->read(is);                                                           \
+    return true;                                                                                               \
+  }                                                                                                            \
+  extern "C" bool C_CPP_Matrix##F##_write(C_CPP_Matrix##F##_t *m, char *filename) {                            \
+    if (m == NULL) return false;                                                                               \
+    std::ofstream os(std::string(filename), std::ofstream::binary | std::ofstream::app);                       \
+    if (!os.is_open()) return false;                                                                           \
+    static_cast<cpp::Matrix##F *>(m->obj)->write(os);                                                          \
+    return true;                                                                                               \
+  }                                                                                                            \
+  extern "C" void C_CPP_Matrix##F##_print(C_CPP_Matrix##F##_t *m) {                                            \
+    if (m == NULL) return;                                                                                     \
+    auto *o = static_cast<cpp::Matrix##F *>(m->obj);                                                           \
+    o->print_infos(std::cerr);                                                                                 \
+    std::cerr << *o << std::endl;                                                                              \
+  }                                                                                                            \
+  extern "C" C_CPP_Vector_t *C_CPP_Matrix##F##_spmv(C_CPP_Matrix##F##_t *m, C_CPP_Vector_t *v) {               \
+    C_CPP_Vector_t *r = C_CPP_Vector_create();                                                                 \
+    if (m == NULL || v == NULL) return r;                                                                      \
+    guarded([&] {                                                                                              \
+      std::vector<double> y = static_cast<cpp::Matrix##F *>(m->obj)->spmv(*vec(v), 0);                         \
+      C_CPP_Vector_copy(r, &y);                                                                                \
+    });                                                                                                        \
+    return r;                                                                                                  \
+  }
+
+TBSLA_C_MATRIX_DEF(COO)
+TBSLA_C_MATRIX_DEF(SCOO)
+TBSLA_C_MATRIX_DEF(CSR)
+TBSLA_C_MATRIX_DEF(ELL)
+TBSLA_C_MATRIX_DEF(DENSE)
+
+// ================================================================ tbsla_hip
+
+namespace {
+struct Arrays {
+  const std::vector<double> *v = nullptr;
+  const std::vector<int> *i0 = nullptr, *i1 = nullptr;
+  int max_col = 0;
+};
+Arrays arrays_of(const cpp::Matrix &m) {
+  Arrays a;
+  switch (m.format()) {
+    case cpp::FORMAT_COO: {
+      const auto &x = dynamic_cast<const cpp::MatrixCOO &>(m);
+      a.v = &x.get_values(); a.i0 = &x.get_row(); a.i1 = &x.get_col();
+      break;
+    }
+    case cpp::FORMAT_SCOO: {
+      const auto &x = dynamic_cast<const cpp::MatrixSCOO &>(m);
+      a.v = &x.get_values(); a.i0 = &x.get_row(); a.i1 = &x.get_col();
+      break;
+    }
+    case cpp::FORMAT_CSR: {
+      const auto &x = dynamic_cast<const cpp::MatrixCSR &>(m);
+      a.v = &x.get_values(); a.i0 = &x.get_rowptr(); a.i1 = &x.get_colidx();
+      break;
+    }
+    case cpp::FORMAT_ELL: {
+      const auto &x = dynamic_cast<const cpp::MatrixELL &>(m);
+      a.v = &x.get_values(); a.i1 = &x.get_columns(); a.max_col = x.get_max_col();
+      break;
+    }
+    case cpp::FORMAT_DENSE: {
+      const auto &x = dynamic_cast<const cpp::MatrixDENSE &>(m);
+      a.v = &x.get_values();
+      break;
+    }
+  }
+  return a;
+}
+}  // namespace
+
+
+extern "C" {
+
+int tbsla_abi_version(void) { return 1; }
+const char *tbsla_last_error(void) { return g_last_error.c_str(); }
+
+tbsla_matrix *tbsla_matrix_create(int format) {
+  cpp::Matrix *m = make_matrix(format);
+  if (!m) {
+    fail(TBSLA_EINVAL, "tbsla: unknown format " + std::to_string(format));
+    return nullptr;
+  }
+  return new tbsla_matrix{std::unique_ptr<cpp::Matrix>(m)};
+}
+
+void tbsla_matrix_destroy(tbsla_matrix *m) { delete m; }
+
+int tbsla_matrix_fill_cdiag(tbsla_matrix *m, int n_row, int n_col, int cdiag, int pr, int pc, int NR, int NC) {
+  if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL matrix");
+  if (NR <= 0 || NC <= 0 || pr < 0 || pc < 0 || pr >= NR || pc >= NC || n_row < 0 || n_col < 0)
+    return fail(TBSLA_EINVAL, "tbsla: bad block layout");
+  return guarded([&] { m->m->fill_cdiag(n_row, n_col, cdiag, pr, pc, NR, NC); });
+}
+
+int tbsla_matrix_fill_cqmat(tbsla_matrix *m, int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr,
+                            int pc, int NR, int NC, uint64_t value_seed) {
+  if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL matrix");
+  if (NR <= 0 || NC <= 0 || pr < 0 || pc < 0 || pr >= NR || pc >= NC || n_row < 0 || n_col < 0 || c < 0)
+    return fail(TBSLA_EINVAL, "tbsla: bad block layout");
+  return guarded([&] { m->m->fill_cqmat_values(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed); });
+}
+
+int tbsla_matrix_fill_cqmat_stochastic(tbsla_matrix *m, int n_row, int n_col, int c, double q,
+                                       unsigned int seed_mult, int pr, int pc, int NR, int NC) {
+  if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL matrix");
+  if (NR <= 0 || NC <= 0 || pr < 0 || pc < 0 || pr >= NR || pc >= NC || n_row < 0 || n_col < 0 || c <= 0)
+    return fail(TBSLA_EINVAL, "tbsla: bad block layout");
+  return guarded([&] { m->m->fill_cqmat_stochastic(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC); });
+}
+
+int tbsla_matrix_normalize_columns(tbsla_matrix *m) {
+  if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL matrix");
+  return guarded([&] { m->m->normalize_columns(); });
+}
+
+tbsla_matrix *tbsla_matrix_convert(const tbsla_matrix *coo, int format) {
+  if (!coo || coo->m->format() != cpp::FORMAT_COO) {
+    fail(TBSLA_EINVAL, "tbsla: convert needs a COO matrix");
+    return nullptr;
+  }
+  const auto &c = dynamic_cast<const cpp::MatrixCOO &>(*coo->m);
+  cpp::Matrix *out = nullptr;
+  int rc = guarded([&] {
+    switch (format) {
+      case TBSLA_COO: out = new cpp::MatrixCOO(c); break;
+      case TBSLA_SCOO: out = new cpp::MatrixSCOO(c); break;
+      case TBSLA_CSR: out = new cpp::MatrixCSR(c); break;
+      case TBSLA_ELL: out = new cpp::MatrixELL(c); break;
+      case TBSLA_DENSE: out = new cpp::MatrixDENSE(c); break;
+      default: throw std::invalid_argument("tbsla: unknown format");
+    }
+  });
+  if (rc) return nullptr;
+  return new tbsla_matrix{std::unique_ptr<cpp::Matrix>(out)};
+}
+
+tbsla_matrix *tbsla_matrix_from_arrays(int format, int n_row, int n_col, const double *values, size_t n_values,
+                                       const int *a, size_t na, const int *b, size_t nb) {
+  std::vector<double> v(values, values + n_values);
+  std::vector<int> va(a, a + na), vb(b, b + nb);
+  cpp::Matrix *out = nullptr;
+  int rc = guarded([&] {
+    switch (format) {
+      case TBSLA_COO: out = new cpp::MatrixCOO(n_row, n_col, v, va, vb); break;
+      case TBSLA_SCOO: out = new cpp::MatrixSCOO(n_row, n_col, v, va, vb); break;
+      case TBSLA_CSR: out = new cpp::MatrixCSR(n_row, n_col, v, va, vb); break;
+      default: throw std::invalid_argument("tbsla: array constructors exist for COO, SCOO and CSR");
+    }
+  });
+  if (rc) return nullptr;
+  return new tbsla_matrix{std::unique_ptr<cpp::Matrix>(out)};
+}
+
+
+int tbsla_matrix_info(const tbsla_matrix *m, long *out) {
+  if (!m || !out) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  const cpp::Matrix &x = *m->m;
+  const Arrays a = arrays_of(x);
+  const long v[16] = {x.format(), x.get_n_row(), x.get_n_col(), x.get_f_row(), x.get_f_col(), x.get_ln_row(),
+                      x.get_ln_col(), x.get_pr(), x.get_pc(), x.get_NR(), x.get_NC(), x.get_nnz(), a.max_col,
+                      static_cast<long>(a.v ? a.v->size() : 0), static_cast<long>(a.i0 ? a.i0->size() : 0),
+                      static_cast<long>(a.i1 ? a.i1->size() : 0)};
+  std::memcpy(out, v, sizeof(v));
+  return TBSLA_OK;
+}
+
+int tbsla_matrix_array(const tbsla_matrix *m, int which, const void **data, size_t *n) {
+  if (!m || !data || !n) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  const Arrays a = arrays_of(*m->m);
+  *data = nullptr;
+  *n = 0;
+  if (which == 0 && a.v) {
+    *data = a.v->data();
+    *n = a.v->size();
+  } else if (which == 1 && a.i0) {
+    *data = a.i0->data();
+    *n = a.i0->size();
+  } else if (which == 2 && a.i1) {
+    *data = a.i1->data();
+    *n = a.i1->size();
+  } else if (which < 0 || which > 2) {
+    return fail(TBSLA_EINVAL, "tbsla: array index must be 0, 1 or 2");
+  }
+  return TBSLA_OK;
+}
+
+int tbsla_matrix_write(tbsla_matrix *m, const char *path) {
+  if (!m || !path) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  std::ofstream os(path, std::ofstream::binary | std::ofstream::app);
+  if (!os.is_open()) return fail(TBSLA_EIO, std::string("tbsla: cannot open ") + path);
+  return guarded([&] { m->m->write(os); });
+}
+
+int tbsla_matrix_read(tbsla_matrix *m, const char *path, long seek) {
+  if (!m || !path) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  std::ifstream is(path, std::ifstream::binary);
+  if (!is.is_open()) return fail(TBSLA_EIO, std::string("tbsla: cannot open ") + path);
+  is.seekg(seek, is.beg);
+  return guarded([&] {
+    m->m->read(is);
+    if (!is) throw std::runtime_error("tbsla: truncated matrix file");
+  });
+}
+
+int tbsla_matrix_count_stochastic(const tbsla_matrix *m) { return m && m->m->count_stochastic() ? 1 : 0; }
+
+int tbsla_matrix_spmv(tbsla_matrix *m, const double *x, size_t nx, int vect_incr, double *y, size_t ny) {
+  if (!m || (!x && nx) || (!y && ny)) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  if (ny < static_cast<size_t>(m->m->get_n_row())) return fail(TBSLA_EINVAL, "tbsla: y shorter than n_row");
+  return guarded([&] {
+    std::vector<double> v(x, x + nx);
+    std::vector<double> r = m->m->spmv(v, vect_incr);
+    std::memcpy(y, r.data(), r.size() * sizeof(double));
+  });
+}
+
+int tbsla_matrix_Ax(tbsla_matrix *m, double *r, size_t nr, const double *x, size_t nx, int vect_incr) {
+  if (!m || (!x && nx) || (!r && nr)) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    std::vector<double> v(x, x + nx), rr(r, r + nr);
+    m->m->Ax(rr, v, vect_incr);
+    std::memcpy(r, rr.data(), nr * sizeof(double));
+  });
+}
+
+int tbsla_matrix_a_axpx(tbsla_matrix *m, const double *x, size_t nx, double *y, size_t ny) {
+  if (!m || (!x && nx) || (!y && ny)) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  if (ny < static_cast<size_t>(m->m->get_n_row())) return fail(TBSLA_EINVAL, "tbsla: y shorter than n_row");
+  return guarded([&] {
+    std::vector<double> v(x, x + nx);
+    std::vector<double> r = m->m->a_axpx_(v, 0);
+    std::memcpy(y, r.data(), r.size() * sizeof(double));
+  });
+}
+
+int tbsla_matrix_page_rank(tbsla_matrix *m, double epsilon, double beta, int max_iteration, double *out, size_t n,
+                           int *iterations) {
+  if (!m || !out) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  if (n < static_cast<size_t>(m->m->get_n_col())) return fail(TBSLA_EINVAL, "tbsla: out shorter than n_col");
+  return guarded([&] {
+    int it = 0;
+    std::vector<double> r = m->m->page_rank(epsilon, beta, max_iteration, it);
+    std::memcpy(out, r.data(), r.size() * sizeof(double));
+    if (iterations) *iterations = it;
+  });
+}
+
+int tbsla_matrix_personalized_page_rank(tbsla_matrix *m, double epsilon, double beta, int max_iteration,
+                                        const int *nodes, int n_nodes, double *out, size_t n, int *iterations) {
+  if (!m || !out || (!nodes && n_nodes > 0) || n_nodes <= 0) return fail(TBSLA_EINVAL, "tbsla: bad argument");
+  if (n < static_cast<size_t>(m->m->get_n_col())) return fail(TBSLA_EINVAL, "tbsla: out shorter than n_col");
+  return guarded([&] {
+    int it = 0;
+    std::vector<int> nd(nodes, nodes + n_nodes);
+    std::vector<double> r = m->m->personalized_page_rank(epsilon, beta, max_iteration, nd, it);
+    std::memcpy(out, r.data(), r.size() * sizeof(double));
+    if (iterations) *iterations = it;
+  });
+}
+
+int tbsla_hip_prepare(tbsla_matrix *m, int value_dtype) {
+  if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL matrix");
+  return guarded([&] { (void)hip::mirror(*m->m, dtype_of(value_dtype)); });
+}
+
+int tbsla_hip_mirror_info(tbsla_matrix *m, int value_dtype, long *out) {
+  if (!m || !out) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    const hip::DeviceMatrix &d = hip::mirror(*m->m, dtype_of(value_dtype));
+    const std::size_t xe = d.vdt == hip::Dtype::F64 ? 8 : 4;
+    std::size_t bytes = d.rowptr.bytes() + d.colidx.bytes() + d.vals.bytes() + d.tiles.bytes() + d.ecols.bytes() +
+                        d.evals.bytes() + d.srow.bytes() + d.scol.bytes() + d.sval.bytes() + d.stiles.bytes() +
+                        d.dvals.bytes();
+    const long v[8] = {d.rows, d.entries, d.x_need, d.y_lo, d.y_hi,
+                       d.format == cpp::FORMAT_CSR ? d.ntiles : d.nstiles, static_cast<long>(bytes),
+                       d.noop ? 0L : static_cast<long>(d.algorithmic_bytes(xe, xe))};
+    std::memcpy(out, v, sizeof(v));
+  });
+}
+
+int tbsla_hip_Ax(tbsla_matrix *m, int value_dtype, const void *x, size_t nx, int x_dtype, void *y, size_t ny,
+                 int y_dtype, int vect_incr, void *stream) {
+  if (!m || (!x && nx) || (!y && ny)) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    hip::Ax_device(*m->m, dtype_of(value_dtype), x, nx, dtype_of(x_dtype), y, ny, dtype_of(y_dtype), vect_incr,
+                   static_cast<hipStream_t>(stream));
+  });
+}
+
+int tbsla_hip_fill(void *y, int y_dtype, size_t n, double value, void *stream) {
+  if (!y && n) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    hip::launch_fill(y, dtype_of(y_dtype), static_cast<long>(n), value, static_cast<hipStream_t>(stream));
+  });
+}
+
+int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, void *stream,
+                        tbsla_pagerank_report *report) {
+  if (!m || !p || !out_dev) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    hip::PageRankParams q;
+    q.epsilon = p->epsilon;
+    q.beta = p->beta;
+    q.max_iteration = p->max_iteration;
+    q.personalized = p->nodes != nullptr && p->n_nodes > 0;
+    q.nodes = p->nodes;
+    q.n_nodes = p->n_nodes;
+    if (p->precision < 0 || p->precision > 3) throw std::invalid_argument("tbsla: bad precision");
+    q.precision = static_cast<hip::PrPrecision>(p->precision);
+    if (p->rule < 0 || p->rule > 1) throw std::invalid_argument("tbsla: bad stopping rule");
+    q.rule = static_cast<hip::PrRule>(p->rule);
+    q.check_every = p->check_every > 0 ? p->check_every : 8;
+    q.anchor_tau = p->anchor_tau;
+    hip::PageRankReport r = hip::page_rank_device(*m->m, q, out_dev, static_cast<hipStream_t>(stream));
+    if (report) {
+      report->iterations = r.iterations;
+      report->anchors = r.anchors;
+      report->last_error = r.last_error;
+      report->bound = r.bound;
+      report->seconds = r.seconds;
+    }
+  });
+}
+
+}  // extern "C"

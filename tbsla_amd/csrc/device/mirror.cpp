@@ -1,0 +1,306 @@
+#include "mirror.hpp"
+
+#include <tbsla/cpp/MatrixCOO.hpp>
+#include <tbsla/cpp/MatrixCSR.hpp>
+#include <tbsla/cpp/MatrixDENSE.hpp>
+#include <tbsla/cpp/MatrixELL.hpp>
+#include <tbsla/cpp/MatrixSCOO.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+
+#include "../host/gen.hpp"
+
+namespace tbsla {
+namespace hip {
+namespace {
+
+// Round-to-nearest-even double -> float -> bfloat16 (upper 16 bits).
+unsigned short to_bf16(double d) {
+  float f = static_cast<float>(d);
+  unsigned u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<unsigned short>(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<unsigned short>(u >> 16);
+}
+
+// Upload `n` values from `src` converted to vdt (PATTERN uploads nothing).
+void upload_values(DevBuf &dst, const double *src, std::size_t n, Dtype vdt) {
+  switch (vdt) {
+    case Dtype::F64:
+      dst.upload(src, n * sizeof(double));
+      break;
+    case Dtype::F32: {
+      std::vector<float> t(n);
+      gen::parallel_for(static_cast<long>(n), [&](long b, long e) {
+        for (long i = b; i < e; i++) t[static_cast<std::size_t>(i)] = static_cast<float>(src[i]);
+      });
+      dst.upload(t.data(), n * sizeof(float));
+      break;
+    }
+    case Dtype::BF16: {
+      std::vector<unsigned short> t(n);
+      gen::parallel_for(static_cast<long>(n), [&](long b, long e) {
+        for (long i = b; i < e; i++) t[static_cast<std::size_t>(i)] = to_bf16(src[i]);
+      });
+      dst.upload(t.data(), n * sizeof(unsigned short));
+      break;
+    }
+    case Dtype::PATTERN:
+      break;
+  }
+}
+
+struct MinMax {
+  long lo = std::numeric_limits<long>::max(), hi = std::numeric_limits<long>::min();
+};
+MinMax minmax_offset(const int *a, std::size_t n, long off) {
+  std::vector<MinMax> res;
+  std::mutex mu;
+  gen::parallel_for(static_cast<long>(n), [&](long b, long e) {
+    MinMax m;
+    for (long i = b; i < e; i++) {
+      const long v = static_cast<long>(a[i]) + off;
+      m.lo = std::min(m.lo, v);
+      m.hi = std::max(m.hi, v);
+    }
+    std::lock_guard<std::mutex> g(mu);
+    res.push_back(m);
+  }, 1 << 16);
+  MinMax out;
+  for (auto &m : res) {
+    out.lo = std::min(out.lo, m.lo);
+    out.hi = std::max(out.hi, m.hi);
+  }
+  return out;
+}
+
+[[noreturn]] void bad(const char *what) { throw std::invalid_argument(std::string("tbsla: ") + what); }
+
+void build_csr(DeviceMatrix &d, const cpp::MatrixCSR &m, Dtype vdt) {
+  const auto &vals = m.get_values();
+  const auto &rp = m.get_rowptr();
+  const auto &ci = m.get_colidx();
+  if (vals.empty() || rp.size() < 2) {  // MatrixCSR.cpp:130-131
+    d.noop = true;
+    return;
+  }
+  d.rows = static_cast<int>(rp.size() - 1);
+  d.rp0 = rp[0];
+  for (int i = 0; i < d.rows; i++)
+    if (rp[static_cast<std::size_t>(i) + 1] < rp[static_cast<std::size_t>(i)]) bad("CSR rowptr is not monotone");
+  const long used = static_cast<long>(rp.back()) - rp[0];
+  if (used > static_cast<long>(vals.size()) || used > static_cast<long>(ci.size()))
+    bad("CSR rowptr points past values/colidx");
+  d.entries = used;
+  d.f_col = m.get_f_col();
+  if (used > 0) {
+    const MinMax mm = minmax_offset(ci.data(), static_cast<std::size_t>(used), -static_cast<long>(d.f_col));
+    if (mm.lo < 0) bad("CSR column index below the block's first column");
+    d.x_need = mm.hi + 1;
+  }
+  d.rowptr.upload(rp.data(), rp.size() * sizeof(int));
+  // entry j of row i is values[rowptr[i] - rowptr[0] + k] (MatrixCSR.cpp:135)
+  d.colidx.upload(ci.data(), static_cast<std::size_t>(used) * sizeof(int));
+  upload_values(d.vals, vals.data(), static_cast<std::size_t>(used), vdt);
+  std::vector<int> t = csr_tiles(rp.data(), d.rows, kCsrTileNnz, kCsrTileRows);
+  d.ntiles = static_cast<int>(t.size()) - 1;
+  d.tiles.upload(t.data(), t.size() * sizeof(int));
+}
+
+void build_ell(DeviceMatrix &d, const cpp::MatrixELL &m, Dtype vdt) {
+  const auto &vals = m.get_values();
+  const auto &cols = m.get_columns();
+  const int K = m.get_max_col();
+  if (m.get_nnz() == 0 || K == 0) {  // MatrixELL.cpp:102-103
+    d.noop = true;
+    return;
+  }
+  const long rows = std::min<long>(m.get_ln_row(), static_cast<long>(vals.size() / static_cast<std::size_t>(K)));
+  if (static_cast<long>(cols.size()) < rows * K) bad("ELL columns shorter than values");
+  d.rows = static_cast<int>(rows);
+  d.K = K;
+  d.entries = rows * K;
+  d.f_col = m.get_f_col();
+  d.ld = (rows + 63) / 64 * 64;
+  std::vector<int> ccm(static_cast<std::size_t>(d.ld * K), d.f_col);
+  std::vector<double> vcm(static_cast<std::size_t>(d.ld * K), 0.0);
+  long hi = 0;
+  std::mutex mu;
+  gen::parallel_for(rows, [&](long b, long e) {
+    long h = 0;
+    for (long i = b; i < e; i++)
+      for (long j = 0; j < K; j++) {
+        const int c = cols[static_cast<std::size_t>(i * K + j)];
+        ccm[static_cast<std::size_t>(j * d.ld + i)] = c;
+        vcm[static_cast<std::size_t>(j * d.ld + i)] = vals[static_cast<std::size_t>(i * K + j)];
+        h = std::max<long>(h, std::max<long>(static_cast<long>(c) - d.f_col, 0));
+      }
+    std::lock_guard<std::mutex> g(mu);
+    hi = std::max(hi, h);
+  });
+  d.x_need = hi + 1;
+  d.ecols.upload(ccm.data(), ccm.size() * sizeof(int));
+  upload_values(d.evals, vcm.data(), vcm.size(), vdt);
+}
+
+template <class M>
+void build_coo_like(DeviceMatrix &d, const M &m, Dtype vdt, bool scoo) {
+  const auto &vals = m.get_values();
+  const auto &row = m.get_row();
+  const auto &col = m.get_col();
+  const std::size_t n = vals.size();
+  if (row.size() < n || col.size() < n) bad("COO row/col shorter than values");
+  d.entries = static_cast<long>(n);
+  d.f_row = scoo ? m.get_f_row() : 0;
+  d.f_col = scoo ? m.get_f_col() : 0;
+  if (n == 0) {
+    d.noop = true;
+    return;
+  }
+  const MinMax cx = minmax_offset(col.data(), n, -static_cast<long>(d.f_col));
+  if (cx.lo < 0) bad("COO column index below the block's first column");
+  d.x_need = cx.hi + 1;
+  const MinMax ry = minmax_offset(row.data(), n, -static_cast<long>(d.f_row));
+  d.y_lo = ry.lo;
+  d.y_hi = ry.hi + 1;
+  bool sorted = true;
+  for (std::size_t k = 1; k < n && sorted; k++)
+    if (row[k] < row[k - 1]) sorted = false;
+  std::vector<int> srow, scol;
+  std::vector<double> sval;
+  const int *prow = row.data(), *pcol = col.data();
+  const double *pval = vals.data();
+  if (!sorted) {
+    // stable counting sort by row: each row keeps its entries in input order
+    const long lo = ry.lo + d.f_row, span = ry.hi - ry.lo + 1;
+    std::vector<long> start(static_cast<std::size_t>(span) + 1, 0);
+    for (std::size_t k = 0; k < n; k++) start[static_cast<std::size_t>(row[k] - lo) + 1]++;
+    for (long r = 0; r < span; r++) start[static_cast<std::size_t>(r) + 1] += start[static_cast<std::size_t>(r)];
+    srow.resize(n);
+    scol.resize(n);
+    sval.resize(n);
+    for (std::size_t k = 0; k < n; k++) {
+      const std::size_t p = static_cast<std::size_t>(start[static_cast<std::size_t>(row[k] - lo)]++);
+      srow[p] = row[k];
+      scol[p] = col[k];
+      sval[p] = vals[k];
+    }
+    prow = srow.data();
+    pcol = scol.data();
+    pval = sval.data();
+  }
+  d.srow.upload(prow, n * sizeof(int));
+  d.scol.upload(pcol, n * sizeof(int));
+  upload_values(d.sval, pval, n, vdt);
+  std::vector<int> t32 = segment_tiles(prow, static_cast<long>(n), kCsrTileNnz);
+  std::vector<long> t(t32.begin(), t32.end());
+  d.nstiles = static_cast<int>(t.size()) - 1;
+  d.stiles.upload(t.data(), t.size() * sizeof(long));
+}
+
+void build_dense(DeviceMatrix &d, const cpp::MatrixDENSE &m, Dtype vdt) {
+  const auto &vals = m.get_values();
+  if (m.get_nnz() == 0 || vals.empty()) {  // MatrixDENSE.cpp:65-66
+    d.noop = true;
+    return;
+  }
+  if (vdt != Dtype::F64) bad("DENSE mirrors are FP64 only");
+  const long rows = m.get_ln_row(), cols = m.get_ln_col();
+  if (static_cast<long>(vals.size()) < rows * cols) bad("DENSE values shorter than ln_row * ln_col");
+  d.rows = static_cast<int>(rows);
+  d.dcols = static_cast<int>(cols);
+  d.entries = rows * cols;
+  d.x_need = cols;
+  d.dvals.upload(vals.data(), static_cast<std::size_t>(rows * cols) * sizeof(double));
+}
+
+}  // namespace
+
+std::vector<int> csr_tiles(const int *rowptr, int rows, int max_nnz, int max_rows) {
+  std::vector<int> t;
+  t.reserve(static_cast<std::size_t>(rows) / 64 + 2);
+  t.push_back(0);
+  int r = 0;
+  while (r < rows) {
+    const int start = r;
+    long nnz = 0;
+    while (r < rows && r - start < max_rows) {
+      const long len = static_cast<long>(rowptr[r + 1]) - rowptr[r];
+      if (nnz + len > max_nnz) break;
+      nnz += len;
+      r++;
+    }
+    if (r == start) r++;  // a row longer than one tile stands alone
+    t.push_back(r);
+  }
+  return t;
+}
+
+std::vector<int> segment_tiles(const int *row, long n, int max_nnz) {
+  std::vector<int> t;
+  t.push_back(0);
+  long s = 0;
+  while (s < n) {
+    long e = std::min(n, s + max_nnz);
+    if (e < n) {
+      long b = e;
+      while (b > s && row[b] == row[b - 1]) b--;  // back up to the start of the row cut at e
+      if (b > s) {
+        e = b;
+      } else {
+        while (e < n && row[e] == row[e - 1]) e++;  // one row longer than a tile
+      }
+    }
+    t.push_back(static_cast<int>(e));
+    s = e;
+  }
+  return t;
+}
+
+std::size_t DeviceMatrix::algorithmic_bytes(std::size_t xe, std::size_t ye) const {
+  const std::size_t ve = dtype_size(vdt);
+  switch (format) {
+    case cpp::FORMAT_CSR:
+      return static_cast<std::size_t>(entries) * (ve + 4) + 4 * (static_cast<std::size_t>(rows) + 1) +
+             xe * static_cast<std::size_t>(x_need) + ye * static_cast<std::size_t>(rows);
+    case cpp::FORMAT_ELL:
+      return static_cast<std::size_t>(entries) * (ve + 4) + xe * static_cast<std::size_t>(x_need) +
+             ye * static_cast<std::size_t>(rows);
+    case cpp::FORMAT_COO:
+    case cpp::FORMAT_SCOO:
+      return static_cast<std::size_t>(entries) * (ve + 8) + xe * static_cast<std::size_t>(x_need) +
+             ye * static_cast<std::size_t>(y_hi - y_lo);
+    case cpp::FORMAT_DENSE:
+      return static_cast<std::size_t>(entries) * 8 + xe * static_cast<std::size_t>(dcols) +
+             ye * static_cast<std::size_t>(rows);
+  }
+  return 0;
+}
+
+const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt) {
+  DeviceCache &cache = m.device_cache();
+  std::lock_guard<std::mutex> g(cache.mu);
+  auto &slot = cache.mirrors[static_cast<int>(vdt)];
+  if (slot) return *slot;
+  auto d = std::unique_ptr<DeviceMatrix>(new DeviceMatrix());
+  d->format = m.format();
+  d->vdt = vdt;
+  d->f_row = m.get_f_row();
+  d->f_col = m.get_f_col();
+  switch (m.format()) {
+    case cpp::FORMAT_CSR: build_csr(*d, dynamic_cast<const cpp::MatrixCSR &>(m), vdt); break;
+    case cpp::FORMAT_ELL: build_ell(*d, dynamic_cast<const cpp::MatrixELL &>(m), vdt); break;
+    case cpp::FORMAT_COO: build_coo_like(*d, dynamic_cast<const cpp::MatrixCOO &>(m), vdt, false); break;
+    case cpp::FORMAT_SCOO: build_coo_like(*d, dynamic_cast<const cpp::MatrixSCOO &>(m), vdt, true); break;
+    case cpp::FORMAT_DENSE: build_dense(*d, dynamic_cast<const cpp::MatrixDENSE &>(m), vdt); break;
+  }
+  TBSLA_HIP_CHECK(hipDeviceSynchronize());
+  slot = std::move(d);
+  return *slot;
+}
+
+}  // namespace hip
+}  // namespace tbsla

@@ -1,0 +1,74 @@
+// Device mirror of one host matrix: the HBM layout the kernels read.
+//
+//   CSR   rowptr/colidx as stored, values in the requested type, plus a
+//         row-tile table: tile t = rows [tiles[t], tiles[t+1]) holding at most
+//         kCsrTileNnz entries (a longer row is a tile of its own).
+//   ELL   column-major (slot j of row i at j*ld + i), ld = rows rounded up to 64.
+//   COO / SCOO
+//         entries in a row-sorted view (a stable counting sort by row when the
+//         input is not already sorted) plus a tile table whose boundaries fall
+//         on row changes, so every row is summed by one thread in entry order.
+//   DENSE row-major, as stored.
+//
+// Every index the kernels will read is bounded here on the host (x_need,
+// y range); the launchers refuse vectors shorter than that.
+#pragma once
+
+#include <tbsla/cpp/Matrix.hpp>
+
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace tbsla {
+namespace hip {
+
+constexpr int kTileThreads = 256;
+constexpr int kTilePer = 8;
+constexpr int kCsrTileNnz = kTileThreads * kTilePer;  // entries staged in LDS per tile
+constexpr int kCsrTileRows = 256;                     // rows per tile at most
+
+struct DeviceMatrix {
+  cpp::Format format = cpp::FORMAT_CSR;
+  Dtype vdt = Dtype::F64;
+  bool noop = false;       // the reference returns before touching r
+  int rows = 0;            // CSR/ELL/DENSE: rows written (assign) / accumulated (DENSE)
+  long entries = 0;        // stored entries / slots read by one product
+  int f_row = 0, f_col = 0;
+  long x_need = 0;         // x must hold at least this many elements
+  long y_lo = 0, y_hi = 0; // COO/SCOO: rows touched are [y_lo, y_hi) before vect_incr
+  // CSR
+  DevBuf rowptr, colidx, vals, tiles;
+  int ntiles = 0, rp0 = 0;
+  // ELL (column-major)
+  int K = 0;
+  long ld = 0;
+  DevBuf ecols, evals;
+  // COO / SCOO sorted view
+  DevBuf srow, scol, sval, stiles;
+  int nstiles = 0;
+  // DENSE
+  int dcols = 0;
+  DevBuf dvals;
+  // count-stochastic CSR: 1/count of each local column
+  DevBuf colscale;
+  std::size_t algorithmic_bytes(std::size_t x_elem, std::size_t y_elem) const;
+};
+
+struct DeviceCache {
+  std::mutex mu;
+  std::unique_ptr<DeviceMatrix> mirrors[4];
+  DevBuf x_stage, y_stage;
+};
+
+// Build (or fetch) the mirror of `m` with values of type vdt.
+const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt);
+
+// Host tile tables (exposed for tests and the autotuner).
+std::vector<int> csr_tiles(const int *rowptr, int rows, int max_nnz, int max_rows);
+std::vector<int> segment_tiles(const int *row_sorted, long n, int max_nnz);
+
+}  // namespace hip
+}  // namespace tbsla

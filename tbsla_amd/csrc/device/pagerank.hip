@@ -1,0 +1,624 @@
+// PageRank iteration kernels and their host driver (see pagerank.hpp).
+//
+// One iteration of the CSR engine is two launches:
+//   pr_update_csr   tile SpMV of the current iterate fused with
+//                   b = beta*(A b_t) + tele, the block max of b, and a
+//                   last-block reduction of the maxima into the state.
+//   pr_normalize    b /= max, the L1 residual against b_t, the compensated
+//                   sum of b (next teleport term and stopping bound), and a
+//                   last-block finalize that updates the state: iteration
+//                   count, convergence, the next teleport term, and (mixed
+//                   precision) whether to move the anchor.
+// Every kernel returns at once when the state says the run has converged, so
+// the host launches `check_every` iterations between two reads of the state.
+#include <hip/hip_runtime.h>
+
+#include <tbsla/cpp/MatrixCSR.hpp>
+
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "kernels.hpp"
+#include "mirror.hpp"
+#include "pagerank.hpp"
+#include "tile_csr.hpp"
+
+namespace tbsla {
+namespace hip {
+namespace {
+
+struct PrState {
+  double beta, eps, bound_target, tele_scale;
+  double tele;          // teleport term used by the next update
+  double maxv;          // max of the last update
+  double err;           // reference residual sum|b_k - b_{k-1}| of the last iteration
+  double sum, sum_prev; // sum of the last / previous normalised iterate (compensated)
+  double bound, bound_prev, bound_anchor;
+  double anchor_tau, floor_eps;
+  int it, max_it, done, rule;
+  int need_anchor, anchors, anchor_it, mixed;
+  unsigned cnt_max, cnt_norm;
+};
+
+constexpr int kNormBlocks = 1024;
+
+// Neumaier-compensated running sum and its exact-error merge (TwoSum).
+struct CSum {
+  double s, c;
+};
+__device__ inline void csum_add(CSum &a, double v) {
+  const double t = a.s + v;
+  if (fabs(a.s) >= fabs(v))
+    a.c += (a.s - t) + v;
+  else
+    a.c += (v - t) + a.s;
+  a.s = t;
+}
+__device__ inline CSum csum_merge(CSum a, CSum b) {
+  const double t = a.s + b.s;
+  const double bp = t - a.s;
+  const double e = (a.s - (t - bp)) + (b.s - bp);
+  return CSum{t, a.c + b.c + e};
+}
+
+__device__ inline double atomic_read(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true in every thread of the last block to arrive; partial results
+// written before the call are visible to that block.
+__device__ inline bool last_block_arrive(unsigned *counter, int *flag_lds) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *flag_lds = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  return *flag_lds != 0;
+}
+
+__device__ inline CSum block_csum(CSum v, double *ps, double *pc) {
+  ps[threadIdx.x] = v.s;
+  pc[threadIdx.x] = v.c;
+  __syncthreads();
+  for (int s = kTileThreads / 2; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) {
+      CSum m = csum_merge(CSum{ps[threadIdx.x], pc[threadIdx.x]}, CSum{ps[threadIdx.x + s], pc[threadIdx.x + s]});
+      ps[threadIdx.x] = m.s;
+      pc[threadIdx.x] = m.c;
+    }
+    __syncthreads();
+  }
+  CSum r{ps[0], pc[0]};
+  __syncthreads();
+  return r;
+}
+
+// Per-row update epilogue: b = beta*(yhat + s) + tele (MIXED) or beta*s + tele.
+template <typename AT, bool MIXED, bool PERSONAL>
+struct UpdateRow {
+  double *b;
+  const double *yhat;
+  const unsigned char *member;
+  double beta, tele, lmax;
+  __device__ void operator()(int row, AT s) {
+#pragma clang fp contract(off)
+    const double v = MIXED ? yhat[row] + static_cast<double>(s) : static_cast<double>(s);
+    double bi = beta * v;
+    if (!PERSONAL || member[row]) bi = bi + tele;
+    b[row] = bi;
+    lmax = fmax(lmax, bi);
+  }
+};
+
+template <typename VT, typename XT, bool MIXED, bool PERSONAL>
+__global__ __launch_bounds__(kTileThreads) void pr_update_csr(const int *__restrict__ rowptr,
+                                                              const int *__restrict__ colidx,
+                                                              const VT *__restrict__ vals, const XT *__restrict__ x,
+                                                              const int *__restrict__ tiles, int rp0, int f_col,
+                                                              const double *__restrict__ yhat,
+                                                              const unsigned char *__restrict__ member,
+                                                              double *__restrict__ b, PrState *st,
+                                                              double *__restrict__ pmax) {
+  if (st->done) return;
+  using AT = typename dev::Val<VT>::A;
+  __shared__ dev::CsrTileLds<AT> lds;
+  __shared__ double red[kTileThreads];
+  __shared__ int last;
+  UpdateRow<AT, MIXED, PERSONAL> epi{b, yhat, member, st->beta, st->tele, -INFINITY};
+  dev::csr_tile_rows<VT, XT>(rowptr, colidx, vals, x, tiles, rp0, f_col, lds, epi);
+  const double bm = dev::block_tree_max<double>(epi.lmax, red);
+  if (threadIdx.x == 0) pmax[blockIdx.x] = bm;
+  if (last_block_arrive(&st->cnt_max, &last)) {
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) m = fmax(m, atomic_read(pmax + i));
+    m = dev::block_tree_max<double>(m, red);
+    if (threadIdx.x == 0) {
+      st->maxv = m;
+      st->cnt_max = 0;
+    }
+  }
+}
+
+// Value-free update for count-stochastic CSR: x is z = colscale .* b_t.
+template <bool PERSONAL>
+__global__ __launch_bounds__(kTileThreads) void pr_update_pattern(const int *__restrict__ rowptr,
+                                                                  const int *__restrict__ colidx,
+                                                                  const double *__restrict__ z,
+                                                                  const int *__restrict__ tiles, int rp0, int f_col,
+                                                                  const unsigned char *__restrict__ member,
+                                                                  double *__restrict__ b, PrState *st,
+                                                                  double *__restrict__ pmax) {
+  if (st->done) return;
+  __shared__ dev::CsrTileLds<double> lds;
+  __shared__ double red[kTileThreads];
+  __shared__ int last;
+  UpdateRow<double, false, PERSONAL> epi{b, nullptr, member, st->beta, st->tele, -INFINITY};
+  dev::csr_pattern_tile_rows(rowptr, colidx, z, tiles, rp0, f_col, lds, epi);
+  const double bm = dev::block_tree_max<double>(epi.lmax, red);
+  if (threadIdx.x == 0) pmax[blockIdx.x] = bm;
+  if (last_block_arrive(&st->cnt_max, &last)) {
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) m = fmax(m, atomic_read(pmax + i));
+    m = dev::block_tree_max<double>(m, red);
+    if (threadIdx.x == 0) {
+      st->maxv = m;
+      st->cnt_max = 0;
+    }
+  }
+}
+
+// Update for any format after a plain product y = A b_t (generic engine).
+template <bool PERSONAL>
+__global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict__ b, long n,
+                                                              const unsigned char *__restrict__ member, PrState *st,
+                                                              double *__restrict__ pmax) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  __shared__ double red[kTileThreads];
+  __shared__ int last;
+  const double beta = st->beta, tele = st->tele;
+  double lmax = -INFINITY;
+  for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kTileThreads) {
+    double bi = beta * b[i];
+    if (!PERSONAL || member[i]) bi = bi + tele;
+    b[i] = bi;
+    lmax = fmax(lmax, bi);
+  }
+  const double bm = dev::block_tree_max<double>(lmax, red);
+  if (threadIdx.x == 0) pmax[blockIdx.x] = bm;
+  if (last_block_arrive(&st->cnt_max, &last)) {
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) m = fmax(m, atomic_read(pmax + i));
+    m = dev::block_tree_max<double>(m, red);
+    if (threadIdx.x == 0) {
+      st->maxv = m;
+      st->cnt_max = 0;
+    }
+  }
+}
+
+// Decision logic of the last block of pr_normalize (one thread).
+__device__ inline void finalize_step(PrState *st, double err, double S) {
+  st->it += 1;
+  st->err = err;
+  st->sum_prev = st->sum;
+  st->sum = S;
+  const double bound = err / S + fabs(st->sum_prev / S - 1.0);
+  st->bound_prev = st->bound;
+  st->bound = bound;
+  bool converged = st->rule == 0 ? (err < st->eps) : (bound <= st->bound_target);
+  int anchor = 0;
+  if (st->mixed) {
+    const double floor = st->floor_eps * st->bound_anchor * 10.0;
+    if (converged && floor > st->bound_target) {
+      converged = false;
+      anchor = 1;
+    }
+    if (!converged) {
+      if (bound <= st->anchor_tau * st->bound_anchor) anchor = 1;
+      if (st->it - st->anchor_it >= 3 && bound > 0.9 * st->bound_prev) anchor = 1;
+    }
+    if (anchor) {
+      st->bound_anchor = bound;
+      st->anchor_it = st->it;
+      st->anchors += 1;
+    }
+  }
+  st->need_anchor = anchor;
+  st->done = (converged || st->it >= st->max_it) ? 1 : 0;
+  st->tele = S * st->tele_scale;
+}
+
+// MODE 0: FP64, b normalised in place, previous iterate `prev`.
+// MODE 1: mixed, previous iterate xhat + dold, writes dnew = b - xhat (FP32).
+// MODE 2: FP64 + z = colscale .* b for the next value-free product.
+template <int MODE>
+__global__ __launch_bounds__(kTileThreads) void pr_normalize(double *__restrict__ b, const double *__restrict__ prev,
+                                                             const double *__restrict__ xhat,
+                                                             const float *__restrict__ dold, float *__restrict__ dnew,
+                                                             double *__restrict__ z,
+                                                             const double *__restrict__ colscale, long n, PrState *st,
+                                                             double *__restrict__ perr, double *__restrict__ ps,
+                                                             double *__restrict__ pc) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  __shared__ double red_s[kTileThreads], red_c[kTileThreads];
+  __shared__ int last;
+  const double m = st->maxv;
+  double err = 0;
+  CSum cs{0, 0};
+  for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kTileThreads) {
+    const double bi = b[i] / m;
+    const double pi = MODE == 1 ? xhat[i] + static_cast<double>(dold[i]) : prev[i];
+    err += fabs(bi - pi);
+    csum_add(cs, bi);
+    if (MODE == 1) {
+      dnew[i] = static_cast<float>(bi - xhat[i]);
+    } else {
+      b[i] = bi;
+      if (MODE == 2) z[i] = colscale[i] * bi;
+    }
+  }
+  const double berr = dev::block_tree_sum<double>(err, red_s);
+  const CSum bs = block_csum(cs, red_s, red_c);
+  if (threadIdx.x == 0) {
+    perr[blockIdx.x] = berr;
+    ps[blockIdx.x] = bs.s;
+    pc[blockIdx.x] = bs.c;
+  }
+  if (last_block_arrive(&st->cnt_norm, &last)) {
+    double e = 0;
+    CSum t{0, 0};
+    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) {
+      e += atomic_read(perr + i);
+      t = csum_merge(t, CSum{atomic_read(ps + i), atomic_read(pc + i)});
+    }
+    e = dev::block_tree_sum<double>(e, red_s);
+    t = block_csum(t, red_s, red_c);
+    if (threadIdx.x == 0) {
+      finalize_step(st, e, t.s + t.c);
+      st->cnt_norm = 0;
+    }
+  }
+}
+
+// Mixed precision: move the anchor to the current iterate when requested.
+__global__ __launch_bounds__(256) void pr_anchor_move(double *__restrict__ xhat, float *__restrict__ delta,
+                                                      double *__restrict__ z, const double *__restrict__ colscale,
+                                                      long n, const PrState *st) {
+  if (st->done || !st->need_anchor) return;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) {
+    const double v = xhat[i] + static_cast<double>(delta[i]);
+    xhat[i] = v;
+    delta[i] = 0.0f;
+    if (z) z[i] = colscale[i] * v;
+  }
+}
+
+template <typename YT>
+struct StoreRowD {
+  YT *y;
+  template <typename AT>
+  __device__ void operator()(int row, AT s) {
+    y[row] = static_cast<YT>(s);
+  }
+};
+
+// Mixed precision: the FP64 anchor product yhat = A64 xhat (values or value-free).
+__global__ __launch_bounds__(kTileThreads) void pr_anchor_spmv(const int *__restrict__ rowptr,
+                                                               const int *__restrict__ colidx,
+                                                               const double *__restrict__ vals,
+                                                               const double *__restrict__ xhat,
+                                                               const int *__restrict__ tiles, int rp0, int f_col,
+                                                               double *__restrict__ yhat, const PrState *st) {
+  if (st->done || !st->need_anchor) return;
+  __shared__ dev::CsrTileLds<double> lds;
+  StoreRowD<double> epi{yhat};
+  dev::csr_tile_rows<double, double>(rowptr, colidx, vals, xhat, tiles, rp0, f_col, lds, epi);
+}
+__global__ __launch_bounds__(kTileThreads) void pr_anchor_spmv_pattern(const int *__restrict__ rowptr,
+                                                                       const int *__restrict__ colidx,
+                                                                       const double *__restrict__ z,
+                                                                       const int *__restrict__ tiles, int rp0,
+                                                                       int f_col, double *__restrict__ yhat,
+                                                                       const PrState *st) {
+  if (st->done || !st->need_anchor) return;
+  __shared__ dev::CsrTileLds<double> lds;
+  StoreRowD<double> epi{yhat};
+  dev::csr_pattern_tile_rows(rowptr, colidx, z, tiles, rp0, f_col, lds, epi);
+}
+
+// v = a (+ d) ; out = v / sum(v) with a compensated sum (two passes).
+__global__ __launch_bounds__(kTileThreads) void pr_sum_kernel(const double *__restrict__ a,
+                                                              const float *__restrict__ d, long n,
+                                                              double *__restrict__ ps, double *__restrict__ pc,
+                                                              double *__restrict__ total, unsigned *counter) {
+#pragma clang fp contract(off)
+  __shared__ double red_s[kTileThreads], red_c[kTileThreads];
+  __shared__ int last;
+  CSum cs{0, 0};
+  for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kTileThreads)
+    csum_add(cs, d ? a[i] + static_cast<double>(d[i]) : a[i]);
+  const CSum bs = block_csum(cs, red_s, red_c);
+  if (threadIdx.x == 0) {
+    ps[blockIdx.x] = bs.s;
+    pc[blockIdx.x] = bs.c;
+  }
+  if (last_block_arrive(counter, &last)) {
+    CSum t{0, 0};
+    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads)
+      t = csum_merge(t, CSum{atomic_read(ps + i), atomic_read(pc + i)});
+    t = block_csum(t, red_s, red_c);
+    if (threadIdx.x == 0) {
+      *total = t.s + t.c;
+      *counter = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void pr_scale_kernel(const double *__restrict__ a, const float *__restrict__ d,
+                                                       long n, const double *__restrict__ total,
+                                                       double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const double S = *total;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256)
+    out[i] = (d ? a[i] + static_cast<double>(d[i]) : a[i]) / S;
+}
+
+__global__ __launch_bounds__(256) void fill_f64(double *y, long n, double v) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) y[i] = v;
+}
+__global__ __launch_bounds__(256) void fill_f32(float *y, long n, float v) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) y[i] = v;
+}
+__global__ __launch_bounds__(256) void scale_into(const double *__restrict__ s, const double *__restrict__ x,
+                                                  double *__restrict__ z, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) z[i] = s[i] * x[i];
+}
+
+unsigned grid_for(long n) { return static_cast<unsigned>(std::max<long>(1, std::min<long>(kNormBlocks, (n + 255) / 256))); }
+
+struct Scratch {
+  DevBuf st, pmax, perr, ps, pc, total, counter, member;
+  DevBuf X0, X1, Z, yhat, xhat, D0, D1, colscale;
+};
+
+}  // namespace
+
+PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, double *out_dev, hipStream_t s) {
+  const long n = m.get_n_col();
+  if (m.get_n_row() != m.get_n_col() || m.get_ln_row() != m.get_n_row() || m.get_ln_col() != m.get_n_col())
+    throw std::invalid_argument("tbsla: page_rank needs a square matrix held in one block");
+  PageRankReport rep;
+  if (n == 0) return rep;
+  const bool mixed = p.precision == PrPrecision::F32 || p.precision == PrPrecision::BF16;
+  const bool csr = m.format() == cpp::FORMAT_CSR;
+  if ((mixed || p.precision == PrPrecision::PATTERN) && !csr)
+    throw std::invalid_argument("tbsla: mixed-precision and value-free PageRank need a CSR matrix");
+  if (p.precision == PrPrecision::PATTERN && !m.count_stochastic())
+    throw std::invalid_argument("tbsla: value-free PageRank needs values from fill_cqmat_stochastic");
+  if (mixed && p.rule != PrRule::SUMNORM)
+    throw std::invalid_argument("tbsla: mixed-precision PageRank uses the SUMNORM stopping rule");
+
+  const DeviceMatrix &d64 = mirror(m, Dtype::F64);
+  const DeviceMatrix *dlo = nullptr;
+  if (p.precision == PrPrecision::F32) dlo = &mirror(m, Dtype::F32);
+  if (p.precision == PrPrecision::BF16) dlo = &mirror(m, Dtype::BF16);
+  const bool fused = csr && !d64.noop && d64.rows == n;
+  if (csr && !d64.noop && d64.rows != n)
+    throw std::invalid_argument("tbsla: CSR rowptr does not cover every row");
+  if (d64.x_need > n) throw std::invalid_argument("tbsla: column index beyond n_col");
+
+  Scratch w;
+  const unsigned gN = grid_for(n);
+  const unsigned gT = fused ? static_cast<unsigned>(std::max(d64.ntiles, 1)) : gN;
+  w.st.alloc(sizeof(PrState));
+  w.pmax.alloc(sizeof(double) * std::max(gT, gN));
+  w.perr.alloc(sizeof(double) * gN);
+  w.ps.alloc(sizeof(double) * gN);
+  w.pc.alloc(sizeof(double) * gN);
+  w.total.alloc(sizeof(double));
+  w.counter.alloc(sizeof(unsigned));
+  w.X0.alloc(sizeof(double) * n);
+  w.X1.alloc(sizeof(double) * n);
+  unsigned char *member = nullptr;
+  const double b0 = p.personalized ? 0.25 : 1.0;
+  if (p.personalized) {
+    std::vector<unsigned char> mem(static_cast<std::size_t>(n), 0);
+    for (int k = 0; k < p.n_nodes; k++)
+      if (p.nodes[k] >= 0 && p.nodes[k] < n) mem[static_cast<std::size_t>(p.nodes[k])] = 1;
+    w.member.upload(mem.data(), mem.size(), s);
+    member = w.member.as<unsigned char>();
+  }
+  const bool pattern = p.precision == PrPrecision::PATTERN;
+  if (pattern || mixed) {
+    // 1/count per column, read back from the stored values (every value of
+    // column j equals fl(1/count_j) after fill_cqmat_stochastic)
+    const auto &csrm = dynamic_cast<const cpp::MatrixCSR &>(m);
+    std::vector<double> cs(static_cast<std::size_t>(n), 0.0);
+    const auto &ci = csrm.get_colidx();
+    const auto &va = csrm.get_values();
+    for (std::size_t k = 0; k < va.size(); k++) cs[static_cast<std::size_t>(ci[k])] = va[k];
+    w.colscale.upload(cs.data(), cs.size() * sizeof(double), s);
+  }
+  PrState h{};
+  h.beta = p.beta;
+  h.eps = p.epsilon;
+  h.bound_target = p.epsilon * (1.0 - p.beta) / p.beta;
+  h.tele_scale = p.personalized ? (1.0 - p.beta) / static_cast<double>(static_cast<std::size_t>(p.n_nodes))
+                                : (1.0 - p.beta) / static_cast<double>(n);
+  h.sum = b0 * static_cast<double>(n);
+  h.tele = h.sum * h.tele_scale;
+  h.bound = 2.0;
+  h.bound_prev = 2.0;
+  h.bound_anchor = 2.0;
+  h.max_it = p.max_iteration;
+  h.rule = static_cast<int>(p.rule);
+  h.mixed = mixed ? 1 : 0;
+  h.need_anchor = mixed ? 1 : 0;
+  h.anchors = mixed ? 1 : 0;
+  h.floor_eps = p.precision == PrPrecision::BF16 ? 4e-3 : 6e-8;
+  h.anchor_tau = p.anchor_tau > 0 ? p.anchor_tau : (p.precision == PrPrecision::BF16 ? 5e-2 : 3e-6);
+  h.done = p.max_iteration <= 0 ? 1 : 0;
+  TBSLA_HIP_CHECK(hipMemcpyAsync(w.st.get(), &h, sizeof(h), hipMemcpyHostToDevice, s));
+  PrState *st = w.st.as<PrState>();
+  double *X[2] = {w.X0.as<double>(), w.X1.as<double>()};
+  hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[0], n, b0);
+  float *D[2] = {nullptr, nullptr};
+  if (mixed) {
+    w.xhat.alloc(sizeof(double) * n);
+    w.yhat.alloc(sizeof(double) * n);
+    w.D0.alloc(sizeof(float) * n);
+    w.D1.alloc(sizeof(float) * n);
+    D[0] = w.D0.as<float>();
+    D[1] = w.D1.as<float>();
+    hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.xhat.as<double>(), n, b0);
+    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[0], n, 0.0f);
+    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[1], n, 0.0f);
+  }
+  if (pattern || (mixed && m.count_stochastic())) {
+    w.Z.alloc(sizeof(double) * n);
+    hipLaunchKernelGGL(scale_into, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), X[0], w.Z.as<double>(), n);
+  }
+  TBSLA_HIP_CHECK(hipGetLastError());
+
+  hipEvent_t e0, e1;
+  TBSLA_HIP_CHECK(hipEventCreate(&e0));
+  TBSLA_HIP_CHECK(hipEventCreate(&e1));
+  TBSLA_HIP_CHECK(hipEventRecord(e0, s));
+  const int check = fused ? std::max(1, p.check_every) : 1;
+  PrState hs{};
+  int launched = 0;
+  const bool mixed_pattern = mixed && m.count_stochastic();
+  while (launched < p.max_iteration) {
+    const int batch = std::min(check, p.max_iteration - launched);
+    for (int k = 0; k < batch; k++) {
+      const int it = launched + k;
+      double *xin = X[it & 1], *xout = X[(it + 1) & 1];
+      if (fused) {
+        if (mixed) {
+          float *dcur = D[it & 1], *dnext = D[(it + 1) & 1];
+          hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, w.xhat.as<double>(), dcur,
+                             mixed_pattern ? w.Z.as<double>() : nullptr, w.colscale.as<double>(), n, st);
+          if (mixed_pattern)
+            hipLaunchKernelGGL(pr_anchor_spmv_pattern, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
+                               d64.colidx.as<int>(), w.Z.as<double>(), d64.tiles.as<int>(), d64.rp0, d64.f_col,
+                               w.yhat.as<double>(), st);
+          else
+            hipLaunchKernelGGL(pr_anchor_spmv, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
+                               d64.colidx.as<int>(), d64.vals.as<double>(), w.xhat.as<double>(), d64.tiles.as<int>(),
+                               d64.rp0, d64.f_col, w.yhat.as<double>(), st);
+          if (p.precision == PrPrecision::F32) {
+            if (p.personalized)
+              hipLaunchKernelGGL((pr_update_csr<float, float, true, true>), dim3(gT), dim3(kTileThreads), 0, s,
+                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<float>(), dcur,
+                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
+                                 w.pmax.as<double>());
+            else
+              hipLaunchKernelGGL((pr_update_csr<float, float, true, false>), dim3(gT), dim3(kTileThreads), 0, s,
+                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<float>(), dcur,
+                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
+                                 w.pmax.as<double>());
+          } else {
+            if (p.personalized)
+              hipLaunchKernelGGL((pr_update_csr<dev::bf16, float, true, true>), dim3(gT), dim3(kTileThreads), 0, s,
+                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<dev::bf16>(), dcur,
+                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
+                                 w.pmax.as<double>());
+            else
+              hipLaunchKernelGGL((pr_update_csr<dev::bf16, float, true, false>), dim3(gT), dim3(kTileThreads), 0, s,
+                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<dev::bf16>(), dcur,
+                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
+                                 w.pmax.as<double>());
+          }
+          hipLaunchKernelGGL(pr_normalize<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr,
+                             w.xhat.as<double>(), dcur, dnext, nullptr, nullptr, n, st, w.perr.as<double>(),
+                             w.ps.as<double>(), w.pc.as<double>());
+        } else if (pattern) {
+          if (p.personalized)
+            hipLaunchKernelGGL(pr_update_pattern<true>, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
+                               d64.colidx.as<int>(), w.Z.as<double>(), d64.tiles.as<int>(), d64.rp0, d64.f_col,
+                               member, xout, st, w.pmax.as<double>());
+          else
+            hipLaunchKernelGGL(pr_update_pattern<false>, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
+                               d64.colidx.as<int>(), w.Z.as<double>(), d64.tiles.as<int>(), d64.rp0, d64.f_col,
+                               member, xout, st, w.pmax.as<double>());
+          hipLaunchKernelGGL(pr_normalize<2>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
+                             nullptr, w.Z.as<double>(), w.colscale.as<double>(), n, st, w.perr.as<double>(),
+                             w.ps.as<double>(), w.pc.as<double>());
+        } else {
+          if (p.personalized)
+            hipLaunchKernelGGL((pr_update_csr<double, double, false, true>), dim3(gT), dim3(kTileThreads), 0, s,
+                               d64.rowptr.as<int>(), d64.colidx.as<int>(), d64.vals.as<double>(), xin,
+                               d64.tiles.as<int>(), d64.rp0, d64.f_col, nullptr, member, xout, st,
+                               w.pmax.as<double>());
+          else
+            hipLaunchKernelGGL((pr_update_csr<double, double, false, false>), dim3(gT), dim3(kTileThreads), 0, s,
+                               d64.rowptr.as<int>(), d64.colidx.as<int>(), d64.vals.as<double>(), xin,
+                               d64.tiles.as<int>(), d64.rp0, d64.f_col, nullptr, member, xout, st,
+                               w.pmax.as<double>());
+          hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
+                             w.pc.as<double>());
+        }
+      } else {
+        // any format: y = 0 (accumulating formats), y += A b_t, update, normalise
+        hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, xout, n, 0.0);
+        if (!d64.noop) launch_spmv(d64, xin, Dtype::F64, xout, Dtype::F64, 0, s);
+        if (p.personalized)
+          hipLaunchKernelGGL(pr_update_vec<true>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st,
+                             w.pmax.as<double>());
+        else
+          hipLaunchKernelGGL(pr_update_vec<false>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st,
+                             w.pmax.as<double>());
+        hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
+                           w.pc.as<double>());
+      }
+    }
+    TBSLA_HIP_CHECK(hipGetLastError());
+    launched += batch;
+    TBSLA_HIP_CHECK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+    TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+    if (hs.done) break;
+  }
+  TBSLA_HIP_CHECK(hipEventRecord(e1, s));
+  const int it = hs.it;
+  // final sum normalisation (Matrix.cpp:52-59), compensated
+  const double *a = mixed ? w.xhat.as<double>() : X[it & 1];
+  const float *dd = mixed ? D[it & 1] : nullptr;
+  hipLaunchKernelGGL(pr_sum_kernel, dim3(gN), dim3(kTileThreads), 0, s, a, dd, n, w.ps.as<double>(),
+                     w.pc.as<double>(), w.total.as<double>(), w.counter.as<unsigned>());
+  hipLaunchKernelGGL(pr_scale_kernel, dim3(gN), dim3(256), 0, s, a, dd, n, w.total.as<double>(), out_dev);
+  TBSLA_HIP_CHECK(hipGetLastError());
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  TBSLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  rep.iterations = it;
+  rep.anchors = hs.anchors;
+  rep.last_error = hs.err;
+  rep.bound = hs.bound;
+  rep.seconds = ms * 1e-3;
+  return rep;
+}
+
+PageRankReport page_rank(const cpp::Matrix &m, const PageRankParams &p, double *out) {
+  const long n = m.get_n_col();
+  DevBuf o;
+  o.alloc(sizeof(double) * static_cast<std::size_t>(std::max(1L, n)));
+  PageRankReport r = page_rank_device(m, p, o.as<double>(), nullptr);
+  if (n > 0) TBSLA_HIP_CHECK(hipMemcpy(out, o.get(), sizeof(double) * static_cast<std::size_t>(n), hipMemcpyDeviceToHost));
+  return r;
+}
+
+}  // namespace hip
+}  // namespace tbsla

@@ -1,0 +1,280 @@
+// SpMV kernels for gfx950 (MI355X, CDNA4, 64-wide wavefronts).
+//
+// csr_tile_kernel   one workgroup per row tile (<= 2048 entries, <= 256 rows).
+//                   Entries are read with coalesced non-temporal loads (8 per
+//                   thread in flight), multiplied by the gathered x, and staged
+//                   in LDS; each row is then summed by one thread in entry
+//                   order.  With FP64 this reproduces the reference loop
+//                   MatrixCSR.cpp:133-139 bit for bit (product rounded, then a
+//                   left-to-right sum starting from 0).
+// ell_kernel        one thread per row over the column-major slots, 4 slots
+//                   per step; same order as MatrixELL.cpp:105-113 (contraction
+//                   off, so bit-identical in FP64).
+// seg_kernel        COO/SCOO: one workgroup per segment-aligned tile of the
+//                   row-sorted view; segment heads found with a block scan,
+//                   each row accumulated into y in entry order
+//                   (MatrixCOO.cpp:113-115 / MatrixSCOO.cpp:120-122 with one
+//                   thread).
+// dense_kernel      one wavefront per row, butterfly reduction
+//                   (MatrixDENSE.cpp:67-72, different summation order).
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "tile_csr.hpp"
+
+namespace tbsla {
+namespace hip {
+namespace {
+
+using dev::bf16;
+using dev::block_tree_sum;
+using dev::nt_load;
+using dev::Val;
+
+// Exclusive scan of one int per thread over a block of kTileThreads threads.
+__device__ inline int block_exclusive_scan(int v, int &total, int *wsum) {
+  constexpr int kWaves = kTileThreads / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int woff = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kWaves; k++) {
+    const int s = wsum[k];
+    if (k < w) woff += s;
+    tot += s;
+  }
+  total = tot;
+  __syncthreads();
+  return woff + inc - v;
+}
+
+template <typename YT>
+struct StoreRow {
+  YT *y;
+  template <typename AT>
+  __device__ void operator()(int row, AT s) const {
+    y[row] = static_cast<YT>(s);
+  }
+};
+
+template <typename VT, typename XT, typename YT>
+__global__ __launch_bounds__(kTileThreads) void csr_tile_kernel(const int *__restrict__ rowptr,
+                                                                const int *__restrict__ colidx,
+                                                                const VT *__restrict__ vals,
+                                                                const XT *__restrict__ x, YT *__restrict__ y,
+                                                                const int *__restrict__ tiles, int rp0,
+                                                                int f_col) {
+  __shared__ dev::CsrTileLds<typename Val<VT>::A> lds;
+  StoreRow<YT> epi{y};
+  dev::csr_tile_rows<VT, XT>(rowptr, colidx, vals, x, tiles, rp0, f_col, lds, epi);
+}
+
+template <typename VT, typename XT, typename YT>
+__global__ __launch_bounds__(256) void ell_kernel(const int *__restrict__ cols, const VT *__restrict__ vals,
+                                                  const XT *__restrict__ x, YT *__restrict__ y, int rows, int K,
+                                                  long ld, int f_col) {
+#pragma clang fp contract(off)
+  using AT = typename Val<VT>::A;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows) return;
+  AT tmp = 0;
+  int j = 0;
+  for (; j + 4 <= K; j += 4) {
+    int c[4];
+    AT v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      c[q] = nt_load(cols, (j + q) * ld + i) - f_col;
+      if (c[q] < 0) c[q] = 0;
+      v[q] = Val<VT>::get(vals, (j + q) * ld + i);
+    }
+    AT xv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) xv[q] = static_cast<AT>(x[c[q]]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) tmp += v[q] * xv[q];
+  }
+  for (; j < K; j++) {
+    int cc = nt_load(cols, j * ld + i) - f_col;
+    if (cc < 0) cc = 0;
+    tmp += Val<VT>::get(vals, j * ld + i) * static_cast<AT>(x[cc]);
+  }
+  y[i] = static_cast<YT>(tmp);
+}
+
+template <typename VT, typename XT, typename YT>
+__global__ __launch_bounds__(kTileThreads) void seg_kernel(const int *__restrict__ row, const int *__restrict__ col,
+                                                           const VT *__restrict__ vals, const XT *__restrict__ x,
+                                                           YT *__restrict__ y, const long *__restrict__ tiles,
+                                                           int x_off, int y_off) {
+  using AT = typename Val<VT>::A;
+  __shared__ AT prod[kCsrTileNnz];
+  __shared__ int key[kCsrTileNnz];
+  __shared__ int seg[kCsrTileNnz + 1];
+  __shared__ int wsum[kTileThreads / 64];
+  const int tid = threadIdx.x;
+  const long e0 = tiles[blockIdx.x];
+  const int ne = static_cast<int>(tiles[blockIdx.x + 1] - e0);
+  if (ne <= kCsrTileNnz) {
+    int c[kTilePer];
+    AT v[kTilePer];
+#pragma unroll
+    for (int k = 0; k < kTilePer; k++) {
+      const int e = k * kTileThreads + tid;
+      if (e < ne) {
+        key[e] = nt_load(row, e0 + e);
+        c[k] = nt_load(col, e0 + e) + x_off;
+        v[k] = Val<VT>::get(vals, e0 + e);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kTilePer; k++) {
+      const int e = k * kTileThreads + tid;
+      if (e < ne) prod[e] = v[k] * static_cast<AT>(x[c[k]]);
+    }
+    __syncthreads();
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < kTilePer; j++) {
+      const int e = tid * kTilePer + j;
+      if (e < ne && (e == 0 || key[e] != key[e - 1])) cnt++;
+    }
+    int nseg = 0;
+    int o = block_exclusive_scan(cnt, nseg, wsum);
+#pragma unroll
+    for (int j = 0; j < kTilePer; j++) {
+      const int e = tid * kTilePer + j;
+      if (e < ne && (e == 0 || key[e] != key[e - 1])) seg[o++] = e;
+    }
+    if (tid == 0) seg[nseg] = ne;
+    __syncthreads();
+    for (int s = tid; s < nseg; s += kTileThreads) {
+      const int a = seg[s], b = seg[s + 1];
+      const long r = static_cast<long>(key[a]) + y_off;
+      AT acc = static_cast<AT>(y[r]);
+      for (int e = a; e < b; e++) acc += prod[e];
+      y[r] = static_cast<YT>(acc);
+    }
+    return;
+  }
+  // One row longer than a tile: chunked tree sums, then y[r] += total.
+  AT total = 0;
+  for (int base = 0; base < ne; base += kCsrTileNnz) {
+    const int cnt = min(kCsrTileNnz, ne - base);
+    AT acc = 0;
+#pragma unroll
+    for (int k = 0; k < kTilePer; k++) {
+      const int e = k * kTileThreads + tid;
+      if (e < cnt) acc += Val<VT>::get(vals, e0 + base + e) * static_cast<AT>(x[nt_load(col, e0 + base + e) + x_off]);
+    }
+    total += block_tree_sum<AT>(acc, prod);
+  }
+  if (tid == 0) {
+    const long r = static_cast<long>(row[e0]) + y_off;
+    y[r] = static_cast<YT>(static_cast<AT>(y[r]) + total);
+  }
+}
+
+template <typename XT, typename YT>
+__global__ __launch_bounds__(256) void dense_kernel(const double *__restrict__ A, const XT *__restrict__ x,
+                                                    YT *__restrict__ y, int rows, int cols, long ld) {
+  const int w = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (w >= rows) return;
+  const double *a = A + static_cast<long>(w) * ld;
+  double acc = 0;
+  for (int j = lane; j < cols; j += 64) acc += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) y[w] = static_cast<YT>(static_cast<double>(y[w]) + acc);
+}
+
+template <typename T>
+__global__ void fill_kernel(T *y, long n, T v) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) y[i] = v;
+}
+
+inline unsigned blocks_for(long n, int per_block) { return static_cast<unsigned>((n + per_block - 1) / per_block); }
+
+template <typename VT, typename XT, typename YT>
+void run_typed(const DeviceMatrix &d, const XT *x, YT *y, int vect_incr, hipStream_t s) {
+  const VT *vals = nullptr;
+  switch (d.format) {
+    case cpp::FORMAT_CSR:
+      vals = d.vals.as<VT>();
+      if (d.ntiles > 0)
+        hipLaunchKernelGGL((csr_tile_kernel<VT, XT, YT>), dim3(d.ntiles), dim3(kTileThreads), 0, s,
+                           d.rowptr.as<int>(), d.colidx.as<int>(), vals, x, y, d.tiles.as<int>(), d.rp0, d.f_col);
+      break;
+    case cpp::FORMAT_ELL:
+      if (d.rows > 0)
+        hipLaunchKernelGGL((ell_kernel<VT, XT, YT>), dim3(blocks_for(d.rows, 256)), dim3(256), 0, s,
+                           d.ecols.as<int>(), d.evals.as<VT>(), x, y, d.rows, d.K, d.ld, d.f_col);
+      break;
+    case cpp::FORMAT_COO:
+    case cpp::FORMAT_SCOO: {
+      const int x_off = d.format == cpp::FORMAT_SCOO ? -d.f_col : 0;
+      const int y_off = vect_incr - (d.format == cpp::FORMAT_SCOO ? d.f_row : 0);
+      if (d.nstiles > 0)
+        hipLaunchKernelGGL((seg_kernel<VT, XT, YT>), dim3(d.nstiles), dim3(kTileThreads), 0, s, d.srow.as<int>(),
+                           d.scol.as<int>(), d.sval.as<VT>(), x, y, d.stiles.as<long>(), x_off, y_off);
+      break;
+    }
+    case cpp::FORMAT_DENSE:
+      if (d.rows > 0)
+        hipLaunchKernelGGL((dense_kernel<XT, YT>), dim3(blocks_for(static_cast<long>(d.rows) * 64, 256)),
+                           dim3(256), 0, s, d.dvals.as<double>(), x, y, d.rows, d.dcols, static_cast<long>(d.dcols));
+      break;
+  }
+  TBSLA_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_spmv(const DeviceMatrix &d, const void *x, Dtype xdt, void *y, Dtype ydt, int vect_incr,
+                 hipStream_t s) {
+  if (d.noop) return;
+  if (d.vdt == Dtype::F64 && xdt == Dtype::F64 && ydt == Dtype::F64) {
+    run_typed<double, double, double>(d, static_cast<const double *>(x), static_cast<double *>(y), vect_incr, s);
+  } else if (d.vdt == Dtype::F32 && xdt == Dtype::F32 && ydt == Dtype::F32 && d.format != cpp::FORMAT_DENSE) {
+    run_typed<float, float, float>(d, static_cast<const float *>(x), static_cast<float *>(y), vect_incr, s);
+  } else if (d.vdt == Dtype::BF16 && xdt == Dtype::F32 && ydt == Dtype::F32 && d.format != cpp::FORMAT_DENSE) {
+    run_typed<bf16, float, float>(d, static_cast<const float *>(x), static_cast<float *>(y), vect_incr, s);
+  } else {
+    throw std::invalid_argument("tbsla: unsupported value/vector type combination for this format");
+  }
+}
+
+void launch_fill(void *y, Dtype ydt, long n, double value, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned g = static_cast<unsigned>(std::min<long>((n + 255) / 256, 65536));
+  if (ydt == Dtype::F64)
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(g), dim3(256), 0, s, static_cast<double *>(y), n, value);
+  else if (ydt == Dtype::F32)
+    hipLaunchKernelGGL(fill_kernel<float>, dim3(g), dim3(256), 0, s, static_cast<float *>(y), n,
+                       static_cast<float>(value));
+  else
+    throw std::invalid_argument("tbsla: fill supports F64/F32 vectors");
+  TBSLA_HIP_CHECK(hipGetLastError());
+}
+
+const char *spmv_kernel_name(cpp::Format f) {
+  switch (f) {
+    case cpp::FORMAT_CSR: return "csr_tile_kernel";
+    case cpp::FORMAT_ELL: return "ell_kernel";
+    case cpp::FORMAT_COO:
+    case cpp::FORMAT_SCOO: return "seg_kernel";
+    case cpp::FORMAT_DENSE: return "dense_kernel";
+  }
+  return "";
+}
+
+}  // namespace hip
+}  // namespace tbsla

@@ -1,0 +1,101 @@
+"""GPU PageRank parity (Matrix.cpp:13-114 semantics) against the reference
+fixtures and the oracle, and the precision variants against the FP64 fixed
+point."""
+import numpy as np
+import pytest
+
+from conftest import FMTS
+
+pytestmark = pytest.mark.gpu
+
+
+def test_page_rank_small_matches_reference(tb, digests, golden, gpu):
+    """Reference vectors (compiled reference, fixtures): same iteration count,
+    L1 distance <= 1e-13 (the GPU sums teleport/residual in a different
+    order than the reference's sequential loops)."""
+    d = digests["pagerank_small"]
+    for (n, c, q, s) in [(30, 8, 0.5, 3), (200, 8, 0.5, 7), (1000, 8, 0.5, 7), (2000, 12, 0.3, 5),
+                         (1500, 16, 0.1, 1)]:
+        for fmt in FMTS:
+            m = tb.FORMATS[fmt]().fill_cqmat_stochastic(n, n, c, q, s)
+            for (eps, beta, it) in [(1e-10, 0.85, 500), (0.85, 0.01, 10)]:
+                pr, its = m.page_rank(eps, beta, it)
+                key = f"pr/{n}c{c}q{q}s{s}/{fmt}/eps{eps}b{beta}"
+                assert abs(its - d[key]) <= 1, (key, its, d[key])
+                assert np.abs(pr - golden[key]).sum() <= 1e-13, (key, np.abs(pr - golden[key]).sum())
+                nodes = np.array([1, n // 3, n - 1], dtype=np.int32)
+                pr, its = m.personalized_page_rank(eps, beta, it, nodes)
+                key = f"ppr/{n}c{c}q{q}s{s}/{fmt}/eps{eps}b{beta}"
+                assert abs(its - d[key]) <= 1, (key, its, d[key])
+                assert np.abs(pr - golden[key]).sum() <= 1e-13, key
+
+
+@pytest.fixture(scope="module")
+def graph_2_20(tb, oracle):
+    n = 1 << 20
+    p = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 8, 0.5, 7)
+    o = oracle.new("CSR").fill_cqmat_stochastic(n, n, 8, 0.5, 7)
+    pr_ref, its = o.page_rank(1e-10, 0.85, 10000)
+    # the FP64 fixed point: the oracle iteration run until its residual stops falling
+    pstar, _ = o.page_rank(0.0, 0.85, 120)
+    return p, pr_ref, its, pstar
+
+
+def test_page_rank_2_20_reference_rule(graph_2_20, digests, gpu):
+    """n=2^20, c=8, q=0.5, seed 7: the reference stops after 44 iterations."""
+    p, pr_ref, its_ref, pstar = graph_2_20
+    assert its_ref == digests["pagerank_big"]["pr/2^20c8q0.5s7/CSR/eps1e-10b0.85"]["iterations"]
+    pr, its = p.page_rank(1e-10, 0.85, 10000)
+    assert abs(its - its_ref) <= 1
+    assert np.abs(pr - pr_ref).sum() <= 1e-12
+    assert abs(pr.sum() - 1) < 1e-14
+
+
+@pytest.mark.parametrize("precision", ["f64", "pattern", "f32", "bf16"])
+def test_page_rank_precisions_reach_1e10(graph_2_20, gpu, precision):
+    """Every precision variant stops on the sum-normalised bound and lands
+    within 1e-10 L1 of the FP64 fixed point (BF16/FP32 values need the FP64
+    anchor product to get there)."""
+    import torch
+    p, _, _, pstar = graph_2_20
+    out = torch.zeros(p.get_n_col(), dtype=torch.float64, device=gpu)
+    rep = p.page_rank_device(out, epsilon=1e-10, beta=0.85, max_iteration=500, precision=precision,
+                             rule="sumnorm")
+    torch.cuda.synchronize()
+    err = np.abs(out.cpu().numpy() - pstar).sum()
+    assert err <= 1e-10, (precision, err, rep)
+    assert rep["iterations"] < 500
+
+
+def test_pattern_path_is_bit_identical(graph_2_20, gpu):
+    """Value-free products equal the FP64-value products bit for bit, so the
+    whole run is identical."""
+    import torch
+    p = graph_2_20[0]
+    a = torch.zeros(p.get_n_col(), dtype=torch.float64, device=gpu)
+    b = torch.zeros_like(a)
+    ra = p.page_rank_device(a, epsilon=1e-10, max_iteration=60, precision="f64", rule="reference")
+    rb = p.page_rank_device(b, epsilon=1e-10, max_iteration=60, precision="pattern", rule="reference")
+    torch.cuda.synchronize()
+    assert ra["iterations"] == rb["iterations"]
+    assert torch.equal(a, b)
+
+
+def test_personalized_device_matches_oracle(tb, oracle, gpu):
+    import torch
+    n = 50000
+    p = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 12, 0.3, 1001)
+    o = oracle.new("CSR").fill_cqmat_stochastic(n, n, 12, 0.3, 1001)
+    nodes = np.array([17], dtype=np.int32)
+    ref, its = o.personalized_page_rank(1e-12, 0.85, 1000, nodes)
+    out = torch.zeros(n, dtype=torch.float64, device=gpu)
+    rep = p.page_rank_device(out, epsilon=1e-12, max_iteration=1000, nodes=nodes)
+    torch.cuda.synchronize()
+    assert abs(rep["iterations"] - its) <= 1
+    assert np.abs(out.cpu().numpy() - ref).sum() <= 1e-12
+
+
+def test_page_rank_zero_iterations(tb, gpu):
+    m = tb.MatrixCSR().fill_cqmat_stochastic(100, 100, 4, 0.5, 2)
+    pr, its = m.page_rank(1e-10, 0.85, 0)
+    assert its == 0 and np.allclose(pr, 1.0 / 100)

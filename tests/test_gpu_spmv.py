@@ -1,0 +1,210 @@
+"""GPU SpMV parity (through the C ABI of libtbsla_hip.so) against the oracle.
+
+The reference tests are the model: the 5x5 known answer (tests/csr.cpp),
+the cdiag closed form over the reference shape grid (tests/test_spmv_cdiag.cpp),
+exact cross-format equality on cqmat matrices (tests/test_spmv_cqmat.cpp).
+FP64 products are compared bit for bit where the kernel keeps the reference
+summation order (CSR, ELL, COO, SCOO) and within 1e-14 relative, elementwise
+against sum|a||x|, for DENSE.
+"""
+import numpy as np
+import pytest
+
+from conftest import CDIAG_SHAPES, CQMAT_SHAPES, FMTS, grid_q
+
+pytestmark = pytest.mark.gpu
+
+EXACT = {"COO", "SCOO", "CSR", "ELL"}
+
+
+def close(fmt, y, ref, absy):
+    if fmt in EXACT:
+        assert np.array_equal(y, ref), (fmt, np.abs(y - ref).max())
+    else:
+        assert np.all(np.abs(y - ref) <= 1e-14 * absy + 1e-300), (fmt, np.abs(y - ref).max())
+
+
+def test_known_answer_5x5(tb, golden, gpu):
+    csr = tb.MatrixCSR(5, 5, golden["ka_values"], golden["ka_rowptr"], golden["ka_colidx"])
+    coo = tb.MatrixCOO(5, 5, golden["ka_values"], golden["ka_row"], golden["ka_colidx"])
+    assert np.array_equal(csr.spmv(golden["ka_x"]), golden["ka_expected"])
+    assert np.array_equal(coo.spmv(golden["ka_x"]), golden["ka_expected"])
+    for cls in (tb.MatrixSCOO, tb.MatrixCSR, tb.MatrixELL, tb.MatrixDENSE):
+        assert np.array_equal(cls(coo).spmv(golden["ka_x"]), golden["ka_expected"]), cls.NAME
+
+
+def test_cdiag_closed_form_all_formats(tb, oracle, gpu):
+    """tests/test_spmv_cdiag.cpp:41-105: direct fill and via COO, exact."""
+    for (nr, nc, c) in CDIAG_SHAPES:
+        x = np.arange(nc, dtype=np.float64)
+        expect = oracle.spmv_res_cdiag(nr, nc, c, x)
+        coo = tb.MatrixCOO().fill_cdiag(nr, nc, c)
+        for fmt in FMTS:
+            m = tb.FORMATS[fmt]().fill_cdiag(nr, nc, c)
+            assert np.array_equal(m.spmv(x), expect), (fmt, nr, nc, c)
+            if fmt != "COO":
+                assert np.array_equal(tb.FORMATS[fmt](coo).spmv(x), expect), (fmt, nr, nc, c)
+
+
+def test_cqmat_cross_format_exact(tb, oracle, gpu):
+    """tests/test_spmv_cqmat.cpp:19-151: every format, direct and converted,
+    equal to the COO product bit for bit (values 1, x = iota), and equal to
+    the oracle."""
+    for (nr, nc, c) in CQMAT_SHAPES[::2]:
+        for s in range(4):
+            for q in grid_q()[::2]:
+                x = np.arange(nc, dtype=np.float64)
+                coo = tb.MatrixCOO().fill_cqmat(nr, nc, c, q, s)
+                rc = coo.spmv(x)
+                assert np.array_equal(rc, oracle.new("COO").fill_cqmat(nr, nc, c, q, s).spmv(x))
+                for fmt in ["SCOO", "CSR", "ELL", "DENSE"]:
+                    assert np.array_equal(tb.FORMATS[fmt](coo).spmv(x), rc), (fmt, nr, nc, c, s, q)
+                    assert np.array_equal(tb.FORMATS[fmt]().fill_cqmat(nr, nc, c, q, s).spmv(x), rc), (fmt, nr, nc, c)
+
+
+def test_blocks_vect_incr_and_accumulate(tb, oracle, gpu):
+    """Block matrices (x local to the block), Ax accumulation for COO/SCOO/DENSE,
+    vect_incr for COO/SCOO, non-integer values: same bits as the oracle's
+    sequential loops (DENSE within 1e-14)."""
+    rng = np.random.default_rng(5)
+    for (n, c, q, s) in [(3000, 16, 0.1, 42), (2500, 9, 0.5, 3)]:
+        for blk in [(0, 0, 1, 1), (1, 1, 2, 3), (2, 0, 3, 2)]:
+            for fmt in FMTS:
+                p = tb.FORMATS[fmt]().fill_cqmat(n, n, c, q, s, *blk, value_seed=42)
+                o = oracle.new(fmt).fill_cqmat(n, n, c, q, s, *blk, value_seed=42)
+                info = p.info()
+                nx = n if fmt == "COO" else info["ln_col"]
+                x = oracle.uniform_vector(1, nx, first=0 if fmt == "COO" else info["f_col"])
+                r0 = rng.random(info["n_row"] + 7)
+                for vi in ([0, 3, 7] if fmt in ("COO", "SCOO") else [0]):
+                    y = p.Ax(r0, x, vi)
+                    yo = o.Ax(r0, x, vi)
+                    absm = np.abs(o.Ax(np.zeros_like(r0), np.abs(x), vi)) + np.abs(r0)
+                    close(fmt, y, yo, absm)
+                close(fmt, p.spmv(x), o.spmv(x), np.abs(o.spmv(np.abs(x))))
+
+
+def test_unsorted_coo_is_exact(tb, oracle, gpu):
+    """A COO whose rows are not sorted is viewed row-sorted on the device
+    (stable), so each row still sums in entry order."""
+    rng = np.random.default_rng(1)
+    n, nnz = 500, 6000
+    row = rng.integers(0, n, nnz).astype(np.int32)
+    col = rng.integers(0, n, nnz).astype(np.int32)
+    val = rng.random(nnz)
+    x = rng.random(n)
+    p = tb.MatrixCOO(n, n, val, row, col)
+    o = oracle.from_arrays("COO", n, n, val, row, col)
+    assert np.array_equal(p.spmv(x), o.spmv(x))
+    r0 = rng.random(n)
+    assert np.array_equal(p.Ax(r0, x), o.Ax(r0, x))
+
+
+def test_long_rows(tb, oracle, gpu):
+    """Rows longer than one LDS tile (2048 entries) take the chunked path."""
+    n = 6000
+    rowptr = np.array([0, 5000, 5001, 5001, 17001] + [17001] * (n - 4), dtype=np.int32)
+    nnz = rowptr[-1]
+    rng = np.random.default_rng(2)
+    col = rng.integers(0, n, nnz).astype(np.int32)
+    val = rng.random(nnz)
+    x = rng.random(n)
+    p = tb.MatrixCSR(n, n, val, rowptr, col)
+    o = oracle.from_arrays("CSR", n, n, val, rowptr, col)
+    y, yo = p.spmv(x), o.spmv(x)
+    assert np.all(np.abs(y - yo) <= 1e-13 * np.abs(yo) + 1e-300)
+    rows = np.repeat(np.arange(n), np.diff(rowptr))
+    coo = tb.MatrixCOO(n, n, val, rows.astype(np.int32), col)
+    yc = coo.spmv(x)
+    assert np.all(np.abs(yc - yo) <= 1e-13 * np.abs(yo) + 1e-300)
+
+
+def test_device_vectors_and_precisions(tb, oracle, gpu):
+    """Ax on torch device tensors; FP32 / BF16 value storage with FP32
+    accumulation against the FP64 oracle."""
+    import torch
+    n = 1 << 16
+    p = tb.MatrixCSR().fill_cqmat(n, n, 16, 0.1, 42, value_seed=42)
+    o = oracle.new("CSR").fill_cqmat(n, n, 16, 0.1, 42, value_seed=42)
+    x = oracle.uniform_vector(1, n)
+    yo = o.spmv(x)
+    xd = torch.from_numpy(x).to(gpu)
+    yd = torch.zeros(n, dtype=torch.float64, device=gpu)
+    p.Ax_device(xd, yd)
+    torch.cuda.synchronize()
+    assert np.array_equal(yd.cpu().numpy(), yo)
+    for vd, tol in (("f32", 2e-6), ("bf16", 2e-2)):
+        y32 = torch.zeros(n, dtype=torch.float32, device=gpu)
+        p.Ax_device(xd.float(), y32, value_dtype=vd)
+        torch.cuda.synchronize()
+        err = np.abs(y32.cpu().numpy().astype(np.float64) - yo) / np.abs(yo).max()
+        assert err.max() < tol, (vd, err.max())
+    e = tb.MatrixELL().fill_cqmat(n, n, 16, 0.1, 42, value_seed=42)
+    y32 = torch.zeros(n, dtype=torch.float32, device=gpu)
+    e.Ax_device(xd.float(), y32, value_dtype="f32")
+    torch.cuda.synchronize()
+    assert (np.abs(y32.cpu().numpy() - yo) / np.abs(yo).max()).max() < 2e-6
+
+
+def test_short_vectors_are_rejected(tb, gpu):
+    """A vector shorter than the matrix's indices is an error, not a fault."""
+    from tbsla_amd._lib import TbslaError
+    m = tb.MatrixCSR().fill_cqmat(100, 100, 5, 0.5, 1)
+    with pytest.raises(TbslaError):
+        m.spmv(np.ones(50))
+    c = tb.MatrixCOO().fill_cqmat(100, 100, 5, 0.5, 1)
+    with pytest.raises(TbslaError):
+        c.Ax(np.zeros(10), np.ones(100))
+
+
+def test_reference_c_abi_spmv(tb, oracle, gpu, tmp_path):
+    """C_CPP_Matrix<F>_* / C_CPP_Vector_* used the way the YML spmv component
+    does (yml/impl/spmv.query:17-39): fill, spmv into a new vector, write it
+    out (Vector.cpp:90-103 format: size_t n, n doubles)."""
+    from tbsla_amd._lib import lib
+    for F in ["COO", "SCOO", "CSR", "ELL", "DENSE"]:
+        m = getattr(lib, f"C_CPP_Matrix{F}_create")()
+        getattr(lib, f"C_CPP_Matrix{F}_fill_cqmat")(m, 400, 400, 8, 0.3, 4, 0, 1, 2, 2)
+        v = lib.C_CPP_Vector_create()
+        lib.C_CPP_Vector_fill(v, 400 if F == "COO" else 200, 0)
+        r = getattr(lib, f"C_CPP_Matrix{F}_spmv")(m, v)
+        path = str(tmp_path / f"r_{F}.bin")
+        assert lib.C_CPP_Vector_write(r, path.encode())
+        raw = open(path, "rb").read()
+        n = int(np.frombuffer(raw[:8], dtype=np.uint64)[0])
+        y = np.frombuffer(raw[8:8 + 8 * n], dtype=np.float64)
+        o = oracle.new(F).fill_cqmat(400, 400, 8, 0.3, 4, 0, 1, 2, 2)
+        assert np.array_equal(y, o.spmv(np.arange(400 if F == "COO" else 200, dtype=np.float64))), F
+        lib.C_CPP_Vector_destroy(r)
+        lib.C_CPP_Vector_destroy(v)
+        getattr(lib, f"C_CPP_Matrix{F}_destroy")(m)
+
+
+@pytest.mark.slow
+def test_full_size_block_formats(tb, oracle, gpu):
+    """One rank's block (0,0) of fill_cqmat(2^24, 2^24, 16, 0.1, 42) on a 4x4
+    grid, values 1 - u01(42, i), x_j = u01(1, j): GPU == oracle bit for bit
+    (CSR, ELL, SCOO; COO on the rank-0 slice with a full-length x)."""
+    n = 1 << 24
+    for fmt in ["CSR", "ELL", "SCOO", "COO"]:
+        print(f"full-size {fmt}", flush=True)
+        p = tb.FORMATS[fmt]().fill_cqmat(n, n, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+        nx = n if fmt == "COO" else p.get_ln_col()
+        x = oracle.uniform_vector(1, nx)
+        y = p.spmv(x)
+        o = oracle.new(fmt).fill_cqmat(n, n, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+        yo = o.spmv(x)
+        assert np.array_equal(y, yo), (fmt, np.abs(y - yo).max())
+        del p, o
+    # DENSE: an 8192 x 8192 block of fill_cqmat(32768, ...)
+    p = tb.MatrixDENSE().fill_cqmat(32768, 32768, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+    o = oracle.new("DENSE").fill_cqmat(32768, 32768, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+    x = oracle.uniform_vector(1, 8192)
+    close("DENSE", p.spmv(x), o.spmv(x), o.spmv(np.abs(x)))
+
+
+def test_cdiag_10000_config(tb, golden, gpu):
+    """fill_cdiag(10000, 10000, 10) with x = u01(1, j): equal to the
+    reference's own product (fixture)."""
+    y = tb.MatrixCSR().fill_cdiag(10000, 10000, 10).spmv(golden["cdiag10000_x"])
+    assert np.array_equal(y, golden["cdiag10000_y"])

@@ -1,0 +1,164 @@
+"""Host side of libtbsla_hip.so (no GPU): generators, conversions, binary I/O
+and the C-ABI surface.  Generators and conversions must be bit-identical to
+the oracle (itself pinned to the reference) and to the reference digests."""
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import BLOCKS, CDIAG_SHAPES, CQMAT_SHAPES, FMTS, INFO_KEYS, ROOT, grid_q
+
+
+def same_matrix(prod, orc, fmt, check_nnz=True):
+    a = prod.arrays()
+    oi, oa = orc.arrays()
+    info = prod.info()
+    for k in INFO_KEYS:
+        assert info[k] == oi[k], (fmt, k, info[k], oi[k])
+    if fmt == "ELL":
+        assert info["max_col"] == oi["max_col"]
+    if check_nnz:
+        assert info["nnz"] == oi["nnz"], (fmt, info["nnz"], oi["nnz"])
+    assert set(a) == set(oa)
+    for k in oa:
+        assert a[k].dtype == oa[k].dtype and np.array_equal(a[k], oa[k], equal_nan=True), (fmt, k)
+
+
+def test_cqmat_grid_bit_exact(tb, oracle):
+    for (nr, nc, c) in CQMAT_SHAPES:
+        for fmt in FMTS:
+            for blk in BLOCKS:
+                for s in range(4):
+                    for q in grid_q()[::3]:
+                        p = tb.FORMATS[fmt]().fill_cqmat(nr, nc, c, q, s, *blk)
+                        o = oracle.new(fmt).fill_cqmat(nr, nc, c, q, s, *blk)
+                        same_matrix(p, o, fmt)
+
+
+def test_cdiag_grid_bit_exact(tb, oracle):
+    for (nr, nc, c) in CDIAG_SHAPES:
+        for fmt in FMTS:
+            for blk in BLOCKS:
+                p = tb.FORMATS[fmt]().fill_cdiag(nr, nc, c, *blk)
+                o = oracle.new(fmt).fill_cdiag(nr, nc, c, *blk)
+                # the reference never sets nnz in CSR/SCOO fill_cdiag
+                same_matrix(p, o, fmt, check_nnz=fmt not in ("CSR", "SCOO"))
+
+
+def test_value_overlay_and_stochastic(tb, oracle):
+    for (nr, nc, c, q, s) in [(300, 300, 16, 0.1, 42), (257, 300, 12, 0.3, 5), (300, 250, 8, 0.5, 7)]:
+        for fmt in FMTS:
+            for blk in BLOCKS:
+                p = tb.FORMATS[fmt]().fill_cqmat(nr, nc, c, q, s, *blk, value_seed=42)
+                o = oracle.new(fmt).fill_cqmat(nr, nc, c, q, s, *blk, value_seed=42)
+                same_matrix(p, o, fmt)
+                p = tb.FORMATS[fmt]().fill_cqmat_stochastic(nr, nc, c, q, s, *blk)
+                o = oracle.new(fmt).fill_cqmat_stochastic(nr, nc, c, q, s, *blk)
+                same_matrix(p, o, fmt)
+                assert p.count_stochastic()
+                p = tb.FORMATS[fmt]().fill_cqmat(nr, nc, c, q, s, *blk, value_seed=7)
+                o = oracle.new(fmt).fill_cqmat(nr, nc, c, q, s, *blk, value_seed=7)
+                if o.arrays()[1]["values"].size:
+                    p.normalize_columns()
+                    o.normalize_columns()
+                    same_matrix(p, o, fmt)
+                    assert not p.count_stochastic()
+
+
+def test_conversions(tb, oracle):
+    for (nr, nc, c, q, s) in [(30, 30, 8, 0.5, 3), (50, 40, 5, 1.0, 1), (64, 64, 12, 0.2, 9)]:
+        pc = tb.MatrixCOO().fill_cqmat(nr, nc, c, q, s, value_seed=11)
+        oc = oracle.new("COO").fill_cqmat(nr, nc, c, q, s, value_seed=11)
+        for fmt, cls in [("SCOO", tb.MatrixSCOO), ("CSR", tb.MatrixCSR), ("ELL", tb.MatrixELL),
+                         ("DENSE", tb.MatrixDENSE)]:
+            same_matrix(cls(pc), oracle.convert(oc, fmt), fmt)
+
+
+def test_array_constructors(tb, golden):
+    m = tb.MatrixCSR(5, 5, golden["ka_values"], golden["ka_rowptr"], golden["ka_colidx"])
+    a = m.arrays()
+    assert np.array_equal(a["rowptr"], golden["ka_rowptr"]) and m.get_n_row() == 5
+    m = tb.MatrixCOO(5, 5, golden["ka_values"], golden["ka_row"], golden["ka_colidx"])
+    assert np.array_equal(m.arrays()["row"], golden["ka_row"])
+
+
+def test_binary_io_matches_oracle_format(tb, oracle, tmp_path):
+    """write() appends the reference binary layout; read() restores it; the
+    oracle's reader (written from MatrixCSR.cpp:206-232) reads our files."""
+    for fmt in FMTS:
+        p = tb.FORMATS[fmt]().fill_cqmat(40, 40, 6, 0.3, 2, 1, 0, 2, 2, value_seed=3)
+        f = str(tmp_path / f"m.{fmt}")
+        p.write(f)
+        p.write(f)  # appended twice, like C_CPP_Matrix<F>_write
+        size1 = os.path.getsize(f) // 2
+        q = tb.FORMATS[fmt]().read(f, size1)
+        same_matrix(q, oracle.read(fmt, f, 0), fmt)
+        same_matrix(p, oracle.read(fmt, f, size1), fmt)
+
+
+@pytest.mark.slow
+def test_full_size_generators_match_reference_digests(tb, digests):
+    cases = {
+        "cqmat2^24c16q0.1s42/CSR/block0.0.4.4": lambda: tb.MatrixCSR().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4),
+        "cqmat2^24c16q0.1s42/ELL/block0.0.4.4": lambda: tb.MatrixELL().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4),
+        "cqmat2^24c16q0.1s42/SCOO/block0.0.4.4": lambda: tb.MatrixSCOO().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4),
+        "cqmat2^24c16q0.1s42/COO/rank0of16": lambda: tb.MatrixCOO().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4),
+        "cqmat32768c16q0.1s42/DENSE/block0.0.4.4": lambda: tb.MatrixDENSE().fill_cqmat(32768, 32768, 16, 0.1, 42, 0, 0, 4, 4),
+        "stoch2^24c8q0.5s7/CSR": lambda: tb.MatrixCSR().fill_cqmat_stochastic(2 ** 24, 2 ** 24, 8, 0.5, 7),
+        "stoch2^22c16q0.3s1256/CSR": lambda: tb.MatrixCSR().fill_cqmat_stochastic(2 ** 22, 2 ** 22, 16, 0.3, 1256),
+        "stoch2^20c16q0.2s3/CSR": lambda: tb.MatrixCSR().fill_cqmat_stochastic(2 ** 20, 2 ** 20, 16, 0.2, 3),
+        "stoch262144c12q0.3s1003/CSR": lambda: tb.MatrixCSR().fill_cqmat_stochastic(262144, 262144, 12, 0.3, 1003),
+        "cdiag10000/CSR": lambda: tb.MatrixCSR().fill_cdiag(10000, 10000, 10),
+    }
+    for key, make in cases.items():
+        m = make()
+        ref = digests["big"][key]
+        info = m.info()
+        for k in INFO_KEYS:
+            assert info[k] == ref[k], (key, k)
+        if not key.startswith("cdiag"):
+            assert info["nnz"] == ref["nnz"], key
+        for k, a in m.arrays().items():
+            assert a.size == ref[f"len_{k}"], (key, k)
+            assert hashlib.sha256(a.tobytes()).hexdigest() == ref[f"sha256_{k}"], (key, k)
+        del m
+
+
+# ----------------------------------------------------------------- the C ABI
+
+def declared_symbols():
+    names = set()
+    for h in ("tbsla_hip.h", "tbsla_cpp_c.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"\b(tbsla_(?:matrix|hip)_\w+|tbsla_abi_version|tbsla_last_error)\s*\(", txt))
+        for F in re.findall(r"TBSLA_C_MATRIX_DECL\((\w+)\)\n", txt):
+            for op in ("create", "destroy", "fill_cdiag", "fill_cqmat", "read", "write", "print", "spmv"):
+                names.add(f"C_CPP_Matrix{F}_{op}")
+        names |= set(re.findall(r"\b(C_CPP_Vector_\w+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol(tb):
+    from tbsla_amd import _lib
+    names = declared_symbols()
+    assert len(names) >= 75
+    for n in sorted(names):
+        assert hasattr(_lib.lib, n), n
+    assert set(_lib.SIGNATURES) == names
+    assert _lib.lib.tbsla_abi_version() == 1
+
+
+def test_c_abi_errors_without_device_calls(tb):
+    from tbsla_amd import _lib
+    assert _lib.lib.tbsla_matrix_create(9) is None
+    assert b"unknown format" in _lib.lib.tbsla_last_error()
+    m = tb.MatrixCSR()
+    with pytest.raises(_lib.TbslaError):
+        m.fill_cqmat(10, 10, 3, 0.1, 1, 2, 0, 2, 2)  # pr >= NR
+    v = _lib.lib.C_CPP_Vector_create()
+    _lib.lib.C_CPP_Vector_fill(v, 5, 3)
+    assert not _lib.lib.C_CPP_Vector_add(v, None)
+    _lib.lib.C_CPP_Vector_destroy(v)
+    _lib.lib.C_CPP_MatrixCSR_destroy(None)
