@@ -105,9 +105,14 @@ void build_csr(DeviceMatrix &d, const cpp::MatrixCSR &m, Dtype vdt) {
   // entry j of row i is values[rowptr[i] - rowptr[0] + k] (MatrixCSR.cpp:135)
   d.colidx.upload(ci.data(), static_cast<std::size_t>(used) * sizeof(int));
   upload_values(d.vals, vals.data(), static_cast<std::size_t>(used), vdt);
-  std::vector<int> t = csr_tiles(rp.data(), d.rows, kCsrTileNnz, kCsrTileRows);
+  std::vector<int> t = csr_entry_tiles(rp.data(), d.rows, kCsrTileNnz);
   d.ntiles = static_cast<int>(t.size()) - 1;
   d.tiles.upload(t.data(), t.size() * sizeof(int));
+  std::vector<int> lr;
+  for (int i = 0; i < d.rows; i++)
+    if (rp[static_cast<std::size_t>(i) + 1] - rp[static_cast<std::size_t>(i)] > kCsrLongRow) lr.push_back(i);
+  d.nlong = static_cast<int>(lr.size());
+  if (d.nlong) d.long_rows.upload(lr.data(), lr.size() * sizeof(int));
 }
 
 void build_ell(DeviceMatrix &d, const cpp::MatrixELL &m, Dtype vdt) {
@@ -219,23 +224,19 @@ void build_dense(DeviceMatrix &d, const cpp::MatrixDENSE &m, Dtype vdt) {
 
 }  // namespace
 
-std::vector<int> csr_tiles(const int *rowptr, int rows, int max_nnz, int max_rows) {
-  std::vector<int> t;
-  t.reserve(static_cast<std::size_t>(rows) / 64 + 2);
-  t.push_back(0);
+std::vector<int> csr_entry_tiles(const int *rowptr, int rows, int tile_nnz) {
+  const long rp0 = rowptr[0];
+  const long nnz = static_cast<long>(rowptr[rows]) - rp0;
+  const long ntiles = nnz / tile_nnz + 1;
+  std::vector<int> t(static_cast<std::size_t>(ntiles) + 1);
+  // tile_row[k] = first row whose first entry is at or after k * tile_nnz
   int r = 0;
-  while (r < rows) {
-    const int start = r;
-    long nnz = 0;
-    while (r < rows && r - start < max_rows) {
-      const long len = static_cast<long>(rowptr[r + 1]) - rowptr[r];
-      if (nnz + len > max_nnz) break;
-      nnz += len;
-      r++;
-    }
-    if (r == start) r++;  // a row longer than one tile stands alone
-    t.push_back(r);
+  for (long k = 0; k < ntiles; k++) {
+    const long e = k * tile_nnz;
+    while (r < rows && static_cast<long>(rowptr[r]) - rp0 < e) r++;
+    t[static_cast<std::size_t>(k)] = r;
   }
+  t[static_cast<std::size_t>(ntiles)] = rows;
   return t;
 }
 

@@ -1,8 +1,9 @@
 // Device mirror of one host matrix: the HBM layout the kernels read.
 //
-//   CSR   rowptr/colidx as stored, values in the requested type, plus a
-//         row-tile table: tile t = rows [tiles[t], tiles[t+1]) holding at most
-//         kCsrTileNnz entries (a longer row is a tile of its own).
+//   CSR   rowptr/colidx as stored, values in the requested type, plus the
+//         entry-tile table: tile t = entries [t*kCsrTileNnz, (t+1)*kCsrTileNnz)
+//         owns rows [tile_row[t], tile_row[t+1]) (the rows whose first entry
+//         lies in the tile), and the list of rows longer than kCsrLongRow.
 //   ELL   column-major (slot j of row i at j*ld + i), ld = rows rounded up to 64.
 //   COO / SCOO
 //         entries in a row-sorted view (a stable counting sort by row when the
@@ -28,7 +29,7 @@ namespace hip {
 constexpr int kTileThreads = 256;
 constexpr int kTilePer = 8;
 constexpr int kCsrTileNnz = kTileThreads * kTilePer;  // entries staged in LDS per tile
-constexpr int kCsrTileRows = 256;                     // rows per tile at most
+constexpr int kCsrLongRow = kCsrTileNnz / 2;          // longer rows: one workgroup each
 
 struct DeviceMatrix {
   cpp::Format format = cpp::FORMAT_CSR;
@@ -40,8 +41,8 @@ struct DeviceMatrix {
   long x_need = 0;         // x must hold at least this many elements
   long y_lo = 0, y_hi = 0; // COO/SCOO: rows touched are [y_lo, y_hi) before vect_incr
   // CSR
-  DevBuf rowptr, colidx, vals, tiles;
-  int ntiles = 0, rp0 = 0;
+  DevBuf rowptr, colidx, vals, tiles, long_rows;
+  int ntiles = 0, nlong = 0, rp0 = 0;
   // ELL (column-major)
   int K = 0;
   long ld = 0;
@@ -67,7 +68,8 @@ struct DeviceCache {
 const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt);
 
 // Host tile tables (exposed for tests and the autotuner).
-std::vector<int> csr_tiles(const int *rowptr, int rows, int max_nnz, int max_rows);
+// tile_row of the CSR entry tiles (ntiles + 1 entries, ntiles = nnz / T + 1).
+std::vector<int> csr_entry_tiles(const int *rowptr, int rows, int tile_nnz);
 std::vector<int> segment_tiles(const int *row_sorted, long n, int max_nnz);
 
 }  // namespace hip

@@ -1,14 +1,17 @@
 // PageRank iteration kernels and their host driver (see pagerank.hpp).
 //
-// One iteration of the CSR engine is two launches:
-//   pr_update_csr   tile SpMV of the current iterate fused with
-//                   b = beta*(A b_t) + tele, the block max of b, and a
-//                   last-block reduction of the maxima into the state.
-//   pr_normalize    b /= max, the L1 residual against b_t, the compensated
-//                   sum of b (next teleport term and stopping bound), and a
-//                   last-block finalize that updates the state: iteration
-//                   count, convergence, the next teleport term, and (mixed
-//                   precision) whether to move the anchor.
+// One iteration of the CSR engine is four launches (five with long rows):
+//   pr_update_csr     entry-tile SpMV of the current iterate fused with
+//                     b = beta*(A b_t) + tele and the block max of b, merged
+//                     into 64 atomic-max slots.
+//   pr_finalize_max   one wavefront: max over the slots into the state.
+//   pr_normalize      b /= max, the L1 residual against b_t and the
+//                     compensated sum of b, as per-block partials.
+//   pr_finalize_norm  one block: merges the partials and updates the state:
+//                     iteration count, convergence, the next teleport term,
+//                     and (mixed precision) whether to move the anchor.
+// Partial results cross workgroups only at kernel boundaries, so no kernel
+// needs a device-wide fence.
 // Every kernel returns at once when the state says the run has converged, so
 // the host launches `check_every` iterations between two reads of the state.
 #include <hip/hip_runtime.h>
@@ -39,7 +42,6 @@ struct PrState {
   double anchor_tau, floor_eps;
   int it, max_it, done, rule;
   int need_anchor, anchors, anchor_it, mixed;
-  unsigned cnt_max, cnt_norm;
 };
 
 constexpr int kNormBlocks = 1024;
@@ -61,23 +63,6 @@ __device__ inline CSum csum_merge(CSum a, CSum b) {
   const double bp = t - a.s;
   const double e = (a.s - (t - bp)) + (b.s - bp);
   return CSum{t, a.c + b.c + e};
-}
-
-__device__ inline double atomic_read(const double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Returns true in every thread of the last block to arrive; partial results
-// written before the call are visible to that block.
-__device__ inline bool last_block_arrive(unsigned *counter, int *flag_lds) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    *flag_lds = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  return *flag_lds != 0;
 }
 
 __device__ inline CSum block_csum(CSum v, double *ps, double *pc) {
@@ -114,72 +99,53 @@ struct UpdateRow {
   }
 };
 
-template <typename VT, typename XT, bool MIXED, bool PERSONAL>
-__global__ __launch_bounds__(kTileThreads) void pr_update_csr(const int *__restrict__ rowptr,
-                                                              const int *__restrict__ colidx,
-                                                              const VT *__restrict__ vals, const XT *__restrict__ x,
-                                                              const int *__restrict__ tiles, int rp0, int f_col,
+// Block maxima are merged with integer atomicMax on 64 slots (a block uses
+// slot blockIdx % 64), each slot holding an order-preserving encoding of a
+// double (0 = empty), so the finalize reads 64 numbers however many blocks ran.
+constexpr int kMaxSlots = 64;
+
+__device__ inline unsigned long long ord_encode(double v) {
+  const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double ord_decode(unsigned long long u) {
+  if (u == 0) return -INFINITY;
+  return __longlong_as_double(static_cast<long long>((u >> 63) ? (u & 0x7fffffffffffffffull) : ~u));
+}
+__device__ inline void slot_max(unsigned long long *slots, double v, double *red) {
+  const double bm = dev::block_tree_max<double>(v, red);
+  if (threadIdx.x == 0) atomicMax(slots + (blockIdx.x % kMaxSlots), ord_encode(bm));
+}
+
+// Fused update over the CSR tiles (LONG = false) or the long rows (LONG =
+// true): b = beta * (A x) + tele, block max into the slots.  VT = dev::Pattern
+// is the value-free product over z = colscale .* b_t.
+template <typename VT, typename XT, bool MIXED, bool PERSONAL, bool LONG>
+__global__ __launch_bounds__(kTileThreads) void pr_update_csr(dev::CsrView<VT> m, const XT *__restrict__ x,
                                                               const double *__restrict__ yhat,
                                                               const unsigned char *__restrict__ member,
                                                               double *__restrict__ b, PrState *st,
-                                                              double *__restrict__ pmax) {
+                                                              unsigned long long *slots) {
   if (st->done) return;
   using AT = typename dev::Val<VT>::A;
   __shared__ dev::CsrTileLds<AT> lds;
   __shared__ double red[kTileThreads];
-  __shared__ int last;
   UpdateRow<AT, MIXED, PERSONAL> epi{b, yhat, member, st->beta, st->tele, -INFINITY};
-  dev::csr_tile_rows<VT, XT>(rowptr, colidx, vals, x, tiles, rp0, f_col, lds, epi);
-  const double bm = dev::block_tree_max<double>(epi.lmax, red);
-  if (threadIdx.x == 0) pmax[blockIdx.x] = bm;
-  if (last_block_arrive(&st->cnt_max, &last)) {
-    double m = -INFINITY;
-    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) m = fmax(m, atomic_read(pmax + i));
-    m = dev::block_tree_max<double>(m, red);
-    if (threadIdx.x == 0) {
-      st->maxv = m;
-      st->cnt_max = 0;
-    }
-  }
-}
-
-// Value-free update for count-stochastic CSR: x is z = colscale .* b_t.
-template <bool PERSONAL>
-__global__ __launch_bounds__(kTileThreads) void pr_update_pattern(const int *__restrict__ rowptr,
-                                                                  const int *__restrict__ colidx,
-                                                                  const double *__restrict__ z,
-                                                                  const int *__restrict__ tiles, int rp0, int f_col,
-                                                                  const unsigned char *__restrict__ member,
-                                                                  double *__restrict__ b, PrState *st,
-                                                                  double *__restrict__ pmax) {
-  if (st->done) return;
-  __shared__ dev::CsrTileLds<double> lds;
-  __shared__ double red[kTileThreads];
-  __shared__ int last;
-  UpdateRow<double, false, PERSONAL> epi{b, nullptr, member, st->beta, st->tele, -INFINITY};
-  dev::csr_pattern_tile_rows(rowptr, colidx, z, tiles, rp0, f_col, lds, epi);
-  const double bm = dev::block_tree_max<double>(epi.lmax, red);
-  if (threadIdx.x == 0) pmax[blockIdx.x] = bm;
-  if (last_block_arrive(&st->cnt_max, &last)) {
-    double m = -INFINITY;
-    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) m = fmax(m, atomic_read(pmax + i));
-    m = dev::block_tree_max<double>(m, red);
-    if (threadIdx.x == 0) {
-      st->maxv = m;
-      st->cnt_max = 0;
-    }
-  }
+  if (LONG)
+    dev::csr_long_row<VT, XT>(m, x, lds, epi);
+  else
+    dev::csr_tile_rows<VT, XT>(m, x, lds, epi);
+  slot_max(slots, epi.lmax, red);
 }
 
 // Update for any format after a plain product y = A b_t (generic engine).
 template <bool PERSONAL>
 __global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict__ b, long n,
                                                               const unsigned char *__restrict__ member, PrState *st,
-                                                              double *__restrict__ pmax) {
+                                                              unsigned long long *slots) {
 #pragma clang fp contract(off)
   if (st->done) return;
   __shared__ double red[kTileThreads];
-  __shared__ int last;
   const double beta = st->beta, tele = st->tele;
   double lmax = -INFINITY;
   for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
@@ -189,17 +155,7 @@ __global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict
     b[i] = bi;
     lmax = fmax(lmax, bi);
   }
-  const double bm = dev::block_tree_max<double>(lmax, red);
-  if (threadIdx.x == 0) pmax[blockIdx.x] = bm;
-  if (last_block_arrive(&st->cnt_max, &last)) {
-    double m = -INFINITY;
-    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) m = fmax(m, atomic_read(pmax + i));
-    m = dev::block_tree_max<double>(m, red);
-    if (threadIdx.x == 0) {
-      st->maxv = m;
-      st->cnt_max = 0;
-    }
-  }
+  slot_max(slots, lmax, red);
 }
 
 // Decision logic of the last block of pr_normalize (one thread).
@@ -248,7 +204,6 @@ __global__ __launch_bounds__(kTileThreads) void pr_normalize(double *__restrict_
 #pragma clang fp contract(off)
   if (st->done) return;
   __shared__ double red_s[kTileThreads], red_c[kTileThreads];
-  __shared__ int last;
   const double m = st->maxv;
   double err = 0;
   CSum cs{0, 0};
@@ -272,20 +227,38 @@ __global__ __launch_bounds__(kTileThreads) void pr_normalize(double *__restrict_
     ps[blockIdx.x] = bs.s;
     pc[blockIdx.x] = bs.c;
   }
-  if (last_block_arrive(&st->cnt_norm, &last)) {
-    double e = 0;
-    CSum t{0, 0};
-    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads) {
-      e += atomic_read(perr + i);
-      t = csum_merge(t, CSum{atomic_read(ps + i), atomic_read(pc + i)});
-    }
-    e = dev::block_tree_sum<double>(e, red_s);
-    t = block_csum(t, red_s, red_c);
-    if (threadIdx.x == 0) {
-      finalize_step(st, e, t.s + t.c);
-      st->cnt_norm = 0;
-    }
+}
+
+// One wavefront: the maximum over the slots, which are cleared for the next
+// update.  Slots are read and cleared with device-scope atomics.
+__global__ __launch_bounds__(64) void pr_finalize_max(unsigned long long *slots, PrState *st) {
+  if (st->done) return;
+  unsigned long long u = __hip_atomic_load(slots + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(slots + threadIdx.x, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(u, off, 64);
+    u = o > u ? o : u;
   }
+  if (threadIdx.x == 0) st->maxv = ord_decode(u);
+}
+
+// One block: residual and compensated sum of the normalise pass, then the
+// step decision.
+__global__ __launch_bounds__(kTileThreads) void pr_finalize_norm(const double *__restrict__ perr,
+                                                                 const double *__restrict__ ps,
+                                                                 const double *__restrict__ pc, int nb, PrState *st) {
+  if (st->done) return;
+  __shared__ double red_s[kTileThreads], red_c[kTileThreads];
+  double e = 0;
+  CSum t{0, 0};
+  for (int i = threadIdx.x; i < nb; i += kTileThreads) {
+    e += perr[i];
+    t = csum_merge(t, CSum{ps[i], pc[i]});
+  }
+  e = dev::block_tree_sum<double>(e, red_s);
+  t = block_csum(t, red_s, red_c);
+  if (threadIdx.x == 0) finalize_step(st, e, t.s + t.c);
 }
 
 // Mixed precision: move the anchor to the current iterate when requested.
@@ -310,38 +283,26 @@ struct StoreRowD {
   }
 };
 
-// Mixed precision: the FP64 anchor product yhat = A64 xhat (values or value-free).
-__global__ __launch_bounds__(kTileThreads) void pr_anchor_spmv(const int *__restrict__ rowptr,
-                                                               const int *__restrict__ colidx,
-                                                               const double *__restrict__ vals,
-                                                               const double *__restrict__ xhat,
-                                                               const int *__restrict__ tiles, int rp0, int f_col,
+// Mixed precision: the FP64 anchor product yhat = A64 xhat (VT = double) or
+// its value-free form over z = colscale .* xhat (VT = dev::Pattern).
+template <typename VT, bool LONG>
+__global__ __launch_bounds__(kTileThreads) void pr_anchor_spmv(dev::CsrView<VT> m, const double *__restrict__ x,
                                                                double *__restrict__ yhat, const PrState *st) {
   if (st->done || !st->need_anchor) return;
   __shared__ dev::CsrTileLds<double> lds;
   StoreRowD<double> epi{yhat};
-  dev::csr_tile_rows<double, double>(rowptr, colidx, vals, xhat, tiles, rp0, f_col, lds, epi);
-}
-__global__ __launch_bounds__(kTileThreads) void pr_anchor_spmv_pattern(const int *__restrict__ rowptr,
-                                                                       const int *__restrict__ colidx,
-                                                                       const double *__restrict__ z,
-                                                                       const int *__restrict__ tiles, int rp0,
-                                                                       int f_col, double *__restrict__ yhat,
-                                                                       const PrState *st) {
-  if (st->done || !st->need_anchor) return;
-  __shared__ dev::CsrTileLds<double> lds;
-  StoreRowD<double> epi{yhat};
-  dev::csr_pattern_tile_rows(rowptr, colidx, z, tiles, rp0, f_col, lds, epi);
+  if (LONG)
+    dev::csr_long_row<VT, double>(m, x, lds, epi);
+  else
+    dev::csr_tile_rows<VT, double>(m, x, lds, epi);
 }
 
 // v = a (+ d) ; out = v / sum(v) with a compensated sum (two passes).
 __global__ __launch_bounds__(kTileThreads) void pr_sum_kernel(const double *__restrict__ a,
                                                               const float *__restrict__ d, long n,
-                                                              double *__restrict__ ps, double *__restrict__ pc,
-                                                              double *__restrict__ total, unsigned *counter) {
+                                                              double *__restrict__ ps, double *__restrict__ pc) {
 #pragma clang fp contract(off)
   __shared__ double red_s[kTileThreads], red_c[kTileThreads];
-  __shared__ int last;
   CSum cs{0, 0};
   for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * kTileThreads)
@@ -351,16 +312,16 @@ __global__ __launch_bounds__(kTileThreads) void pr_sum_kernel(const double *__re
     ps[blockIdx.x] = bs.s;
     pc[blockIdx.x] = bs.c;
   }
-  if (last_block_arrive(counter, &last)) {
-    CSum t{0, 0};
-    for (int i = threadIdx.x; i < static_cast<int>(gridDim.x); i += kTileThreads)
-      t = csum_merge(t, CSum{atomic_read(ps + i), atomic_read(pc + i)});
-    t = block_csum(t, red_s, red_c);
-    if (threadIdx.x == 0) {
-      *total = t.s + t.c;
-      *counter = 0;
-    }
-  }
+}
+
+__global__ __launch_bounds__(kTileThreads) void pr_sum_finalize(const double *__restrict__ ps,
+                                                                const double *__restrict__ pc, int nb,
+                                                                double *__restrict__ total) {
+  __shared__ double red_s[kTileThreads], red_c[kTileThreads];
+  CSum t{0, 0};
+  for (int i = threadIdx.x; i < nb; i += kTileThreads) t = csum_merge(t, CSum{ps[i], pc[i]});
+  t = block_csum(t, red_s, red_c);
+  if (threadIdx.x == 0) *total = t.s + t.c;
 }
 
 __global__ __launch_bounds__(256) void pr_scale_kernel(const double *__restrict__ a, const float *__restrict__ d,
@@ -385,8 +346,43 @@ __global__ __launch_bounds__(256) void scale_into(const double *__restrict__ s, 
 
 unsigned grid_for(long n) { return static_cast<unsigned>(std::max<long>(1, std::min<long>(kNormBlocks, (n + 255) / 256))); }
 
+template <typename VT, typename XT, bool MIXED, bool PERSONAL>
+void launch_update(const DeviceMatrix &d, const XT *x, const double *yhat, const unsigned char *member, double *b,
+                   PrState *st, unsigned long long *slots, hipStream_t s) {
+  const dev::CsrView<VT> m = dev::csr_view<VT>(d);
+  if (d.ntiles > 0)
+    hipLaunchKernelGGL((pr_update_csr<VT, XT, MIXED, PERSONAL, false>), dim3(d.ntiles), dim3(kTileThreads), 0, s, m,
+                       x, yhat, member, b, st, slots);
+  if (d.nlong > 0)
+    hipLaunchKernelGGL((pr_update_csr<VT, XT, MIXED, PERSONAL, true>), dim3(d.nlong), dim3(kTileThreads), 0, s, m, x,
+                       yhat, member, b, st, slots);
+}
+
+template <typename VT, typename XT, bool MIXED>
+void launch_update(bool personal, const DeviceMatrix &d, const XT *x, const double *yhat,
+                   const unsigned char *member, double *b, PrState *st, unsigned long long *slots, hipStream_t s) {
+  if (personal)
+    launch_update<VT, XT, MIXED, true>(d, x, yhat, member, b, st, slots, s);
+  else
+    launch_update<VT, XT, MIXED, false>(d, x, yhat, member, b, st, slots, s);
+}
+
+template <typename VT>
+void launch_anchor(const DeviceMatrix &d, const double *x, double *yhat, const PrState *st, hipStream_t s) {
+  const dev::CsrView<VT> m = dev::csr_view<VT>(d);
+  if (d.ntiles > 0)
+    hipLaunchKernelGGL((pr_anchor_spmv<VT, false>), dim3(d.ntiles), dim3(kTileThreads), 0, s, m, x, yhat, st);
+  if (d.nlong > 0)
+    hipLaunchKernelGGL((pr_anchor_spmv<VT, true>), dim3(d.nlong), dim3(kTileThreads), 0, s, m, x, yhat, st);
+}
+
+__global__ void zero_u64(unsigned long long *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 struct Scratch {
-  DevBuf st, pmax, perr, ps, pc, total, counter, member;
+  DevBuf st, slots, perr, ps, pc, total, member;
   DevBuf X0, X1, Z, yhat, xhat, D0, D1, colscale;
 };
 
@@ -394,7 +390,9 @@ struct Scratch {
 
 PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, double *out_dev, hipStream_t s) {
   const long n = m.get_n_col();
-  if (m.get_n_row() != m.get_n_col() || m.get_ln_row() != m.get_n_row() || m.get_ln_col() != m.get_n_col())
+  // (COO keeps NR/NC in ln_row/ln_col, MatrixCOO.cpp:344-345, so the block
+  // layout is read from NR/NC)
+  if (m.get_n_row() != m.get_n_col() || m.get_NR() != 1 || m.get_NC() != 1)
     throw std::invalid_argument("tbsla: page_rank needs a square matrix held in one block");
   PageRankReport rep;
   if (n == 0) return rep;
@@ -418,14 +416,12 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
 
   Scratch w;
   const unsigned gN = grid_for(n);
-  const unsigned gT = fused ? static_cast<unsigned>(std::max(d64.ntiles, 1)) : gN;
   w.st.alloc(sizeof(PrState));
-  w.pmax.alloc(sizeof(double) * std::max(gT, gN));
+  w.slots.alloc(sizeof(unsigned long long) * kMaxSlots);
   w.perr.alloc(sizeof(double) * gN);
   w.ps.alloc(sizeof(double) * gN);
   w.pc.alloc(sizeof(double) * gN);
   w.total.alloc(sizeof(double));
-  w.counter.alloc(sizeof(unsigned));
   w.X0.alloc(sizeof(double) * n);
   w.X1.alloc(sizeof(double) * n);
   unsigned char *member = nullptr;
@@ -471,6 +467,8 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   PrState *st = w.st.as<PrState>();
   double *X[2] = {w.X0.as<double>(), w.X1.as<double>()};
   hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[0], n, b0);
+  unsigned long long *slots = w.slots.as<unsigned long long>();
+  hipLaunchKernelGGL(zero_u64, dim3(1), dim3(kMaxSlots), 0, s, slots, kMaxSlots);
   float *D[2] = {nullptr, nullptr};
   if (mixed) {
     w.xhat.alloc(sizeof(double) * n);
@@ -497,6 +495,14 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   PrState hs{};
   int launched = 0;
   const bool mixed_pattern = mixed && m.count_stochastic();
+  auto fin_max = [&]() { hipLaunchKernelGGL(pr_finalize_max, dim3(1), dim3(kMaxSlots), 0, s, slots, st); };
+  auto fin_norm = [&]() {
+    hipLaunchKernelGGL(pr_finalize_norm, dim3(1), dim3(kTileThreads), 0, s, w.perr.as<double>(), w.ps.as<double>(),
+                       w.pc.as<double>(), static_cast<int>(gN), st);
+  };
+  const bool pers = p.personalized;
+  double *Z = w.Z.as<double>();
+  const double *colscale = w.colscale.as<double>();
   while (launched < p.max_iteration) {
     const int batch = std::min(check, p.max_iteration - launched);
     for (int k = 0; k < batch; k++) {
@@ -505,82 +511,48 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
       if (fused) {
         if (mixed) {
           float *dcur = D[it & 1], *dnext = D[(it + 1) & 1];
-          hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, w.xhat.as<double>(), dcur,
-                             mixed_pattern ? w.Z.as<double>() : nullptr, w.colscale.as<double>(), n, st);
+          double *xhat = w.xhat.as<double>(), *yhat = w.yhat.as<double>();
+          hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, xhat, dcur, mixed_pattern ? Z : nullptr,
+                             colscale, n, st);
           if (mixed_pattern)
-            hipLaunchKernelGGL(pr_anchor_spmv_pattern, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
-                               d64.colidx.as<int>(), w.Z.as<double>(), d64.tiles.as<int>(), d64.rp0, d64.f_col,
-                               w.yhat.as<double>(), st);
+            launch_anchor<dev::Pattern>(d64, Z, yhat, st, s);
           else
-            hipLaunchKernelGGL(pr_anchor_spmv, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
-                               d64.colidx.as<int>(), d64.vals.as<double>(), w.xhat.as<double>(), d64.tiles.as<int>(),
-                               d64.rp0, d64.f_col, w.yhat.as<double>(), st);
-          if (p.precision == PrPrecision::F32) {
-            if (p.personalized)
-              hipLaunchKernelGGL((pr_update_csr<float, float, true, true>), dim3(gT), dim3(kTileThreads), 0, s,
-                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<float>(), dcur,
-                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
-                                 w.pmax.as<double>());
-            else
-              hipLaunchKernelGGL((pr_update_csr<float, float, true, false>), dim3(gT), dim3(kTileThreads), 0, s,
-                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<float>(), dcur,
-                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
-                                 w.pmax.as<double>());
-          } else {
-            if (p.personalized)
-              hipLaunchKernelGGL((pr_update_csr<dev::bf16, float, true, true>), dim3(gT), dim3(kTileThreads), 0, s,
-                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<dev::bf16>(), dcur,
-                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
-                                 w.pmax.as<double>());
-            else
-              hipLaunchKernelGGL((pr_update_csr<dev::bf16, float, true, false>), dim3(gT), dim3(kTileThreads), 0, s,
-                                 dlo->rowptr.as<int>(), dlo->colidx.as<int>(), dlo->vals.as<dev::bf16>(), dcur,
-                                 dlo->tiles.as<int>(), dlo->rp0, dlo->f_col, w.yhat.as<double>(), member, xout, st,
-                                 w.pmax.as<double>());
-          }
-          hipLaunchKernelGGL(pr_normalize<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr,
-                             w.xhat.as<double>(), dcur, dnext, nullptr, nullptr, n, st, w.perr.as<double>(),
-                             w.ps.as<double>(), w.pc.as<double>());
+            launch_anchor<double>(d64, xhat, yhat, st, s);
+          if (p.precision == PrPrecision::F32)
+            launch_update<float, float, true>(pers, *dlo, dcur, yhat, member, xout, st, slots, s);
+          else
+            launch_update<dev::bf16, float, true>(pers, *dlo, dcur, yhat, member, xout, st, slots, s);
+          fin_max();
+          hipLaunchKernelGGL(pr_normalize<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,
+                             nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
+          fin_norm();
         } else if (pattern) {
-          if (p.personalized)
-            hipLaunchKernelGGL(pr_update_pattern<true>, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
-                               d64.colidx.as<int>(), w.Z.as<double>(), d64.tiles.as<int>(), d64.rp0, d64.f_col,
-                               member, xout, st, w.pmax.as<double>());
-          else
-            hipLaunchKernelGGL(pr_update_pattern<false>, dim3(gT), dim3(kTileThreads), 0, s, d64.rowptr.as<int>(),
-                               d64.colidx.as<int>(), w.Z.as<double>(), d64.tiles.as<int>(), d64.rp0, d64.f_col,
-                               member, xout, st, w.pmax.as<double>());
+          launch_update<dev::Pattern, double, false>(pers, d64, Z, nullptr, member, xout, st, slots, s);
+          fin_max();
           hipLaunchKernelGGL(pr_normalize<2>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
-                             nullptr, w.Z.as<double>(), w.colscale.as<double>(), n, st, w.perr.as<double>(),
-                             w.ps.as<double>(), w.pc.as<double>());
+                             nullptr, Z, colscale, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
+          fin_norm();
         } else {
-          if (p.personalized)
-            hipLaunchKernelGGL((pr_update_csr<double, double, false, true>), dim3(gT), dim3(kTileThreads), 0, s,
-                               d64.rowptr.as<int>(), d64.colidx.as<int>(), d64.vals.as<double>(), xin,
-                               d64.tiles.as<int>(), d64.rp0, d64.f_col, nullptr, member, xout, st,
-                               w.pmax.as<double>());
-          else
-            hipLaunchKernelGGL((pr_update_csr<double, double, false, false>), dim3(gT), dim3(kTileThreads), 0, s,
-                               d64.rowptr.as<int>(), d64.colidx.as<int>(), d64.vals.as<double>(), xin,
-                               d64.tiles.as<int>(), d64.rp0, d64.f_col, nullptr, member, xout, st,
-                               w.pmax.as<double>());
+          launch_update<double, double, false>(pers, d64, xin, nullptr, member, xout, st, slots, s);
+          fin_max();
           hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
                              nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
                              w.pc.as<double>());
+          fin_norm();
         }
       } else {
         // any format: y = 0 (accumulating formats), y += A b_t, update, normalise
         hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, xout, n, 0.0);
         if (!d64.noop) launch_spmv(d64, xin, Dtype::F64, xout, Dtype::F64, 0, s);
-        if (p.personalized)
-          hipLaunchKernelGGL(pr_update_vec<true>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st,
-                             w.pmax.as<double>());
+        if (pers)
+          hipLaunchKernelGGL(pr_update_vec<true>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st, slots);
         else
-          hipLaunchKernelGGL(pr_update_vec<false>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st,
-                             w.pmax.as<double>());
+          hipLaunchKernelGGL(pr_update_vec<false>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st, slots);
+        fin_max();
         hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
                            nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
                            w.pc.as<double>());
+        fin_norm();
       }
     }
     TBSLA_HIP_CHECK(hipGetLastError());
@@ -595,7 +567,9 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   const double *a = mixed ? w.xhat.as<double>() : X[it & 1];
   const float *dd = mixed ? D[it & 1] : nullptr;
   hipLaunchKernelGGL(pr_sum_kernel, dim3(gN), dim3(kTileThreads), 0, s, a, dd, n, w.ps.as<double>(),
-                     w.pc.as<double>(), w.total.as<double>(), w.counter.as<unsigned>());
+                     w.pc.as<double>());
+  hipLaunchKernelGGL(pr_sum_finalize, dim3(1), dim3(kTileThreads), 0, s, w.ps.as<double>(), w.pc.as<double>(),
+                     static_cast<int>(gN), w.total.as<double>());
   hipLaunchKernelGGL(pr_scale_kernel, dim3(gN), dim3(256), 0, s, a, dd, n, w.total.as<double>(), out_dev);
   TBSLA_HIP_CHECK(hipGetLastError());
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));

@@ -1,12 +1,13 @@
 // SpMV kernels for gfx950 (MI355X, CDNA4, 64-wide wavefronts).
 //
-// csr_tile_kernel   one workgroup per row tile (<= 2048 entries, <= 256 rows).
-//                   Entries are read with coalesced non-temporal loads (8 per
-//                   thread in flight), multiplied by the gathered x, and staged
-//                   in LDS; each row is then summed by one thread in entry
-//                   order.  With FP64 this reproduces the reference loop
+// csr_tile_kernel   one workgroup per fixed 2048-entry tile.  Entries are
+//                   read with 16-byte loads (8 per thread in flight),
+//                   multiplied by the gathered x, and staged in LDS; each row
+//                   that starts in the tile is then summed by one thread in
+//                   entry order.  With FP64 this reproduces the reference loop
 //                   MatrixCSR.cpp:133-139 bit for bit (product rounded, then a
 //                   left-to-right sum starting from 0).
+// csr_long_kernel   rows longer than 1024 entries, one workgroup each.
 // ell_kernel        one thread per row over the column-major slots, 4 slots
 //                   per step; same order as MatrixELL.cpp:105-113 (contraction
 //                   off, so bit-identical in FP64).
@@ -65,15 +66,19 @@ struct StoreRow {
 };
 
 template <typename VT, typename XT, typename YT>
-__global__ __launch_bounds__(kTileThreads) void csr_tile_kernel(const int *__restrict__ rowptr,
-                                                                const int *__restrict__ colidx,
-                                                                const VT *__restrict__ vals,
-                                                                const XT *__restrict__ x, YT *__restrict__ y,
-                                                                const int *__restrict__ tiles, int rp0,
-                                                                int f_col) {
+__global__ __launch_bounds__(kTileThreads) void csr_tile_kernel(dev::CsrView<VT> m, const XT *__restrict__ x,
+                                                                YT *__restrict__ y) {
   __shared__ dev::CsrTileLds<typename Val<VT>::A> lds;
   StoreRow<YT> epi{y};
-  dev::csr_tile_rows<VT, XT>(rowptr, colidx, vals, x, tiles, rp0, f_col, lds, epi);
+  dev::csr_tile_rows<VT, XT>(m, x, lds, epi);
+}
+
+template <typename VT, typename XT, typename YT>
+__global__ __launch_bounds__(kTileThreads) void csr_long_kernel(dev::CsrView<VT> m, const XT *__restrict__ x,
+                                                                YT *__restrict__ y) {
+  __shared__ dev::CsrTileLds<typename Val<VT>::A> lds;
+  StoreRow<YT> epi{y};
+  dev::csr_long_row<VT, XT>(m, x, lds, epi);
 }
 
 template <typename VT, typename XT, typename YT>
@@ -205,14 +210,15 @@ inline unsigned blocks_for(long n, int per_block) { return static_cast<unsigned>
 
 template <typename VT, typename XT, typename YT>
 void run_typed(const DeviceMatrix &d, const XT *x, YT *y, int vect_incr, hipStream_t s) {
-  const VT *vals = nullptr;
   switch (d.format) {
-    case cpp::FORMAT_CSR:
-      vals = d.vals.as<VT>();
+    case cpp::FORMAT_CSR: {
+      const dev::CsrView<VT> m = dev::csr_view<VT>(d);
       if (d.ntiles > 0)
-        hipLaunchKernelGGL((csr_tile_kernel<VT, XT, YT>), dim3(d.ntiles), dim3(kTileThreads), 0, s,
-                           d.rowptr.as<int>(), d.colidx.as<int>(), vals, x, y, d.tiles.as<int>(), d.rp0, d.f_col);
+        hipLaunchKernelGGL((csr_tile_kernel<VT, XT, YT>), dim3(d.ntiles), dim3(kTileThreads), 0, s, m, x, y);
+      if (d.nlong > 0)
+        hipLaunchKernelGGL((csr_long_kernel<VT, XT, YT>), dim3(d.nlong), dim3(kTileThreads), 0, s, m, x, y);
       break;
+    }
     case cpp::FORMAT_ELL:
       if (d.rows > 0)
         hipLaunchKernelGGL((ell_kernel<VT, XT, YT>), dim3(blocks_for(d.rows, 256)), dim3(256), 0, s,

@@ -1,5 +1,15 @@
 // Device building blocks shared by spmv.hip and pagerank.hip (device code;
 // include only from .hip translation units).
+//
+// CSR products use fixed entry tiles: tile t holds entries [t*T, (t+1)*T)
+// (T = kCsrTileNnz) and owns the rows whose first entry lies in it
+// (tile_row[t] .. tile_row[t+1]).  A tile's entries are read with 16-byte
+// loads at an aligned offset that needs no descriptor load, the products
+// v * x[c] are rounded and staged in LDS, and each owned row is summed by one
+// thread from 0, left to right, finishing a row that runs past the tile from
+// global memory in the same order.  That reproduces the sequential FP64 row
+// sum of MatrixCSR.cpp:133-139 exactly.  Rows longer than kCsrLongRow entries
+// are left to csr_long_row (one workgroup per row, fixed-order tree).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,24 +24,50 @@ struct bf16 {
   unsigned short bits;
 };
 
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned short v4us __attribute__((ext_vector_type(4)));
+
 template <typename T>
 struct Val;
 template <>
 struct Val<double> {
   using A = double;
-  __device__ static double get(const double *p, long i) { return __builtin_nontemporal_load(p + i); }
+  __device__ static double get(const double *p, long i) { return p[i]; }
+  // four consecutive values starting at a multiple of 4
+  __device__ static void get4(const double *p, long i, double (&v)[4]) {
+    const v2d a = *reinterpret_cast<const v2d *>(p + i);
+    const v2d b = *reinterpret_cast<const v2d *>(p + i + 2);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+  }
 };
 template <>
 struct Val<float> {
   using A = float;
-  __device__ static float get(const float *p, long i) { return __builtin_nontemporal_load(p + i); }
+  __device__ static float get(const float *p, long i) { return p[i]; }
+  __device__ static void get4(const float *p, long i, float (&v)[4]) {
+    const v4f a = *reinterpret_cast<const v4f *>(p + i);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+  }
 };
 template <>
 struct Val<bf16> {
   using A = float;
-  __device__ static float get(const bf16 *p, long i) {
-    unsigned short b = __builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p) + i);
-    return __uint_as_float(static_cast<unsigned>(b) << 16);
+  __device__ static float cvt(unsigned short b) { return __uint_as_float(static_cast<unsigned>(b) << 16); }
+  __device__ static float get(const bf16 *p, long i) { return cvt(reinterpret_cast<const unsigned short *>(p)[i]); }
+  __device__ static void get4(const bf16 *p, long i, float (&v)[4]) {
+    const v4us a = *reinterpret_cast<const v4us *>(reinterpret_cast<const unsigned short *>(p) + i);
+    v[0] = cvt(a.x);
+    v[1] = cvt(a.y);
+    v[2] = cvt(a.z);
+    v[3] = cvt(a.w);
   }
 };
 
@@ -64,122 +100,151 @@ __device__ inline AT block_tree_max(AT v, AT *part) {
   return r;
 }
 
-// LDS of one CSR tile workgroup.
+// Device view of a CSR mirror (entry j of the matrix is colidx[j], vals[j],
+// with j = rowptr[i] - rp0 + k for the k-th entry of row i).
+template <typename VT>
+struct CsrView {
+  const int *rowptr;
+  const int *colidx;
+  const VT *vals;  // nullptr for the value-free view
+  const int *tile_row;
+  const int *long_rows;
+  long nnz;
+  int rp0, f_col;
+};
+
+template <typename VT>
+inline CsrView<VT> csr_view(const DeviceMatrix &d) {
+  return CsrView<VT>{d.rowptr.as<int>(), d.colidx.as<int>(), d.vals.as<VT>(), d.tiles.as<int>(),
+                     d.long_rows.as<int>(), d.entries, d.rp0, d.f_col};
+}
+
 template <typename AT>
 struct CsrTileLds {
   AT prod[kCsrTileNnz];
-  int rp[kCsrTileRows + 1];
 };
 
-// Row sums of one CSR tile (rows [tiles[b], tiles[b+1])).  Products are
-// rounded and staged in LDS, then each row is summed left to right from 0 by
-// one thread, which reproduces the sequential FP64 row sum exactly.  A tile
-// holding one row longer than kCsrTileNnz is summed chunk by chunk with a
-// fixed-order tree.  epi(row, sum) is called once per row, by the thread that
-// owns it (thread 0 for a long row).  Templated on the x element type XT,
-// whose values are converted to the accumulation type.
-template <typename VT, typename XT, typename Epi>
-__device__ inline void csr_tile_rows(const int *__restrict__ rowptr, const int *__restrict__ colidx,
-                                     const VT *__restrict__ vals, const XT *__restrict__ x,
-                                     const int *__restrict__ tiles, int rp0, int f_col,
-                                     CsrTileLds<typename Val<VT>::A> &lds, Epi &epi) {
+// Value source: stored values (VT) or value-free (every product is z[c]).
+template <typename VT, typename XT>
+struct Prod {
   using AT = typename Val<VT>::A;
+  __device__ static void group(const CsrView<VT> &m, const XT *__restrict__ x, long e, const int (&c)[4],
+                               AT (&p)[4]) {
+    AT v[4];
+    Val<VT>::get4(m.vals, e, v);
+#pragma unroll
+    for (int q = 0; q < 4; q++) p[q] = v[q] * static_cast<AT>(x[c[q]]);
+  }
+  __device__ static AT one(const CsrView<VT> &m, const XT *__restrict__ x, long e) {
+    const AT v = Val<VT>::get(m.vals, e);
+    const AT xv = static_cast<AT>(x[m.colidx[e] - m.f_col]);
+    return v * xv;
+  }
+};
+struct Pattern {};
+template <>
+struct Val<Pattern> {
+  using A = double;
+};
+template <typename XT>
+struct Prod<Pattern, XT> {
+  using AT = double;
+  __device__ static void group(const CsrView<Pattern> &, const XT *__restrict__ z, long, const int (&c)[4],
+                               double (&p)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) p[q] = static_cast<double>(z[c[q]]);
+  }
+  __device__ static double one(const CsrView<Pattern> &m, const XT *__restrict__ z, long e) {
+    return static_cast<double>(z[m.colidx[e] - m.f_col]);
+  }
+};
+
+// Rows of one entry tile (blockIdx.x); epi(row, sum) once per owned row by
+// the thread that sums it.
+template <typename VT, typename XT, typename Epi>
+__device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict__ x,
+                                     CsrTileLds<typename Val<VT>::A> &lds, Epi &epi) {
+#pragma clang fp contract(off)
+  using AT = typename Val<VT>::A;
+  constexpr int G = kCsrTileNnz / 4 / kTileThreads;  // 4-entry groups per thread
   const int tid = threadIdx.x;
-  const int r0 = tiles[blockIdx.x], r1 = tiles[blockIdx.x + 1];
-  const int nr = r1 - r0;
-  for (int i = tid; i <= nr; i += kTileThreads) lds.rp[i] = rowptr[r0 + i] - rp0;
+  const long e0 = static_cast<long>(blockIdx.x) * kCsrTileNnz;
+  const int ne = static_cast<int>(min(static_cast<long>(kCsrTileNnz), m.nnz - e0));
+  int c[G][4];
+  AT p[G][4];
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const int g = k * kTileThreads + tid;
+    if (4 * g < ne) {
+      const v4i ci = *reinterpret_cast<const v4i *>(m.colidx + e0 + 4 * g);
+      c[k][0] = ci.x - m.f_col;
+      c[k][1] = ci.y - m.f_col;
+      c[k][2] = ci.z - m.f_col;
+      c[k][3] = ci.w - m.f_col;
+      // entries past the matrix end read the zeroed padding: column 0
+      for (int q = 0; q < 4; q++)
+        if (4 * g + q >= ne) c[k][q] = 0;
+    }
+  }
+  const int rlo = m.tile_row[blockIdx.x], rhi = m.tile_row[blockIdx.x + 1];
+  int a = 0, b = 0;
+  if (rlo + tid < rhi) {
+    a = m.rowptr[rlo + tid] - m.rp0;
+    b = m.rowptr[rlo + tid + 1] - m.rp0;
+  }
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const int g = k * kTileThreads + tid;
+    if (4 * g < ne) Prod<VT, XT>::group(m, x, e0 + 4 * g, c[k], p[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    const int g = k * kTileThreads + tid;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (4 * g + q < ne) lds.prod[4 * g + q] = p[k][q];
+  }
   __syncthreads();
-  const long e0 = lds.rp[0];
-  const int ne = lds.rp[nr] - lds.rp[0];
-  if (ne <= kCsrTileNnz) {
-    int c[kTilePer];
-    AT v[kTilePer];
+  const long e1 = e0 + ne;
+  for (int r = rlo + tid; r < rhi; r += kTileThreads) {
+    if (r != rlo + tid) {
+      a = m.rowptr[r] - m.rp0;
+      b = m.rowptr[r + 1] - m.rp0;
+    }
+    if (b - a > kCsrLongRow) continue;  // summed by csr_long_row
+    AT s = 0;
+    const int lend = static_cast<int>(min(static_cast<long>(b), e1) - e0);
+    for (int e = a - static_cast<int>(e0); e < lend; e++) s += lds.prod[e];
+    // the rest of a row that runs past the tile, same order, 8 loads in flight
+    for (long e = e1; e < b; e += 8) {
+      AT q8[8];
 #pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < ne) {
-        c[k] = nt_load(colidx, e0 + e) - f_col;
-        v[k] = Val<VT>::get(vals, e0 + e);
-      }
-    }
+      for (int u = 0; u < 8; u++)
+        if (e + u < b) q8[u] = Prod<VT, XT>::one(m, x, e + u);
 #pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < ne) lds.prod[e] = v[k] * static_cast<AT>(x[c[k]]);
+      for (int u = 0; u < 8; u++)
+        if (e + u < b) s += q8[u];
     }
-    __syncthreads();
-    for (int i = tid; i < nr; i += kTileThreads) {
-      AT s = 0;
-      const int a = lds.rp[i] - lds.rp[0], b = lds.rp[i + 1] - lds.rp[0];
-      for (int e = a; e < b; e++) s += lds.prod[e];
-      epi(r0 + i, s);
-    }
-    return;
+    epi(r, s);
   }
-  AT total = 0;
-  for (int base = 0; base < ne; base += kCsrTileNnz) {
-    const int cnt = min(kCsrTileNnz, ne - base);
-    AT acc = 0;
-#pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < cnt) {
-        const int cc = nt_load(colidx, e0 + base + e) - f_col;
-        acc += Val<VT>::get(vals, e0 + base + e) * static_cast<AT>(x[cc]);
-      }
-    }
-    total += block_tree_sum<AT>(acc, lds.prod);
-  }
-  if (tid == 0) epi(r0, total);
 }
 
-// Value-free variant for count-stochastic matrices: every value of column j
-// equals colscale[j] and z[j] = colscale[j] * x[j] was formed beforehand, so
-// the row sum is a plain sum of gathered z (bit-identical to values * x).
-template <typename Epi>
-__device__ inline void csr_pattern_tile_rows(const int *__restrict__ rowptr, const int *__restrict__ colidx,
-                                             const double *__restrict__ z, const int *__restrict__ tiles, int rp0,
-                                             int f_col, CsrTileLds<double> &lds, Epi &epi) {
-  const int tid = threadIdx.x;
-  const int r0 = tiles[blockIdx.x], r1 = tiles[blockIdx.x + 1];
-  const int nr = r1 - r0;
-  for (int i = tid; i <= nr; i += kTileThreads) lds.rp[i] = rowptr[r0 + i] - rp0;
-  __syncthreads();
-  const long e0 = lds.rp[0];
-  const int ne = lds.rp[nr] - lds.rp[0];
-  if (ne <= kCsrTileNnz) {
-    int c[kTilePer];
-#pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < ne) c[k] = nt_load(colidx, e0 + e) - f_col;
-    }
-#pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < ne) lds.prod[e] = z[c[k]];
-    }
-    __syncthreads();
-    for (int i = tid; i < nr; i += kTileThreads) {
-      double s = 0;
-      const int a = lds.rp[i] - lds.rp[0], b = lds.rp[i + 1] - lds.rp[0];
-      for (int e = a; e < b; e++) s += lds.prod[e];
-      epi(r0 + i, s);
-    }
-    return;
+// One long row (m.long_rows[blockIdx.x]) per workgroup: chunk products, a
+// fixed-order tree per chunk, chunk totals added in order.  epi by thread 0.
+template <typename VT, typename XT, typename Epi>
+__device__ inline void csr_long_row(const CsrView<VT> &m, const XT *__restrict__ x,
+                                    CsrTileLds<typename Val<VT>::A> &lds, Epi &epi) {
+  using AT = typename Val<VT>::A;
+  const int r = m.long_rows[blockIdx.x];
+  const long a = m.rowptr[r] - m.rp0, b = m.rowptr[r + 1] - m.rp0;
+  AT total = 0;
+  for (long base = a; base < b; base += kCsrTileNnz) {
+    const long cnt = min(static_cast<long>(kCsrTileNnz), b - base);
+    AT acc = 0;
+    for (long e = threadIdx.x; e < cnt; e += kTileThreads) acc += Prod<VT, XT>::one(m, x, base + e);
+    total += block_tree_sum<AT>(acc, lds.prod);
   }
-  double total = 0;
-  for (int base = 0; base < ne; base += kCsrTileNnz) {
-    const int cnt = min(kCsrTileNnz, ne - base);
-    double acc = 0;
-#pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < cnt) acc += z[nt_load(colidx, e0 + base + e) - f_col];
-    }
-    total += block_tree_sum<double>(acc, lds.prod);
-  }
-  if (tid == 0) epi(r0, total);
+  if (threadIdx.x == 0) epi(r, total);
 }
 
 }  // namespace dev

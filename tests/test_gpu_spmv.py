@@ -101,9 +101,14 @@ def test_unsorted_coo_is_exact(tb, oracle, gpu):
 
 
 def test_long_rows(tb, oracle, gpu):
-    """Rows longer than one LDS tile (2048 entries) take the chunked path."""
+    """Rows longer than 1024 entries take the one-workgroup-per-row path
+    (fixed-order tree, within 1e-13); rows up to 1024 entries that run past
+    their 2048-entry tile stay bit-exact."""
     n = 6000
-    rowptr = np.array([0, 5000, 5001, 5001, 17001] + [17001] * (n - 4), dtype=np.int32)
+    lens = np.zeros(n, dtype=np.int64)
+    lens[:12] = [5000, 1, 0, 12000, 1024, 1025, 700, 3, 1024, 2047, 2048, 1]
+    lens[12:40] = 37
+    rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     nnz = rowptr[-1]
     rng = np.random.default_rng(2)
     col = rng.integers(0, n, nnz).astype(np.int32)
@@ -113,6 +118,8 @@ def test_long_rows(tb, oracle, gpu):
     o = oracle.from_arrays("CSR", n, n, val, rowptr, col)
     y, yo = p.spmv(x), o.spmv(x)
     assert np.all(np.abs(y - yo) <= 1e-13 * np.abs(yo) + 1e-300)
+    short = lens <= 1024
+    assert np.array_equal(y[short], yo[short])
     rows = np.repeat(np.arange(n), np.diff(rowptr))
     coo = tb.MatrixCOO(n, n, val, rows.astype(np.int32), col)
     yc = coo.spmv(x)
