@@ -82,6 +82,15 @@ def cpu_baseline(tb_csr, x, bytes_alg):
                       f"{dt * 1e3:.1f} ms each"}
 
 
+def copy_calibration():
+    """Five copies of 2^27 doubles: 1 GiB read and 1 GiB written per launch."""
+    import torch
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    ms = time_kernel(torch, lambda: dst.copy_(src), 5, 1)
+    log(f"copy of {src.numel() * 8} bytes: {ms * 1e3:.1f} us, {2 * src.numel() * 8 / ms / 1e6:.0f} GB/s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,11 +99,17 @@ def main():
     ap.add_argument("--no-pagerank", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-formats", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (the reference kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the 2D block "
-              "distribution is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU's block: there is no multi-GPU path (sharding over the "
+              "2D block grid is not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import torch
     import tbsla_amd as tb

@@ -10,7 +10,7 @@
  * summation orders follow the reference so that results are bit-identical
  * to a single-threaded reference run when compiled with -ffp-contract=off.
  *
- * Pinning (what checks each part; see tests/test_oracle_*.py):
+ * Pinning (what checks each part; tests/test_oracle.py):
  *   lnv/pflv           brute-force partition invariants + the compiled reference
  *   cdiag generators   closed form compute_spmv_res_cdiag + compiled reference arrays
  *   cqmat generators   compiled reference arrays (bit-exact) + golden SHA-256 digests

@@ -215,3 +215,23 @@ def test_cdiag_10000_config(tb, golden, gpu):
     reference's own product (fixture)."""
     y = tb.MatrixCSR().fill_cdiag(10000, 10000, 10).spmv(golden["cdiag10000_x"])
     assert np.array_equal(y, golden["cdiag10000_y"])
+
+
+def test_cdiag_10000_config_literal_band(tb, oracle, gpu):
+    """The literal reading of the 10,000-row configuration: ten diagonals at
+    offsets 0..9 (columns i..i+9 clipped to the matrix), values 1.0, built
+    through the MatrixCSR array constructor (99,955 entries).  Equal to the
+    oracle, and to the closed form sum(x[i:i+10]) summed left to right."""
+    n = 10000
+    cols = [np.arange(i, min(i + 10, n)) for i in range(n)]
+    rowptr = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int32)
+    colidx = np.concatenate(cols).astype(np.int32)
+    assert rowptr[-1] == 99955
+    val = np.ones(rowptr[-1])
+    x = oracle.uniform_vector(1, n)
+    y = tb.MatrixCSR(n, n, val, rowptr, colidx).spmv(x)
+    assert np.array_equal(y, oracle.from_arrays("CSR", n, n, val, rowptr, colidx).spmv(x))
+    closed = np.zeros(n)
+    for k in range(10):
+        closed[: n - k] += x[k:]
+    assert np.array_equal(y, closed)
