@@ -105,6 +105,44 @@ typedef struct {
 int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, void *stream,
                         tbsla_pagerank_report *report);
 
+/* PageRank with explicit device inputs (CSR, FP64): values_dev (n_values =
+ * the stored entry count, CSR entry order) replaces the stored edge values,
+ * teleport_dev (n_teleport = n_col) is the teleport distribution v, so that
+ * the iteration is b = beta*A b + (1-beta)*(1^T b)*v.  Either may be NULL
+ * (stored values / the reference's uniform teleport).  No reference
+ * counterpart: the forward pass of the differentiable PageRank. */
+int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, const double *values_dev,
+                           size_t n_values, const double *teleport_dev, size_t n_teleport, double *out_dev,
+                           void *stream, tbsla_pagerank_report *report);
+
+typedef struct {
+  double beta;
+  double epsilon;    /* stop when |lambda_k - lambda_{k-1}|_1 <= epsilon * |lambda_k|_1 */
+  int max_iteration;
+  int precision;     /* TBSLA_F32: FP32 transposed products; TBSLA_F64 */
+  int check_every;   /* iterations between convergence reads */
+} tbsla_adjoint_params;
+
+typedef struct {
+  int iterations;
+  double last_change;
+  double s;          /* 1^T M p */
+  double seconds;
+} tbsla_adjoint_report;
+
+/* Gradient of L(p) through the PageRank fixed point p = M p / 1^T M p,
+ * M = beta*A + (1-beta) v 1^T, given p (pr_dev, the forward result, n_col
+ * doubles) and w = dL/dp (w_dev).  Writes dL/dA per stored CSR entry
+ * (grad_values_dev, n_values doubles) and dL/dv (grad_teleport_dev, n_col);
+ * either output may be NULL.  values_dev / teleport_dev as in
+ * tbsla_hip_page_rank_ex.  Implicit adjoint fixed point
+ * lambda = M^T (lambda - (p^T lambda) 1)/s + w on the transposed CSR
+ * structure.  No reference counterpart. */
+int tbsla_hip_page_rank_adjoint(tbsla_matrix *m, const tbsla_adjoint_params *p, const double *values_dev,
+                                size_t n_values, const double *teleport_dev, size_t n_teleport,
+                                const double *pr_dev, const double *w_dev, size_t n, double *grad_values_dev,
+                                double *grad_teleport_dev, void *stream, tbsla_adjoint_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,16 @@ class PageRankReport(C.Structure):
                 ("bound", C.c_double), ("seconds", C.c_double)]
 
 
+class AdjointParams(C.Structure):
+    _fields_ = [("beta", C.c_double), ("epsilon", C.c_double), ("max_iteration", C.c_int),
+                ("precision", C.c_int), ("check_every", C.c_int)]
+
+
+class AdjointReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("last_change", C.c_double), ("s", C.c_double),
+                ("seconds", C.c_double)]
+
+
 # Every entry point the headers declare, with its ctypes signature.
 _vp, _i, _u, _d, _sz, _l = C.c_void_p, C.c_int, C.c_uint, C.c_double, C.c_size_t, C.c_long
 _dp = C.POINTER(C.c_double)
@@ -67,6 +77,10 @@ SIGNATURES = {
     "tbsla_hip_Ax": (_i, [_vp, _i, _vp, _sz, _i, _vp, _sz, _i, _i, _vp]),
     "tbsla_hip_fill": (_i, [_vp, _i, _sz, _d, _vp]),
     "tbsla_hip_page_rank": (_i, [_vp, C.POINTER(PageRankParams), _dp, _vp, C.POINTER(PageRankReport)]),
+    "tbsla_hip_page_rank_ex": (_i, [_vp, C.POINTER(PageRankParams), _vp, _sz, _vp, _sz, _vp, _vp,
+                                    C.POINTER(PageRankReport)]),
+    "tbsla_hip_page_rank_adjoint": (_i, [_vp, C.POINTER(AdjointParams), _vp, _sz, _vp, _sz, _vp, _vp, _sz, _vp,
+                                         _vp, _vp, C.POINTER(AdjointReport)]),
     # reference C ABI (interface/c): declared for the export check
     "C_CPP_Vector_create": (_vp, []),
     "C_CPP_Vector_destroy": (None, [_vp]),

@@ -17,6 +17,7 @@
 
 #include "../../../include/tbsla_cpp_c.h"
 #include "../../../include/tbsla_hip.h"
+#include "../device/adjoint.hpp"
 #include "../device/kernels.hpp"
 #include "../device/mirror.hpp"
 #include "../device/pagerank.hpp"
@@ -492,9 +493,62 @@ int tbsla_hip_fill(void *y, int y_dtype, size_t n, double value, void *stream) {
 
 int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, void *stream,
                         tbsla_pagerank_report *report) {
+  return tbsla_hip_page_rank_ex(m, p, nullptr, 0, nullptr, 0, out_dev, stream, report);
+}
+
+namespace {
+// Stored CSR entries (rowptr[rows] - rowptr[0]), the length of per-entry arrays.
+size_t csr_entries(const cpp::Matrix &m) {
+  if (m.format() != cpp::FORMAT_CSR) throw std::invalid_argument("tbsla: per-entry arrays need a CSR matrix");
+  const auto &rp = dynamic_cast<const cpp::MatrixCSR &>(m).get_rowptr();
+  return rp.size() < 2 ? 0 : static_cast<size_t>(rp.back() - rp.front());
+}
+}  // namespace
+
+int tbsla_hip_page_rank_adjoint(tbsla_matrix *m, const tbsla_adjoint_params *p, const double *values_dev,
+                                size_t n_values, const double *teleport_dev, size_t n_teleport,
+                                const double *pr_dev, const double *w_dev, size_t n, double *grad_values_dev,
+                                double *grad_teleport_dev, void *stream, tbsla_adjoint_report *report) {
+  if (!m || !p || !pr_dev || !w_dev) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    const size_t ncol = static_cast<size_t>(m->m->get_n_col());
+    const size_t ne = csr_entries(*m->m);
+    if (n < ncol) throw std::invalid_argument("tbsla: PageRank vector / weight shorter than n_col");
+    if (values_dev && n_values < ne) throw std::invalid_argument("tbsla: values shorter than the stored entries");
+    if (teleport_dev && n_teleport < ncol) throw std::invalid_argument("tbsla: teleport shorter than n_col");
+    if (p->precision != TBSLA_F32 && p->precision != TBSLA_F64)
+      throw std::invalid_argument("tbsla: adjoint precision must be F32 or F64");
+    hip::AdjointParams q;
+    q.beta = p->beta;
+    q.epsilon = p->epsilon;
+    q.max_iteration = p->max_iteration;
+    q.precision = p->precision == TBSLA_F32 ? hip::Dtype::F32 : hip::Dtype::F64;
+    q.check_every = p->check_every > 0 ? p->check_every : 8;
+    q.values = values_dev;
+    q.teleport = teleport_dev;
+    hip::AdjointReport r = hip::page_rank_adjoint(*m->m, q, pr_dev, w_dev, grad_values_dev, grad_teleport_dev,
+                                                  static_cast<hipStream_t>(stream));
+    if (report) {
+      report->iterations = r.iterations;
+      report->last_change = r.last_change;
+      report->s = r.s;
+      report->seconds = r.seconds;
+    }
+  });
+}
+
+int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, const double *values_dev,
+                           size_t n_values, const double *teleport_dev, size_t n_teleport, double *out_dev,
+                           void *stream, tbsla_pagerank_report *report) {
   if (!m || !p || !out_dev) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
   return guarded([&] {
+    if (values_dev && n_values < csr_entries(*m->m))
+      throw std::invalid_argument("tbsla: values shorter than the stored entries");
+    if (teleport_dev && n_teleport < static_cast<size_t>(m->m->get_n_col()))
+      throw std::invalid_argument("tbsla: teleport shorter than n_col");
     hip::PageRankParams q;
+    q.values = values_dev;
+    q.teleport = teleport_dev;
     q.epsilon = p->epsilon;
     q.beta = p->beta;
     q.max_iteration = p->max_iteration;

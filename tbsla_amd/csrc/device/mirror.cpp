@@ -281,6 +281,56 @@ std::size_t DeviceMatrix::algorithmic_bytes(std::size_t xe, std::size_t ye) cons
   return 0;
 }
 
+const DeviceMatrix &transpose_mirror(const cpp::Matrix &m) {
+  if (m.format() != cpp::FORMAT_CSR) bad("transposed products need a CSR matrix");
+  const auto &c = dynamic_cast<const cpp::MatrixCSR &>(m);
+  DeviceCache &cache = m.device_cache();
+  std::lock_guard<std::mutex> g(cache.mu);
+  if (cache.transposed) return *cache.transposed;
+  const auto &rp = c.get_rowptr();
+  const auto &ci = c.get_colidx();
+  auto d = std::unique_ptr<DeviceMatrix>(new DeviceMatrix());
+  d->format = cpp::FORMAT_CSR;
+  d->vdt = Dtype::PATTERN;
+  const int rows = rp.size() < 2 ? 0 : static_cast<int>(rp.size() - 1);
+  const int cols = c.get_ln_col();
+  const long used = rows ? static_cast<long>(rp.back()) - rp[0] : 0;
+  if (used > static_cast<long>(ci.size())) bad("CSR rowptr points past colidx");
+  std::vector<int> ptr(static_cast<std::size_t>(cols) + 1, 0);
+  for (long k = 0; k < used; k++) {
+    const long j = static_cast<long>(ci[static_cast<std::size_t>(k)]) - c.get_f_col();
+    if (j < 0 || j >= cols) bad("CSR column index outside the block");
+    ptr[static_cast<std::size_t>(j) + 1]++;
+  }
+  for (int j = 0; j < cols; j++) ptr[static_cast<std::size_t>(j) + 1] += ptr[static_cast<std::size_t>(j)];
+  std::vector<int> next(ptr.begin(), ptr.end() - 1), ridx(static_cast<std::size_t>(used)),
+      perm(static_cast<std::size_t>(used));
+  for (int i = 0; i < rows; i++)
+    for (long k = rp[static_cast<std::size_t>(i)] - rp[0]; k < rp[static_cast<std::size_t>(i) + 1] - rp[0]; k++) {
+      const int j = ci[static_cast<std::size_t>(k)] - c.get_f_col();
+      const int pos = next[static_cast<std::size_t>(j)]++;
+      ridx[static_cast<std::size_t>(pos)] = i;
+      perm[static_cast<std::size_t>(pos)] = static_cast<int>(k);
+    }
+  d->rows = cols;
+  d->entries = used;
+  d->x_need = rows;
+  d->rowptr.upload(ptr.data(), ptr.size() * sizeof(int));
+  d->colidx.upload(ridx.data(), ridx.size() * sizeof(int));
+  d->perm.upload(perm.data(), perm.size() * sizeof(int));
+  std::vector<int> t = csr_entry_tiles(ptr.data(), cols, kCsrTileNnz);
+  d->ntiles = static_cast<int>(t.size()) - 1;
+  d->tiles.upload(t.data(), t.size() * sizeof(int));
+  std::vector<int> lr;
+  for (int j = 0; j < cols; j++)
+    if (ptr[static_cast<std::size_t>(j) + 1] - ptr[static_cast<std::size_t>(j)] > kCsrLongRow) lr.push_back(j);
+  d->nlong = static_cast<int>(lr.size());
+  if (d->nlong) d->long_rows.upload(lr.data(), lr.size() * sizeof(int));
+  TBSLA_HIP_CHECK(hipDeviceSynchronize());
+  cache.transposed = std::move(d);
+  return *cache.transposed;
+}
+
 const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt) {
   DeviceCache &cache = m.device_cache();
   std::lock_guard<std::mutex> g(cache.mu);

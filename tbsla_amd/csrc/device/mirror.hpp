@@ -55,17 +55,26 @@ struct DeviceMatrix {
   DevBuf dvals;
   // count-stochastic CSR: 1/count of each local column
   DevBuf colscale;
+  // transposed CSR structure: CSR entry index of each transposed entry
+  DevBuf perm;
   std::size_t algorithmic_bytes(std::size_t x_elem, std::size_t y_elem) const;
 };
 
 struct DeviceCache {
   std::mutex mu;
   std::unique_ptr<DeviceMatrix> mirrors[4];
+  std::unique_ptr<DeviceMatrix> transposed;
   DevBuf x_stage, y_stage;
 };
 
 // Build (or fetch) the mirror of `m` with values of type vdt.
 const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt);
+
+// Value-free CSR structure of A^T for a CSR matrix A (rows = A's local
+// columns, columns = A's local rows): row j lists the entries of column j of
+// A in increasing row order, and perm[k] is the CSR entry index (relative to
+// rowptr[0]) of transposed entry k, so values_T[k] = values[perm[k]].
+const DeviceMatrix &transpose_mirror(const cpp::Matrix &m);
 
 // Host tile tables (exposed for tests and the autotuner).
 // tile_row of the CSR entry tiles (ntiles + 1 entries, ntiles = nnz / T + 1).

@@ -82,18 +82,33 @@ __device__ inline CSum block_csum(CSum v, double *ps, double *pc) {
   return r;
 }
 
-// Per-row update epilogue: b = beta*(yhat + s) + tele (MIXED) or beta*s + tele.
-template <typename AT, bool MIXED, bool PERSONAL>
+// Teleport term of row i, by mode:
+//   TM = 0  uniform: + tele on every row (tele = (1-beta)/n * sum b_t)
+//   TM = 1  node set: + tele on listed rows (tele = (1-beta)/|nodes| * sum b_t)
+//   TM = 2  distribution v: + tele * v[i] (tele = (1-beta) * sum b_t)
+struct Teleport {
+  const unsigned char *member;
+  const double *v;
+};
+template <int TM>
+__device__ inline double add_teleport(double bi, double tele, const Teleport &tp, long i) {
+#pragma clang fp contract(off)
+  if (TM == 0) return bi + tele;
+  if (TM == 1) return tp.member[i] ? bi + tele : bi;
+  return bi + tele * tp.v[i];
+}
+
+// Per-row update epilogue: b = beta*(yhat + s) + teleport (MIXED) or beta*s + teleport.
+template <typename AT, bool MIXED, int TM>
 struct UpdateRow {
   double *b;
   const double *yhat;
-  const unsigned char *member;
+  Teleport tp;
   double beta, tele, lmax;
   __device__ void operator()(int row, AT s) {
 #pragma clang fp contract(off)
     const double v = MIXED ? yhat[row] + static_cast<double>(s) : static_cast<double>(s);
-    double bi = beta * v;
-    if (!PERSONAL || member[row]) bi = bi + tele;
+    const double bi = add_teleport<TM>(beta * v, tele, tp, row);
     b[row] = bi;
     lmax = fmax(lmax, bi);
   }
@@ -120,17 +135,16 @@ __device__ inline void slot_max(unsigned long long *slots, double v, double *red
 // Fused update over the CSR tiles (LONG = false) or the long rows (LONG =
 // true): b = beta * (A x) + tele, block max into the slots.  VT = dev::Pattern
 // is the value-free product over z = colscale .* b_t.
-template <typename VT, typename XT, bool MIXED, bool PERSONAL, bool LONG>
+template <typename VT, typename XT, bool MIXED, int TM, bool LONG>
 __global__ __launch_bounds__(kTileThreads) void pr_update_csr(dev::CsrView<VT> m, const XT *__restrict__ x,
-                                                              const double *__restrict__ yhat,
-                                                              const unsigned char *__restrict__ member,
+                                                              const double *__restrict__ yhat, Teleport tp,
                                                               double *__restrict__ b, PrState *st,
                                                               unsigned long long *slots) {
   if (st->done) return;
   using AT = typename dev::Val<VT>::A;
   __shared__ dev::CsrTileLds<AT> lds;
   __shared__ double red[kTileThreads];
-  UpdateRow<AT, MIXED, PERSONAL> epi{b, yhat, member, st->beta, st->tele, -INFINITY};
+  UpdateRow<AT, MIXED, TM> epi{b, yhat, tp, st->beta, st->tele, -INFINITY};
   if (LONG)
     dev::csr_long_row<VT, XT>(m, x, lds, epi);
   else
@@ -139,10 +153,9 @@ __global__ __launch_bounds__(kTileThreads) void pr_update_csr(dev::CsrView<VT> m
 }
 
 // Update for any format after a plain product y = A b_t (generic engine).
-template <bool PERSONAL>
-__global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict__ b, long n,
-                                                              const unsigned char *__restrict__ member, PrState *st,
-                                                              unsigned long long *slots) {
+template <int TM>
+__global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict__ b, long n, Teleport tp,
+                                                              PrState *st, unsigned long long *slots) {
 #pragma clang fp contract(off)
   if (st->done) return;
   __shared__ double red[kTileThreads];
@@ -150,8 +163,7 @@ __global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict
   double lmax = -INFINITY;
   for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * kTileThreads) {
-    double bi = beta * b[i];
-    if (!PERSONAL || member[i]) bi = bi + tele;
+    const double bi = add_teleport<TM>(beta * b[i], tele, tp, i);
     b[i] = bi;
     lmax = fmax(lmax, bi);
   }
@@ -346,25 +358,29 @@ __global__ __launch_bounds__(256) void scale_into(const double *__restrict__ s, 
 
 unsigned grid_for(long n) { return static_cast<unsigned>(std::max<long>(1, std::min<long>(kNormBlocks, (n + 255) / 256))); }
 
-template <typename VT, typename XT, bool MIXED, bool PERSONAL>
-void launch_update(const DeviceMatrix &d, const XT *x, const double *yhat, const unsigned char *member, double *b,
-                   PrState *st, unsigned long long *slots, hipStream_t s) {
-  const dev::CsrView<VT> m = dev::csr_view<VT>(d);
+template <typename VT, typename XT, bool MIXED, int TM>
+void launch_update(const DeviceMatrix &d, const VT *vals, const XT *x, const double *yhat, const Teleport &tp,
+                   double *b, PrState *st, unsigned long long *slots, hipStream_t s) {
+  dev::CsrView<VT> m = dev::csr_view<VT>(d);
+  if (vals) m.vals = vals;
   if (d.ntiles > 0)
-    hipLaunchKernelGGL((pr_update_csr<VT, XT, MIXED, PERSONAL, false>), dim3(d.ntiles), dim3(kTileThreads), 0, s, m,
-                       x, yhat, member, b, st, slots);
+    hipLaunchKernelGGL((pr_update_csr<VT, XT, MIXED, TM, false>), dim3(d.ntiles), dim3(kTileThreads), 0, s, m, x,
+                       yhat, tp, b, st, slots);
   if (d.nlong > 0)
-    hipLaunchKernelGGL((pr_update_csr<VT, XT, MIXED, PERSONAL, true>), dim3(d.nlong), dim3(kTileThreads), 0, s, m, x,
-                       yhat, member, b, st, slots);
+    hipLaunchKernelGGL((pr_update_csr<VT, XT, MIXED, TM, true>), dim3(d.nlong), dim3(kTileThreads), 0, s, m, x, yhat,
+                       tp, b, st, slots);
 }
 
+// Values override `vals` (nullptr: the mirror's own values).
 template <typename VT, typename XT, bool MIXED>
-void launch_update(bool personal, const DeviceMatrix &d, const XT *x, const double *yhat,
-                   const unsigned char *member, double *b, PrState *st, unsigned long long *slots, hipStream_t s) {
-  if (personal)
-    launch_update<VT, XT, MIXED, true>(d, x, yhat, member, b, st, slots, s);
+void launch_update(int tmode, const DeviceMatrix &d, const VT *vals, const XT *x, const double *yhat,
+                   const Teleport &tp, double *b, PrState *st, unsigned long long *slots, hipStream_t s) {
+  if (tmode == 1)
+    launch_update<VT, XT, MIXED, 1>(d, vals, x, yhat, tp, b, st, slots, s);
+  else if (tmode == 2)
+    launch_update<VT, XT, MIXED, 2>(d, vals, x, yhat, tp, b, st, slots, s);
   else
-    launch_update<VT, XT, MIXED, false>(d, x, yhat, member, b, st, slots, s);
+    launch_update<VT, XT, MIXED, 0>(d, vals, x, yhat, tp, b, st, slots, s);
 }
 
 template <typename VT>
@@ -404,6 +420,8 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     throw std::invalid_argument("tbsla: value-free PageRank needs values from fill_cqmat_stochastic");
   if (mixed && p.rule != PrRule::SUMNORM)
     throw std::invalid_argument("tbsla: mixed-precision PageRank uses the SUMNORM stopping rule");
+  if (p.values && (!csr || p.precision != PrPrecision::F64))
+    throw std::invalid_argument("tbsla: explicit edge values need a CSR matrix and FP64 precision");
 
   const DeviceMatrix &d64 = mirror(m, Dtype::F64);
   const DeviceMatrix *dlo = nullptr;
@@ -424,14 +442,17 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   w.total.alloc(sizeof(double));
   w.X0.alloc(sizeof(double) * n);
   w.X1.alloc(sizeof(double) * n);
-  unsigned char *member = nullptr;
-  const double b0 = p.personalized ? 0.25 : 1.0;
-  if (p.personalized) {
-    std::vector<unsigned char> mem(static_cast<std::size_t>(n), 0);
+  // teleport mode: 0 uniform, 1 node set (Matrix.cpp:63-114), 2 distribution vector
+  const int tmode = p.teleport ? 2 : (p.personalized ? 1 : 0);
+  Teleport tp{nullptr, p.teleport};
+  const double b0 = p.personalized && !p.teleport ? 0.25 : 1.0;
+  std::vector<unsigned char> mem;
+  if (tmode == 1) {
+    mem.assign(static_cast<std::size_t>(n), 0);
     for (int k = 0; k < p.n_nodes; k++)
       if (p.nodes[k] >= 0 && p.nodes[k] < n) mem[static_cast<std::size_t>(p.nodes[k])] = 1;
     w.member.upload(mem.data(), mem.size(), s);
-    member = w.member.as<unsigned char>();
+    tp.member = w.member.as<unsigned char>();
   }
   const bool pattern = p.precision == PrPrecision::PATTERN;
   if (pattern || mixed) {
@@ -448,8 +469,9 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   h.beta = p.beta;
   h.eps = p.epsilon;
   h.bound_target = p.epsilon * (1.0 - p.beta) / p.beta;
-  h.tele_scale = p.personalized ? (1.0 - p.beta) / static_cast<double>(static_cast<std::size_t>(p.n_nodes))
-                                : (1.0 - p.beta) / static_cast<double>(n);
+  h.tele_scale = tmode == 2   ? (1.0 - p.beta)
+                 : tmode == 1 ? (1.0 - p.beta) / static_cast<double>(static_cast<std::size_t>(p.n_nodes))
+                              : (1.0 - p.beta) / static_cast<double>(n);
   h.sum = b0 * static_cast<double>(n);
   h.tele = h.sum * h.tele_scale;
   h.bound = 2.0;
@@ -486,6 +508,8 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     hipLaunchKernelGGL(scale_into, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), X[0], w.Z.as<double>(), n);
   }
   TBSLA_HIP_CHECK(hipGetLastError());
+  // the set-up copies read pageable host memory (state, node set, 1/count)
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
 
   hipEvent_t e0, e1;
   TBSLA_HIP_CHECK(hipEventCreate(&e0));
@@ -500,7 +524,6 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     hipLaunchKernelGGL(pr_finalize_norm, dim3(1), dim3(kTileThreads), 0, s, w.perr.as<double>(), w.ps.as<double>(),
                        w.pc.as<double>(), static_cast<int>(gN), st);
   };
-  const bool pers = p.personalized;
   double *Z = w.Z.as<double>();
   const double *colscale = w.colscale.as<double>();
   while (launched < p.max_iteration) {
@@ -519,21 +542,21 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
           else
             launch_anchor<double>(d64, xhat, yhat, st, s);
           if (p.precision == PrPrecision::F32)
-            launch_update<float, float, true>(pers, *dlo, dcur, yhat, member, xout, st, slots, s);
+            launch_update<float, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
           else
-            launch_update<dev::bf16, float, true>(pers, *dlo, dcur, yhat, member, xout, st, slots, s);
+            launch_update<dev::bf16, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
           fin_max();
           hipLaunchKernelGGL(pr_normalize<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,
                              nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
           fin_norm();
         } else if (pattern) {
-          launch_update<dev::Pattern, double, false>(pers, d64, Z, nullptr, member, xout, st, slots, s);
+          launch_update<dev::Pattern, double, false>(tmode, d64, nullptr, Z, nullptr, tp, xout, st, slots, s);
           fin_max();
           hipLaunchKernelGGL(pr_normalize<2>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
                              nullptr, Z, colscale, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
           fin_norm();
         } else {
-          launch_update<double, double, false>(pers, d64, xin, nullptr, member, xout, st, slots, s);
+          launch_update<double, double, false>(tmode, d64, p.values, xin, nullptr, tp, xout, st, slots, s);
           fin_max();
           hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
                              nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
@@ -544,10 +567,12 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
         // any format: y = 0 (accumulating formats), y += A b_t, update, normalise
         hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, xout, n, 0.0);
         if (!d64.noop) launch_spmv(d64, xin, Dtype::F64, xout, Dtype::F64, 0, s);
-        if (pers)
-          hipLaunchKernelGGL(pr_update_vec<true>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st, slots);
+        if (tmode == 1)
+          hipLaunchKernelGGL(pr_update_vec<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, tp, st, slots);
+        else if (tmode == 2)
+          hipLaunchKernelGGL(pr_update_vec<2>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, tp, st, slots);
         else
-          hipLaunchKernelGGL(pr_update_vec<false>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, member, st, slots);
+          hipLaunchKernelGGL(pr_update_vec<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, tp, st, slots);
         fin_max();
         hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
                            nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),

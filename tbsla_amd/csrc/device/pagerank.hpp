@@ -45,6 +45,13 @@ struct PageRankParams {
   PrRule rule = PrRule::REFERENCE;
   int check_every = 8;      // iterations launched between convergence reads
   double anchor_tau = 0;    // 0: default per precision
+  // Optional device inputs (FP64, CSR only):
+  //   values    edge values in CSR entry order, used instead of the stored
+  //             ones (precision F64)
+  //   teleport  teleport distribution v (n_col entries): the update adds
+  //             (1-beta) * sum(b_t) * v[i], i.e. M = beta*A + (1-beta) v 1^T
+  const double *values = nullptr;
+  const double *teleport = nullptr;
 };
 
 struct PageRankReport {

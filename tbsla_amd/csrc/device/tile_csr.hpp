@@ -101,7 +101,9 @@ __device__ inline AT block_tree_max(AT v, AT *part) {
 }
 
 // Device view of a CSR mirror (entry j of the matrix is colidx[j], vals[j],
-// with j = rowptr[i] - rp0 + k for the k-th entry of row i).
+// with j = rowptr[i] - rp0 + k for the k-th entry of row i).  colidx is the
+// mirror's padded buffer; vals may be a caller's array of exactly nnz
+// values, so no value is read at index nnz or beyond.
 template <typename VT>
 struct CsrView {
   const int *rowptr;
@@ -195,7 +197,12 @@ __device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict_
 #pragma unroll
   for (int k = 0; k < G; k++) {
     const int g = k * kTileThreads + tid;
-    if (4 * g < ne) Prod<VT, XT>::group(m, x, e0 + 4 * g, c[k], p[k]);
+    if (4 * g + 4 <= ne)
+      Prod<VT, XT>::group(m, x, e0 + 4 * g, c[k], p[k]);
+    else if (4 * g < ne)  // the matrix's last entries: no value read past nnz
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (4 * g + q < ne) p[k][q] = Prod<VT, XT>::one(m, x, e0 + 4 * g + q);
   }
 #pragma unroll
   for (int k = 0; k < G; k++) {

@@ -198,8 +198,13 @@ class Matrix:
         return y
 
     def page_rank_device(self, out, epsilon=1e-10, beta=0.85, max_iteration=100, precision="f64",
-                         rule="reference", check_every=8, anchor_tau=0.0, nodes=None, stream=None):
-        """PageRank into the float64 tensor ``out``; returns the run report."""
+                         rule="reference", check_every=8, anchor_tau=0.0, nodes=None, stream=None,
+                         values=None, teleport=None):
+        """PageRank into the float64 tensor ``out``; returns the run report.
+
+        ``values`` (float64 device tensor, one per stored CSR entry) replaces
+        the stored edge values; ``teleport`` (float64 device tensor, n_col)
+        is the teleport distribution v of b = beta*A b + (1-beta)(1^T b) v."""
         p = _lib.PageRankParams()
         p.epsilon, p.beta, p.max_iteration = epsilon, beta, max_iteration
         p.precision = dtype_code(precision)
@@ -211,12 +216,66 @@ class Matrix:
             keep = np.ascontiguousarray(nodes, dtype=np.int32)
             p.nodes = iptr(keep)
             p.n_nodes = keep.size
+        _check_f64(out, "out")
+        vals_p, vals_n = _opt_f64(values, "values")
+        tele_p, tele_n = _opt_f64(teleport, "teleport")
         rep = _lib.PageRankReport()
-        check(lib.tbsla_hip_page_rank(self._h, C.byref(p), C.cast(_tensor_ptr(out), _lib._dp),
-                                      _stream_handle(stream), C.byref(rep)))
+        check(lib.tbsla_hip_page_rank_ex(self._h, C.byref(p), vals_p, vals_n, tele_p, tele_n, _tensor_ptr(out),
+                                         _stream_handle(stream), C.byref(rep)))
         del keep
         return {"iterations": rep.iterations, "anchors": rep.anchors, "last_error": rep.last_error,
                 "bound": rep.bound, "seconds": rep.seconds}
+
+    def page_rank_adjoint(self, pr, w, grad_values=None, grad_teleport=None, beta=0.85, epsilon=1e-7,
+                          max_iteration=1000, precision="f32", check_every=8, values=None, teleport=None,
+                          stream=None):
+        """Gradients of L(p) through the PageRank fixed point ``pr`` given
+        ``w`` = dL/dp: dL/dA per stored CSR entry into ``grad_values`` and
+        dL/dv into ``grad_teleport`` (float64 device tensors, either may be
+        None).  ``values`` / ``teleport`` as in page_rank_device.  Returns
+        the run report."""
+        _check_f64(pr, "pr")
+        _check_f64(w, "w")
+        if pr.numel() != w.numel():
+            raise ValueError("pr and w must have the same length")
+        p = _lib.AdjointParams()
+        p.beta, p.epsilon, p.max_iteration = beta, epsilon, max_iteration
+        p.precision = dtype_code(precision)
+        p.check_every = check_every
+        vals_p, vals_n = _opt_f64(values, "values")
+        tele_p, tele_n = _opt_f64(teleport, "teleport")
+        gv_p, gv_n = _opt_f64(grad_values, "grad_values")
+        gt_p, gt_n = _opt_f64(grad_teleport, "grad_teleport")
+        if grad_values is not None and gv_n < self.n_entries():
+            raise ValueError("grad_values shorter than the stored entries")
+        if grad_teleport is not None and gt_n < self.get_n_col():
+            raise ValueError("grad_teleport shorter than n_col")
+        rep = _lib.AdjointReport()
+        check(lib.tbsla_hip_page_rank_adjoint(self._h, C.byref(p), vals_p, vals_n, tele_p, tele_n, _tensor_ptr(pr),
+                                              _tensor_ptr(w), pr.numel(), gv_p, gt_p, _stream_handle(stream),
+                                              C.byref(rep)))
+        return {"iterations": rep.iterations, "last_change": rep.last_change, "s": rep.s,
+                "seconds": rep.seconds}
+
+    def n_entries(self) -> int:
+        """Stored CSR entries (rowptr[-1] - rowptr[0]): the length of per-entry arrays."""
+        if self.FORMAT != _lib.CSR:
+            raise TypeError("per-entry arrays need a CSR matrix")
+        rp = self._array(1, np.int32)
+        return int(rp[-1] - rp[0]) if rp.size > 1 else 0
+
+
+def _check_f64(t, name):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous float64 CUDA tensor")
+
+
+def _opt_f64(t, name):
+    if t is None:
+        return None, 0
+    _check_f64(t, name)
+    return _tensor_ptr(t), t.numel()
 
 
 class MatrixCOO(Matrix):
