@@ -143,6 +143,48 @@ int tbsla_hip_page_rank_adjoint(tbsla_matrix *m, const tbsla_adjoint_params *p, 
                                 const double *pr_dev, const double *w_dev, size_t n, double *grad_values_dev,
                                 double *grad_teleport_dev, void *stream, tbsla_adjoint_report *report);
 
+/* ---- batched personalised PageRank (no reference counterpart: the
+ * reference runs Matrix::personalized_page_rank once per teleport set,
+ * Matrix.cpp:63-114; here every graph and teleport set advances together) */
+typedef struct tbsla_batch tbsla_batch;
+
+/* precision: TBSLA_F64 or TBSLA_F32 (values, iterates and products; the
+ * residual, maxima and sums are FP64 in both) */
+tbsla_batch *tbsla_batch_create(int precision);
+void tbsla_batch_destroy(tbsla_batch *b);
+/* A square CSR graph with n_rhs teleport node sets: set k is
+ * nodes[rhs_ptr[k] .. rhs_ptr[k+1]).  The matrix must stay alive until
+ * tbsla_batch_prepare (or the first tbsla_batch_run) returns. */
+int tbsla_batch_add_graph(tbsla_batch *b, const tbsla_matrix *m, const int *rhs_ptr, const int *nodes, int n_rhs);
+int tbsla_batch_prepare(tbsla_batch *b);
+int tbsla_batch_n_rhs(const tbsla_batch *b);
+/* offsets[n_rhs + 1]: start of each teleport set's vector in the output */
+int tbsla_batch_layout(const tbsla_batch *b, long *offsets, size_t n);
+
+typedef struct {
+  double epsilon;
+  double beta;
+  int max_iteration;
+  int rule;          /* 0: reference residual < epsilon; 1: sum-normalised bound */
+  int check_every;   /* iterations per captured hipGraph replay */
+  int graph_capture; /* 1: replay a hipGraph; 0: plain launches */
+  int autotune;      /* 1: time candidate launch shapes on the first run */
+} tbsla_batch_params;
+
+typedef struct {
+  int iterations;              /* largest iteration count over the teleport sets */
+  double seconds;              /* device time of the iteration loop */
+  double tune_seconds;
+  double bytes_per_iteration;  /* algorithmic bytes of one iteration, all sets active */
+  int wg_per_cu, unroll, xcd_map; /* launch shape in use */
+} tbsla_batch_report;
+
+/* Sum-normalised vectors of every teleport set into out_dev (FP64, layout
+ * from tbsla_batch_layout); rhs_iterations (n_rhs_it >= n_rhs, or NULL)
+ * receives each set's iteration count. */
+int tbsla_batch_run(tbsla_batch *b, const tbsla_batch_params *p, double *out_dev, size_t n_out, int *rhs_iterations,
+                    size_t n_rhs_it, void *stream, tbsla_batch_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,11 +7,16 @@ Python mirror of the reference classes (matrix.py).  The library is loaded on
 first use of a matrix class, so ``python -m tbsla_amd.build`` works before it
 exists.
 """
-__all__ = ["FORMATS", "Matrix", "MatrixCOO", "MatrixSCOO", "MatrixCSR", "MatrixELL", "MatrixDENSE"]
+_MATRIX = ["FORMATS", "Matrix", "MatrixCOO", "MatrixSCOO", "MatrixCSR", "MatrixELL", "MatrixDENSE"]
+_BATCH = ["PageRankBatch", "personalised_workload", "u01"]
+__all__ = _MATRIX + _BATCH
 
 
 def __getattr__(name):
-    if name in __all__:
+    if name in _MATRIX:
         from . import matrix
         return getattr(matrix, name)
+    if name in _BATCH:
+        from . import batch
+        return getattr(batch, name)
     raise AttributeError(name)

@@ -47,6 +47,17 @@ class AdjointReport(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("epsilon", C.c_double), ("beta", C.c_double), ("max_iteration", C.c_int), ("rule", C.c_int),
+                ("check_every", C.c_int), ("graph_capture", C.c_int), ("autotune", C.c_int)]
+
+
+class BatchReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("tune_seconds", C.c_double),
+                ("bytes_per_iteration", C.c_double), ("wg_per_cu", C.c_int), ("unroll", C.c_int),
+                ("xcd_map", C.c_int)]
+
+
 # Every entry point the headers declare, with its ctypes signature.
 _vp, _i, _u, _d, _sz, _l = C.c_void_p, C.c_int, C.c_uint, C.c_double, C.c_size_t, C.c_long
 _dp = C.POINTER(C.c_double)
@@ -81,6 +92,13 @@ SIGNATURES = {
                                     C.POINTER(PageRankReport)]),
     "tbsla_hip_page_rank_adjoint": (_i, [_vp, C.POINTER(AdjointParams), _vp, _sz, _vp, _sz, _vp, _vp, _sz, _vp,
                                          _vp, _vp, C.POINTER(AdjointReport)]),
+    "tbsla_batch_create": (_vp, [_i]),
+    "tbsla_batch_destroy": (None, [_vp]),
+    "tbsla_batch_add_graph": (_i, [_vp, _vp, _ip, _ip, _i]),
+    "tbsla_batch_prepare": (_i, [_vp]),
+    "tbsla_batch_n_rhs": (_i, [_vp]),
+    "tbsla_batch_layout": (_i, [_vp, C.POINTER(_l), _sz]),
+    "tbsla_batch_run": (_i, [_vp, C.POINTER(BatchParams), _vp, _sz, _ip, _sz, _vp, C.POINTER(BatchReport)]),
     # reference C ABI (interface/c): declared for the export check
     "C_CPP_Vector_create": (_vp, []),
     "C_CPP_Vector_destroy": (None, [_vp]),

@@ -18,6 +18,7 @@
 #include "../../../include/tbsla_cpp_c.h"
 #include "../../../include/tbsla_hip.h"
 #include "../device/adjoint.hpp"
+#include "../device/batch.hpp"
 #include "../device/kernels.hpp"
 #include "../device/mirror.hpp"
 #include "../device/pagerank.hpp"
@@ -69,6 +70,10 @@ hip::Dtype dtype_of(int d) {
 
 struct tbsla_matrix {
   std::unique_ptr<cpp::Matrix> m;
+};
+
+struct tbsla_batch {
+  std::unique_ptr<tbsla::hip::PageRankBatch> b;
 };
 
 // ===================================================================== C_CPP
@@ -568,6 +573,78 @@ int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, cons
       report->last_error = r.last_error;
       report->bound = r.bound;
       report->seconds = r.seconds;
+    }
+  });
+}
+
+tbsla_batch *tbsla_batch_create(int precision) {
+  tbsla_batch *b = nullptr;
+  const int rc = guarded([&] {
+    if (precision != TBSLA_F64 && precision != TBSLA_F32)
+      throw std::invalid_argument("tbsla: batched PageRank runs in F64 or F32");
+    b = new tbsla_batch{std::unique_ptr<hip::PageRankBatch>(new hip::PageRankBatch(dtype_of(precision)))};
+  });
+  return rc == TBSLA_OK ? b : nullptr;
+}
+
+void tbsla_batch_destroy(tbsla_batch *b) { delete b; }
+
+int tbsla_batch_add_graph(tbsla_batch *b, const tbsla_matrix *m, const int *rhs_ptr, const int *nodes, int n_rhs) {
+  if (!b || !m || !rhs_ptr || n_rhs <= 0) return fail(TBSLA_EINVAL, "tbsla: bad argument");
+  return guarded([&] {
+    std::vector<std::vector<int>> sets(static_cast<std::size_t>(n_rhs));
+    for (int k = 0; k < n_rhs; k++) {
+      if (rhs_ptr[k + 1] < rhs_ptr[k] || rhs_ptr[k] < 0) throw std::invalid_argument("tbsla: rhs_ptr is not monotone");
+      if (rhs_ptr[k + 1] > rhs_ptr[k] && !nodes) throw std::invalid_argument("tbsla: NULL nodes");
+      sets[static_cast<std::size_t>(k)].assign(nodes + rhs_ptr[k], nodes + rhs_ptr[k + 1]);
+    }
+    b->b->add_graph(*m->m, sets);
+  });
+}
+
+int tbsla_batch_prepare(tbsla_batch *b) {
+  if (!b) return fail(TBSLA_EINVAL, "tbsla: NULL batch");
+  return guarded([&] { b->b->prepare(); });
+}
+
+int tbsla_batch_n_rhs(const tbsla_batch *b) { return b ? b->b->n_rhs() : 0; }
+
+int tbsla_batch_layout(const tbsla_batch *b, long *offsets, size_t n) {
+  if (!b || !offsets) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    const std::vector<long> o = b->b->rhs_offsets();
+    if (n < o.size()) throw std::invalid_argument("tbsla: offsets shorter than n_rhs + 1");
+    std::memcpy(offsets, o.data(), o.size() * sizeof(long));
+  });
+}
+
+int tbsla_batch_run(tbsla_batch *b, const tbsla_batch_params *p, double *out_dev, size_t n_out, int *rhs_iterations,
+                    size_t n_rhs_it, void *stream, tbsla_batch_report *report) {
+  if (!b || !p || !out_dev) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    const std::vector<long> o = b->b->rhs_offsets();
+    if (n_out < static_cast<size_t>(o.back())) throw std::invalid_argument("tbsla: output shorter than the batch");
+    if (rhs_iterations && n_rhs_it < o.size() - 1) throw std::invalid_argument("tbsla: rhs_iterations too short");
+    if (p->rule < 0 || p->rule > 1) throw std::invalid_argument("tbsla: bad stopping rule");
+    hip::BatchParams q;
+    q.epsilon = p->epsilon;
+    q.beta = p->beta;
+    q.max_iteration = p->max_iteration;
+    q.rule = static_cast<hip::PrRule>(p->rule);
+    q.check_every = p->check_every > 0 ? p->check_every : 16;
+    q.graph_capture = p->graph_capture != 0;
+    q.autotune = p->autotune != 0;
+    const hip::BatchReport r = b->b->run(q, out_dev, static_cast<hipStream_t>(stream));
+    if (rhs_iterations)
+      for (std::size_t k = 0; k < r.rhs_iterations.size(); k++) rhs_iterations[k] = r.rhs_iterations[k];
+    if (report) {
+      report->iterations = r.iterations;
+      report->seconds = r.seconds;
+      report->tune_seconds = r.tune_seconds;
+      report->bytes_per_iteration = r.bytes_per_iteration;
+      report->wg_per_cu = r.config.wg_per_cu;
+      report->unroll = r.config.unroll;
+      report->xcd_map = r.config.xcd_map;
     }
   });
 }

@@ -1,0 +1,1018 @@
+// Batched personalised PageRank engine (layout and launch sequence in
+// batch.hpp).
+#include <hip/hip_runtime.h>
+
+#include <tbsla/cpp/MatrixCSR.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+#include "batch.hpp"
+#include "mirror.hpp"
+#include "tile_csr.hpp"
+
+namespace tbsla {
+namespace hip {
+namespace {
+
+constexpr int kLanes = 64;           // RHS per SpMM group (one per lane)
+constexpr int kSpmmRowsPerWave = 16;
+constexpr int kSpmmRows = 4 * kSpmmRowsPerWave;  // rows per SpMM work item
+constexpr int kSlots = 16;           // atomic-max slots per RHS
+constexpr int kChunkRows = 2048;     // rows per pass item, single-RHS graphs
+constexpr int kSpmmChunkRows = 256;  // rows per pass item, SpMM groups
+constexpr int kMaxGroups = 64;
+constexpr int kQueues = 8;           // one per XCD
+
+struct RhsState {
+  double tele_scale, tele, maxv, err, sum, sum_prev, bound, total;
+  int it, done, pad0, pad1;
+};
+
+struct PassPart {
+  double e, s, c;
+};
+
+struct RhsInfo {
+  long base;  // first partial (index into the PassPart array)
+  int count, stride;
+};
+
+struct PassItem {
+  int kind;  // 0: rows [r0, r1) of single-RHS graph `id`; 1: rows [r0, r1) of SpMM group `id`
+  int id, r0, r1;
+  long part;  // first partial written by the item
+};
+
+template <typename VT>
+struct SpmmGroup {
+  const int *rowptr;
+  const int *colidx;
+  const VT *vals;
+  const unsigned long long *member;  // bit j: row is in the node set of lane j
+  int rows, rp0, f_col, n_rhs, rhs_base, item_base, nitems, pad;
+  long xoff;  // first element of the group's n x 64 iterate
+};
+
+template <typename VT, typename XT>
+struct BatchDev {
+  dev::CsrView<VT> a;  // block-diagonal single-RHS graphs
+  int a_ntiles, a_nlong, a_rows, n_groups;
+  const int *row_graph;
+  const int *row_base;  // first row of each single-RHS graph
+  const unsigned char *a_member;
+  XT *xa0, *xa1, *xb0, *xb1;  // iterate buffers by parity
+  const SpmmGroup<VT> *g;     // n_groups descriptors (device memory)
+  RhsState *rs;
+  unsigned long long *slots;
+  const int *q_items;
+  const int *q_off;           // kQueues + 1 queue boundaries in q_items
+  int xcd_map;
+  double beta;
+};
+
+__device__ inline unsigned long long ord_encode(double v) {
+  const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double ord_decode(unsigned long long u) {
+  if (u == 0) return -INFINITY;
+  return __longlong_as_double(static_cast<long long>((u >> 63) ? (u & 0x7fffffffffffffffull) : ~u));
+}
+
+struct CSum {
+  double s, c;
+};
+__device__ inline void csum_add(CSum &a, double v) {
+  const double t = a.s + v;
+  if (fabs(a.s) >= fabs(v))
+    a.c += (a.s - t) + v;
+  else
+    a.c += (v - t) + a.s;
+  a.s = t;
+}
+__device__ inline CSum csum_merge(CSum a, CSum b) {
+  const double t = a.s + b.s;
+  const double bp = t - a.s;
+  const double e = (a.s - (t - bp)) + (b.s - bp);
+  return CSum{t, a.c + b.c + e};
+}
+
+// Row epilogue of the single-RHS tiles: b = beta*s (+ t on listed rows),
+// running maxima of the tile's first and last graph (rows of any graph in
+// between, possible only for graphs shorter than a tile, go to the slots).
+template <typename XT>
+struct TileEpi {
+  XT *y;
+  const int *row_graph;
+  RhsState *rs;
+  const unsigned char *member;
+  unsigned long long *slots;
+  double beta;
+  int gA, gB, slot;
+  double mA, mB;
+  __device__ void operator()(int row, double s) {
+#pragma clang fp contract(off)
+    const int g = row_graph[row];
+    if (rs[g].done) return;
+    double bi = beta * s;
+    if (member[row]) bi = bi + rs[g].tele;
+    const XT bs = static_cast<XT>(bi);
+    y[row] = bs;
+    const double bv = static_cast<double>(bs);
+    if (g == gA)
+      mA = fmax(mA, bv);
+    else if (g == gB)
+      mB = fmax(mB, bv);
+    else
+      atomicMax(slots + g * kSlots + slot, ord_encode(bv));
+  }
+  __device__ void operator()(int row, float s) { (*this)(row, static_cast<double>(s)); }
+};
+
+template <typename VT, typename XT, int UNR>
+__device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int blk, const XT *__restrict__ xin,
+                          XT *__restrict__ xout, double *red) {
+#pragma clang fp contract(off)
+  using AT = typename dev::Val<VT>::A;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rhs = G.rhs_base + lane;
+  const bool active = lane < G.n_rhs && !d.rs[rhs].done;
+  const double tele = active ? d.rs[rhs].tele : 0.0;
+  const XT *X = xin + G.xoff;
+  XT *Y = xout + G.xoff;
+  double lmax = -INFINITY;
+  if (__ballot(active) != 0) {
+    for (int i = 0; i < kSpmmRowsPerWave; i++) {
+      const int row = __builtin_amdgcn_readfirstlane(blk * kSpmmRows + wave * kSpmmRowsPerWave + i);
+      if (row >= G.rows) break;
+      const int a0 = G.rowptr[row] - G.rp0, a1 = G.rowptr[row + 1] - G.rp0;
+      AT acc = 0;
+      for (int e = a0; e < a1; e += UNR) {
+        AT vv[UNR], xv[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+          if (e + u < a1) {
+            const long c = G.colidx[e + u] - G.f_col;
+            vv[u] = dev::Val<VT>::get(G.vals, e + u);
+            xv[u] = active ? static_cast<AT>(X[c * kLanes + lane]) : AT(0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; u++)
+          if (e + u < a1) acc += vv[u] * xv[u];
+      }
+      if (active) {
+        double bi = d.beta * static_cast<double>(acc);
+        if ((G.member[row] >> lane) & 1ull) bi = bi + tele;
+        const XT bs = static_cast<XT>(bi);
+        Y[static_cast<long>(row) * kLanes + lane] = bs;
+        lmax = fmax(lmax, static_cast<double>(bs));
+      }
+    }
+  }
+  red[threadIdx.x] = lmax;
+  __syncthreads();
+  if (wave == 0) {
+    const double m = fmax(fmax(red[lane], red[64 + lane]), fmax(red[128 + lane], red[192 + lane]));
+    if (active && m > -INFINITY) atomicMax(d.slots + rhs * kSlots + (blk % kSlots), ord_encode(m));
+  }
+  __syncthreads();
+}
+
+// Persistent step: every workgroup walks its queue with a static stride
+// (queue = XCD of the workgroup when xcd_map is set, else one shared list).
+template <typename VT, typename XT, int UNR>
+__global__ __launch_bounds__(kTileThreads) void batch_step(BatchDev<VT, XT> d, int par) {
+  using AT = typename dev::Val<VT>::A;
+  __shared__ dev::CsrTileLds<AT> lds;
+  __shared__ double red[kTileThreads];
+  const XT *xa_in = par ? d.xa1 : d.xa0, *xb_in = par ? d.xb1 : d.xb0;
+  XT *xa_out = par ? d.xa0 : d.xa1, *xb_out = par ? d.xb0 : d.xb1;
+  int q, local, stride;
+  if (d.xcd_map) {
+    q = blockIdx.x % kQueues;
+    local = blockIdx.x / kQueues;
+    stride = gridDim.x / kQueues;
+  } else {
+    q = 0;
+    local = blockIdx.x;
+    stride = gridDim.x;
+  }
+  const int q_end = d.q_off[q + 1];
+  for (int j = d.q_off[q] + local; j < q_end; j += stride) {
+    const int w = d.q_items[j];
+    if (w < d.a_ntiles + d.a_nlong) {
+      const bool tile = w < d.a_ntiles;
+      int gA, gB;
+      if (tile) {
+        const int rlo = d.a.tile_row[w], rhi = d.a.tile_row[w + 1];
+        if (rlo >= rhi) continue;
+        gA = d.row_graph[rlo];
+        gB = d.row_graph[rhi - 1];
+        if (gA == gB && d.rs[gA].done) continue;
+      } else {
+        gA = gB = d.row_graph[d.a.long_rows[w - d.a_ntiles]];
+        if (d.rs[gA].done) continue;
+      }
+      TileEpi<XT> epi{xa_out, d.row_graph, d.rs, d.a_member, d.slots, d.beta, gA, gB, w % kSlots,
+                      -INFINITY, -INFINITY};
+      if (tile)
+        dev::csr_tile_rows_at<VT, XT>(d.a, xa_in, lds, epi, w);
+      else
+        dev::csr_long_row_at<VT, XT>(d.a, xa_in, lds, epi, w - d.a_ntiles);
+      const double bA = dev::block_tree_max<double>(epi.mA, red);
+      const double bB = dev::block_tree_max<double>(epi.mB, red);
+      if (threadIdx.x == 0) {
+        if (bA > -INFINITY) atomicMax(d.slots + gA * kSlots + (w % kSlots), ord_encode(bA));
+        if (gB != gA && bB > -INFINITY) atomicMax(d.slots + gB * kSlots + (w % kSlots), ord_encode(bB));
+      }
+    } else {
+      int s = 0;
+      while (s + 1 < d.n_groups && w >= d.g[s + 1].item_base) s++;
+      spmm_item<VT, XT, UNR>(d, d.g[s], w - d.g[s].item_base, xb_in, xb_out, red);
+    }
+  }
+}
+
+// One wavefront per RHS: max over its slots (cleared for the next step).
+__global__ __launch_bounds__(64) void batch_finalize_max(RhsState *rs, unsigned long long *slots) {
+  const int r = blockIdx.x, l = threadIdx.x;
+  if (rs[r].done) return;
+  unsigned long long u = 0;
+  if (l < kSlots) {
+    u = __hip_atomic_load(slots + r * kSlots + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(slots + r * kSlots + l, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(u, off, 64);
+    u = o > u ? o : u;
+  }
+  if (l == 0) rs[r].maxv = ord_decode(u);
+}
+
+// MODE 0: normalise the step's output (b /= max, residual, compensated sum).
+// MODE 1: compensated sum of each RHS's final iterate.
+// MODE 2: out = final iterate / total.
+template <typename VT, typename XT, int MODE>
+__global__ __launch_bounds__(kTileThreads) void batch_pass(BatchDev<VT, XT> d, const PassItem *__restrict__ items,
+                                                           PassPart *__restrict__ part,
+                                                           const long *__restrict__ out_off,
+                                                           double *__restrict__ out, int par) {
+#pragma clang fp contract(off)
+  __shared__ double red_e[kTileThreads], red_s[kTileThreads], red_c[kTileThreads];
+  const PassItem it = items[blockIdx.x];
+  if (it.kind == 0) {
+    const int g = it.id;
+    const RhsState &st = d.rs[g];
+    if (MODE == 0 && st.done) return;
+    XT *cur = MODE == 0 ? (par ? d.xa0 : d.xa1) : ((st.it & 1) ? d.xa1 : d.xa0);
+    const XT *prev = par ? d.xa1 : d.xa0;
+    const double m = st.maxv, S = st.total;
+    double err = 0;
+    CSum cs{0, 0};
+    const long o = out_off[g] - d.row_base[g];
+    for (int i = it.r0 + threadIdx.x; i < it.r1; i += kTileThreads) {
+      if (MODE == 0) {
+        const XT bs = static_cast<XT>(static_cast<double>(cur[i]) / m);
+        const double bi = static_cast<double>(bs);
+        err += fabs(bi - static_cast<double>(prev[i]));
+        csum_add(cs, bi);
+        cur[i] = bs;
+      } else if (MODE == 1) {
+        csum_add(cs, static_cast<double>(cur[i]));
+      } else {
+        out[o + i] = static_cast<double>(cur[i]) / S;
+      }
+    }
+    if (MODE == 2) return;
+    err = dev::block_tree_sum<double>(err, red_e);
+    // compensated merge across the block, fixed order
+    red_s[threadIdx.x] = cs.s;
+    red_c[threadIdx.x] = cs.c;
+    __syncthreads();
+    for (int k = kTileThreads / 2; k > 0; k >>= 1) {
+      if (static_cast<int>(threadIdx.x) < k) {
+        const CSum mm = csum_merge(CSum{red_s[threadIdx.x], red_c[threadIdx.x]},
+                                   CSum{red_s[threadIdx.x + k], red_c[threadIdx.x + k]});
+        red_s[threadIdx.x] = mm.s;
+        red_c[threadIdx.x] = mm.c;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[it.part] = PassPart{err, red_s[0], red_c[0]};
+    return;
+  }
+  // SpMM group: wave w takes rows r0 + w, r0 + w + 4, ...; lane = RHS
+  const SpmmGroup<VT> &G = d.g[it.id];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rhs = G.rhs_base + lane;
+  const bool valid = lane < G.n_rhs;
+  const RhsState *st = valid ? &d.rs[rhs] : nullptr;
+  const bool active = valid && (MODE != 0 || !st->done);
+  double err = 0;
+  CSum cs{0, 0};
+  if (active) {
+    XT *cur = (MODE == 0 ? (par ? d.xb0 : d.xb1) : ((st->it & 1) ? d.xb1 : d.xb0)) + G.xoff;
+    const XT *prev = (par ? d.xb1 : d.xb0) + G.xoff;
+    const double m = st->maxv, S = st->total;
+    for (int i = it.r0 + wave; i < it.r1; i += 4) {
+      const long k = static_cast<long>(i) * kLanes + lane;
+      if (MODE == 0) {
+        const XT bs = static_cast<XT>(static_cast<double>(cur[k]) / m);
+        const double bi = static_cast<double>(bs);
+        err += fabs(bi - static_cast<double>(prev[k]));
+        csum_add(cs, bi);
+        cur[k] = bs;
+      } else if (MODE == 1) {
+        csum_add(cs, static_cast<double>(cur[k]));
+      } else {
+        out[out_off[rhs] + i] = static_cast<double>(cur[k]) / S;
+      }
+    }
+  }
+  if (MODE == 2) return;
+  red_e[threadIdx.x] = err;
+  red_s[threadIdx.x] = cs.s;
+  red_c[threadIdx.x] = cs.c;
+  __syncthreads();
+  if (wave == 0) {
+    double e = red_e[lane];
+    CSum t{red_s[lane], red_c[lane]};
+    for (int w = 1; w < 4; w++) {
+      e += red_e[w * 64 + lane];
+      t = csum_merge(t, CSum{red_s[w * 64 + lane], red_c[w * 64 + lane]});
+    }
+    if (active) part[it.part + lane] = PassPart{e, t.s, t.c};
+  }
+}
+
+// One wavefront per RHS: merge its partials in a fixed order.  FINAL = 0:
+// the step decision (reference residual or sum-normalised bound) and the
+// next teleport term; FINAL = 1: the total of the final iterate.
+template <int FINAL>
+__global__ __launch_bounds__(64) void batch_finalize_norm(RhsState *rs, const RhsInfo *__restrict__ info,
+                                                          const PassPart *__restrict__ part, double eps,
+                                                          double bound_target, int rule, int max_it) {
+  const int r = blockIdx.x, l = threadIdx.x;
+  RhsState &st = rs[r];
+  if (!FINAL && st.done) return;
+  const RhsInfo in = info[r];
+  double e = 0;
+  CSum t{0, 0};
+  for (int k = l; k < in.count; k += 64) {
+    const PassPart p = part[in.base + static_cast<long>(k) * in.stride];
+    e += p.e;
+    t = csum_merge(t, CSum{p.s, p.c});
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off, 64);
+    const CSum o{__shfl_xor(t.s, off, 64), __shfl_xor(t.c, off, 64)};
+    // the lower lane keeps the left operand so both lanes compute the same value
+    t = (l & off) ? csum_merge(o, t) : csum_merge(t, o);
+  }
+  if (l != 0) return;
+  const double S = t.s + t.c;
+  if (FINAL) {
+    st.total = S;
+    return;
+  }
+  st.it += 1;
+  st.err = e;
+  st.sum_prev = st.sum;
+  st.sum = S;
+  st.bound = e / S + fabs(st.sum_prev / S - 1.0);
+  const bool converged = rule == 0 ? (e < eps) : (st.bound <= bound_target);
+  st.done = (converged || st.it >= max_it) ? 1 : 0;
+  st.tele = S * st.tele_scale;
+}
+
+template <typename T>
+__global__ void batch_fill(T *p, long n, T v) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) p[i] = v;
+}
+
+unsigned fill_grid(long n) { return static_cast<unsigned>(std::max<long>(1, std::min<long>(4096, (n + 255) / 256))); }
+
+}  // namespace
+
+std::string BatchConfig::str() const {
+  std::ostringstream o;
+  o << "wg_per_cu=" << wg_per_cu << " unroll=" << unroll << " xcd_map=" << xcd_map;
+  return o.str();
+}
+
+struct PageRankBatch::Impl {
+  Dtype prec;
+  struct Graph {
+    const cpp::MatrixCSR *m;
+    std::vector<std::vector<int>> rhs;
+  };
+  std::vector<Graph> graphs;
+  bool prepared = false;
+  int cus = 256;
+  // single-RHS block-diagonal matrix
+  DevBuf a_rowptr, a_colidx, a_vals, a_tiles, a_long, row_graph, row_base, a_member, xa[2];
+  int a_rows = 0, a_ntiles = 0, a_nlong = 0, n_a = 0;
+  long a_nnz = 0;
+  // SpMM graphs and groups
+  struct SGraph {
+    DevBuf rowptr, colidx, vals;
+    int rows = 0, rp0 = 0;
+    long nnz = 0;
+  };
+  std::vector<std::unique_ptr<SGraph>> sgraphs;
+  struct Group {
+    int sgraph, n_rhs, rhs_base, item_base, nitems;
+    long xoff;
+    DevBuf member;
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  DevBuf xb[2];
+  long xb_len = 0;
+  // RHS bookkeeping (device RHS order: single-RHS graphs, then SpMM lanes)
+  int n_rhs_total = 0;
+  std::vector<long> user_offsets;    // output offset per RHS in user order (+ total)
+  std::vector<int> user_to_dev;      // device RHS id of each user RHS
+  std::vector<int> dev_rows;         // rows of each device RHS's graph
+  std::vector<double> dev_nodes;     // |nodes| of each device RHS
+  DevBuf rs, slots, info, parts, items, out_off;
+  int n_items = 0;
+  long n_parts = 0;
+  DevBuf group_desc;  // SpmmGroup<VT> array
+  // work queues
+  DevBuf q_items[2], q_off_dev[2];
+  std::vector<int> q_off[2];
+  int n_work = 0;
+  BatchConfig cfg;
+  bool tuned = false;
+  double bytes_step = 0, bytes_pass = 0;
+
+  std::size_t vsize() const { return prec == Dtype::F64 ? 8 : 4; }
+
+  hipStream_t stream = nullptr;  // private stream: capture needs a non-default stream
+  ~Impl() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <typename VT, typename XT>
+  BatchDev<VT, XT> dev_view(int xcd_map, double beta) const;
+  template <typename VT, typename XT>
+  void enqueue_iteration(const BatchDev<VT, XT> &d, int par, const BatchParams &p, hipStream_t s) const;
+  template <typename VT, typename XT>
+  void launch_step(const BatchDev<VT, XT> &d, int par, int unroll, int wg_per_cu, hipStream_t s) const;
+  template <typename VT, typename XT>
+  BatchReport run_typed(const BatchParams &p, double *out, hipStream_t s);
+  void build();
+  void init_state(const BatchParams &p, hipStream_t s);
+  template <typename VT>
+  void upload_groups() {
+    std::vector<SpmmGroup<VT>> gd;
+    for (const auto &gp : groups) {
+      const SGraph &sg = *sgraphs[static_cast<std::size_t>(gp->sgraph)];
+      gd.push_back(SpmmGroup<VT>{sg.rowptr.as<int>(), sg.colidx.as<int>(), sg.vals.as<VT>(),
+                                 gp->member.as<unsigned long long>(), sg.rows, sg.rp0, 0, gp->n_rhs, gp->rhs_base,
+                                 gp->item_base, gp->nitems, 0, gp->xoff});
+    }
+    group_desc.upload(gd.data(), gd.size() * sizeof(SpmmGroup<VT>));
+  }
+};
+
+namespace {
+
+void upload_converted(DevBuf &dst, const double *src, std::size_t n, Dtype prec) {
+  if (prec == Dtype::F64) {
+    dst.upload(src, n * sizeof(double));
+  } else {
+    std::vector<float> t(n);
+    for (std::size_t i = 0; i < n; i++) t[i] = static_cast<float>(src[i]);
+    dst.upload(t.data(), n * sizeof(float));
+  }
+}
+
+}  // namespace
+
+void PageRankBatch::Impl::build() {
+  int dev = 0;
+  TBSLA_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  TBSLA_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  cus = std::max(8, prop.multiProcessorCount);
+  const std::size_t vb = vsize();
+  // ---- device RHS numbering
+  std::vector<int> single, multi;
+  for (std::size_t g = 0; g < graphs.size(); g++) (graphs[g].rhs.size() == 1 ? single : multi).push_back(static_cast<int>(g));
+  n_a = static_cast<int>(single.size());
+  std::vector<std::vector<int>> user_rhs_dev(graphs.size());
+  int next = 0;
+  for (int g : single) user_rhs_dev[static_cast<std::size_t>(g)].push_back(next++);
+  for (int g : multi)
+    for (std::size_t k = 0; k < graphs[static_cast<std::size_t>(g)].rhs.size(); k++)
+      user_rhs_dev[static_cast<std::size_t>(g)].push_back(next++);
+  n_rhs_total = next;
+  dev_rows.assign(static_cast<std::size_t>(n_rhs_total), 0);
+  dev_nodes.assign(static_cast<std::size_t>(n_rhs_total), 0);
+  user_offsets.clear();
+  user_to_dev.clear();
+  long off = 0;
+  std::vector<long> dev_out(static_cast<std::size_t>(n_rhs_total), 0);
+  for (std::size_t g = 0; g < graphs.size(); g++) {
+    const long n = graphs[g].m->get_n_row();
+    for (std::size_t k = 0; k < graphs[g].rhs.size(); k++) {
+      const int r = user_rhs_dev[g][k];
+      user_offsets.push_back(off);
+      user_to_dev.push_back(r);
+      dev_out[static_cast<std::size_t>(r)] = off;
+      dev_rows[static_cast<std::size_t>(r)] = static_cast<int>(n);
+      dev_nodes[static_cast<std::size_t>(r)] = static_cast<double>(graphs[g].rhs[k].size());
+      off += n;
+    }
+  }
+  user_offsets.push_back(off);
+
+  auto csr_of = [&](int g, std::vector<int> &rp, const int **ci, const double **va, long &nnz) {
+    const cpp::MatrixCSR &m = *graphs[static_cast<std::size_t>(g)].m;
+    const auto &r = m.get_rowptr();
+    const int rows = m.get_n_row();
+    if (static_cast<int>(r.size()) != rows + 1) throw std::invalid_argument("tbsla: batch graph rowptr does not cover every row");
+    rp.assign(r.begin(), r.end());
+    nnz = static_cast<long>(rp.back()) - rp[0];
+    for (int i = 0; i < rows; i++)
+      if (rp[static_cast<std::size_t>(i) + 1] < rp[static_cast<std::size_t>(i)]) throw std::invalid_argument("tbsla: CSR rowptr is not monotone");
+    // entry k of the graph is colidx/values[rowptr[0] + k] (MatrixCSR.cpp:135)
+    if (rp[0] < 0 || static_cast<long>(m.get_colidx().size()) < rp[0] + nnz ||
+        static_cast<long>(m.get_values().size()) < rp[0] + nnz)
+      throw std::invalid_argument("tbsla: CSR rowptr points past values/colidx");
+    *ci = m.get_colidx().data() + rp[0];
+    *va = m.get_values().data() + rp[0];
+    for (long k = 0; k < nnz; k++) {
+      const int c = (*ci)[k] - m.get_f_col();
+      if (c < 0 || c >= m.get_n_col()) throw std::invalid_argument("tbsla: column index outside the graph");
+    }
+  };
+
+  // ---- single-RHS graphs: one block-diagonal CSR
+  {
+    std::vector<int> rp_all{0}, ci_all, rg, rb;
+    std::vector<double> va_all;
+    std::vector<unsigned char> mem;
+    for (int gi = 0; gi < n_a; gi++) {
+      const int g = single[static_cast<std::size_t>(gi)];
+      std::vector<int> rp;
+      const int *ci;
+      const double *va;
+      long nnz;
+      csr_of(g, rp, &ci, &va, nnz);
+      const int rows = static_cast<int>(rp.size()) - 1;
+      const long base_row = static_cast<long>(rg.size());
+      const long base_e = static_cast<long>(ci_all.size());
+      if (base_row + rows > (1L << 31) - 2 || base_e + nnz > (1L << 31) - 2)
+        throw std::invalid_argument("tbsla: batch exceeds 2^31 rows or entries");
+      rb.push_back(static_cast<int>(base_row));
+      const int f_col = graphs[static_cast<std::size_t>(g)].m->get_f_col();
+      for (long k = 0; k < nnz; k++) {
+        ci_all.push_back(static_cast<int>(base_row + ci[k] - f_col));
+        va_all.push_back(va[k]);
+      }
+      for (int i = 0; i < rows; i++) {
+        rp_all.push_back(static_cast<int>(base_e + rp[static_cast<std::size_t>(i) + 1] - rp[0]));
+        rg.push_back(gi);
+      }
+      std::vector<unsigned char> m(static_cast<std::size_t>(rows), 0);
+      for (int node : graphs[static_cast<std::size_t>(g)].rhs[0])
+        if (node >= 0 && node < rows) m[static_cast<std::size_t>(node)] = 1;
+      mem.insert(mem.end(), m.begin(), m.end());
+    }
+    a_rows = static_cast<int>(rg.size());
+    a_nnz = static_cast<long>(ci_all.size());
+    if (a_rows > 0) {
+      a_rowptr.upload(rp_all.data(), rp_all.size() * sizeof(int));
+      a_colidx.upload(ci_all.data(), ci_all.size() * sizeof(int));
+      upload_converted(a_vals, va_all.data(), va_all.size(), prec);
+      row_graph.upload(rg.data(), rg.size() * sizeof(int));
+      row_base.upload(rb.data(), rb.size() * sizeof(int));
+      a_member.upload(mem.data(), mem.size());
+      std::vector<int> t = csr_entry_tiles(rp_all.data(), a_rows, kCsrTileNnz);
+      a_ntiles = static_cast<int>(t.size()) - 1;
+      a_tiles.upload(t.data(), t.size() * sizeof(int));
+      std::vector<int> lr;
+      for (int i = 0; i < a_rows; i++)
+        if (rp_all[static_cast<std::size_t>(i) + 1] - rp_all[static_cast<std::size_t>(i)] > kCsrLongRow) lr.push_back(i);
+      a_nlong = static_cast<int>(lr.size());
+      if (a_nlong) a_long.upload(lr.data(), lr.size() * sizeof(int));
+      for (int k = 0; k < 2; k++) xa[k].alloc(vb * static_cast<std::size_t>(a_rows));
+    }
+  }
+
+  // ---- multi-RHS graphs: one CSR each, groups of up to 64 RHS
+  int item_base = a_ntiles + a_nlong;
+  xb_len = 0;
+  for (int g : multi) {
+    auto sg = std::unique_ptr<SGraph>(new SGraph());
+    std::vector<int> rp;
+    const int *ci;
+    const double *va;
+    long nnz;
+    csr_of(g, rp, &ci, &va, nnz);
+    sg->rows = static_cast<int>(rp.size()) - 1;
+    sg->rp0 = rp[0];
+    sg->nnz = nnz;
+    sg->rowptr.upload(rp.data(), rp.size() * sizeof(int));
+    const int f_col = graphs[static_cast<std::size_t>(g)].m->get_f_col();
+    std::vector<int> cc(static_cast<std::size_t>(nnz));
+    for (long k = 0; k < nnz; k++) cc[static_cast<std::size_t>(k)] = ci[k] - f_col;
+    sg->colidx.upload(cc.data(), cc.size() * sizeof(int));
+    upload_converted(sg->vals, va, static_cast<std::size_t>(nnz), prec);
+    const auto &rhs = graphs[static_cast<std::size_t>(g)].rhs;
+    for (std::size_t k0 = 0; k0 < rhs.size(); k0 += kLanes) {
+      if (static_cast<int>(groups.size()) >= kMaxGroups)
+        throw std::invalid_argument("tbsla: batch supports at most 64 groups of 64 right-hand sides");
+      auto gr = std::unique_ptr<Group>(new Group());
+      gr->sgraph = static_cast<int>(sgraphs.size());
+      gr->n_rhs = static_cast<int>(std::min<std::size_t>(kLanes, rhs.size() - k0));
+      gr->rhs_base = user_rhs_dev[static_cast<std::size_t>(g)][k0];
+      gr->item_base = item_base;
+      gr->nitems = (sg->rows + kSpmmRows - 1) / kSpmmRows;
+      item_base += gr->nitems;
+      gr->xoff = xb_len;
+      xb_len += static_cast<long>(sg->rows) * kLanes;
+      std::vector<unsigned long long> mb(static_cast<std::size_t>(sg->rows), 0ull);
+      for (int j = 0; j < gr->n_rhs; j++)
+        for (int node : rhs[k0 + static_cast<std::size_t>(j)])
+          if (node >= 0 && node < sg->rows) mb[static_cast<std::size_t>(node)] |= 1ull << j;
+      gr->member.upload(mb.data(), mb.size() * sizeof(unsigned long long));
+      groups.push_back(std::move(gr));
+    }
+    sgraphs.push_back(std::move(sg));
+  }
+  n_work = item_base;
+  if (xb_len > 0)
+    for (int k = 0; k < 2; k++) xb[k].alloc(vb * static_cast<std::size_t>(xb_len));
+
+  // ---- pass items and partial layout
+  std::vector<PassItem> it;
+  std::vector<RhsInfo> inf(static_cast<std::size_t>(n_rhs_total));
+  long np = 0;
+  {
+    std::vector<int> rbh;
+    long base = 0;
+    for (int gi = 0; gi < n_a; gi++) {
+      const int rows = graphs[static_cast<std::size_t>(single[static_cast<std::size_t>(gi)])].m->get_n_row();
+      inf[static_cast<std::size_t>(gi)] = RhsInfo{np, 0, 1};
+      for (int r0 = 0; r0 < rows; r0 += kChunkRows) {
+        it.push_back(PassItem{0, gi, static_cast<int>(base + r0), static_cast<int>(base + std::min(rows, r0 + kChunkRows)), np});
+        np++;
+        inf[static_cast<std::size_t>(gi)].count++;
+      }
+      base += rows;
+    }
+    for (std::size_t s = 0; s < groups.size(); s++) {
+      const Group &gr = *groups[s];
+      const int rows = sgraphs[static_cast<std::size_t>(gr.sgraph)]->rows;
+      const long first = np;
+      int cnt = 0;
+      for (int r0 = 0; r0 < rows; r0 += kSpmmChunkRows) {
+        it.push_back(PassItem{1, static_cast<int>(s), r0, std::min(rows, r0 + kSpmmChunkRows), np});
+        np += kLanes;
+        cnt++;
+      }
+      for (int j = 0; j < gr.n_rhs; j++) inf[static_cast<std::size_t>(gr.rhs_base + j)] = RhsInfo{first + j, cnt, kLanes};
+    }
+  }
+  n_items = static_cast<int>(it.size());
+  n_parts = np;
+  if (n_items) items.upload(it.data(), it.size() * sizeof(PassItem));
+  info.upload(inf.data(), inf.size() * sizeof(RhsInfo));
+  parts.alloc(sizeof(PassPart) * static_cast<std::size_t>(std::max(1L, np)));
+  out_off.upload(dev_out.data(), dev_out.size() * sizeof(long));
+  rs.alloc(sizeof(RhsState) * static_cast<std::size_t>(std::max(1, n_rhs_total)));
+  slots.alloc(sizeof(unsigned long long) * kSlots * static_cast<std::size_t>(std::max(1, n_rhs_total)));
+
+  // ---- work queues.  Weights are the bytes an item reads.
+  std::vector<double> wgt(static_cast<std::size_t>(n_work), 0.0);
+  std::vector<int> item_graph(static_cast<std::size_t>(n_work), -1);
+  {
+    std::vector<int> rp_all;
+    if (a_rows > 0) {
+      rp_all.resize(static_cast<std::size_t>(a_rows) + 1);
+      TBSLA_HIP_CHECK(hipMemcpy(rp_all.data(), a_rowptr.get(), rp_all.size() * sizeof(int), hipMemcpyDeviceToHost));
+      std::vector<int> t(static_cast<std::size_t>(a_ntiles) + 1), rg(static_cast<std::size_t>(a_rows));
+      TBSLA_HIP_CHECK(hipMemcpy(t.data(), a_tiles.get(), t.size() * sizeof(int), hipMemcpyDeviceToHost));
+      TBSLA_HIP_CHECK(hipMemcpy(rg.data(), row_graph.get(), rg.size() * sizeof(int), hipMemcpyDeviceToHost));
+      for (int k = 0; k < a_ntiles; k++) {
+        const long e0 = static_cast<long>(k) * kCsrTileNnz;
+        wgt[static_cast<std::size_t>(k)] = static_cast<double>(std::min<long>(kCsrTileNnz, a_nnz - e0)) * (vb + 4 + vb);
+        const int r = std::min(t[static_cast<std::size_t>(k)], a_rows - 1);
+        item_graph[static_cast<std::size_t>(k)] = rg[static_cast<std::size_t>(r)];
+      }
+      std::vector<int> lr(static_cast<std::size_t>(a_nlong));
+      if (a_nlong) TBSLA_HIP_CHECK(hipMemcpy(lr.data(), a_long.get(), lr.size() * sizeof(int), hipMemcpyDeviceToHost));
+      for (int k = 0; k < a_nlong; k++) {
+        const int r = lr[static_cast<std::size_t>(k)];
+        wgt[static_cast<std::size_t>(a_ntiles + k)] =
+            static_cast<double>(rp_all[static_cast<std::size_t>(r) + 1] - rp_all[static_cast<std::size_t>(r)]) * (vb + 4 + vb);
+        item_graph[static_cast<std::size_t>(a_ntiles + k)] = rg[static_cast<std::size_t>(r)];
+      }
+    }
+    for (const auto &gp : groups) {
+      const SGraph &sg = *sgraphs[static_cast<std::size_t>(gp->sgraph)];
+      const double per_row = static_cast<double>(sg.nnz) / std::max(1, sg.rows) * (vb + 4 + kLanes * vb) + 2.0 * kLanes * vb;
+      for (int b = 0; b < gp->nitems; b++) wgt[static_cast<std::size_t>(gp->item_base + b)] = per_row * kSpmmRows;
+    }
+  }
+  // interleaved: one list in item order
+  {
+    std::vector<int> all(static_cast<std::size_t>(n_work));
+    std::iota(all.begin(), all.end(), 0);
+    q_off[0].assign(kQueues + 1, n_work);
+    q_off[0][0] = 0;
+    if (n_work) q_items[0].upload(all.data(), all.size() * sizeof(int));
+  }
+  // XCD-pinned: SpMM row ranges split evenly, single-RHS graphs whole, by weight
+  {
+    std::vector<std::vector<int>> q(kQueues);
+    std::vector<double> load(kQueues, 0.0);
+    for (const auto &gp : groups)
+      for (int x = 0; x < kQueues; x++) {
+        const int b0 = static_cast<int>(static_cast<long>(gp->nitems) * x / kQueues);
+        const int b1 = static_cast<int>(static_cast<long>(gp->nitems) * (x + 1) / kQueues);
+        for (int b = b0; b < b1; b++) {
+          q[static_cast<std::size_t>(x)].push_back(gp->item_base + b);
+          load[static_cast<std::size_t>(x)] += wgt[static_cast<std::size_t>(gp->item_base + b)];
+        }
+      }
+    std::vector<std::vector<int>> by_graph(static_cast<std::size_t>(std::max(1, n_a)));
+    std::vector<double> gw(static_cast<std::size_t>(std::max(1, n_a)), 0.0);
+    for (int k = 0; k < a_ntiles + a_nlong; k++) {
+      const int g = item_graph[static_cast<std::size_t>(k)];
+      by_graph[static_cast<std::size_t>(g)].push_back(k);
+      gw[static_cast<std::size_t>(g)] += wgt[static_cast<std::size_t>(k)];
+    }
+    std::vector<int> order(static_cast<std::size_t>(n_a));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gw[static_cast<std::size_t>(x)] > gw[static_cast<std::size_t>(y)]; });
+    for (int g : order) {
+      const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      for (int k : by_graph[static_cast<std::size_t>(g)]) q[static_cast<std::size_t>(x)].push_back(k);
+      load[static_cast<std::size_t>(x)] += gw[static_cast<std::size_t>(g)];
+    }
+    std::vector<int> flat;
+    q_off[1].assign(1, 0);
+    for (int x = 0; x < kQueues; x++) {
+      flat.insert(flat.end(), q[static_cast<std::size_t>(x)].begin(), q[static_cast<std::size_t>(x)].end());
+      q_off[1].push_back(static_cast<int>(flat.size()));
+    }
+    if (!flat.empty()) q_items[1].upload(flat.data(), flat.size() * sizeof(int));
+  }
+  for (int k = 0; k < 2; k++) q_off_dev[k].upload(q_off[k].data(), q_off[k].size() * sizeof(int));
+  if (!groups.empty()) {
+    if (prec == Dtype::F64)
+      upload_groups<double>();
+    else
+      upload_groups<float>();
+  }
+  // algorithmic bytes of one iteration with every RHS active
+  bytes_step = static_cast<double>(a_nnz) * (vb + 4) + static_cast<double>(a_rows) * (4 + 1 + 2 * vb);
+  bytes_pass = 3.0 * vb * static_cast<double>(a_rows);
+  for (const auto &gp : groups) {
+    const SGraph &sg = *sgraphs[static_cast<std::size_t>(gp->sgraph)];
+    bytes_step += static_cast<double>(sg.nnz) * (vb + 4) + static_cast<double>(sg.rows) * (4 + 8 + 2.0 * gp->n_rhs * vb);
+    bytes_pass += 3.0 * vb * static_cast<double>(sg.rows) * gp->n_rhs;
+  }
+  TBSLA_HIP_CHECK(hipDeviceSynchronize());
+  prepared = true;
+}
+
+template <typename VT, typename XT>
+BatchDev<VT, XT> PageRankBatch::Impl::dev_view(int xcd_map, double beta) const {
+  BatchDev<VT, XT> d;
+  std::memset(&d, 0, sizeof(d));
+  d.a = dev::CsrView<VT>{a_rowptr.as<int>(), a_colidx.as<int>(), a_vals.as<VT>(), a_tiles.as<int>(),
+                         a_long.as<int>(), a_nnz, 0, 0};
+  d.a_ntiles = a_ntiles;
+  d.a_nlong = a_nlong;
+  d.a_rows = a_rows;
+  d.n_groups = static_cast<int>(groups.size());
+  d.row_graph = row_graph.as<int>();
+  d.row_base = row_base.as<int>();
+  d.a_member = a_member.as<unsigned char>();
+  d.xa0 = xa[0].as<XT>();
+  d.xa1 = xa[1].as<XT>();
+  d.xb0 = xb[0].as<XT>();
+  d.xb1 = xb[1].as<XT>();
+  d.g = group_desc.as<SpmmGroup<VT>>();
+  d.rs = rs.as<RhsState>();
+  d.slots = slots.as<unsigned long long>();
+  d.q_items = q_items[xcd_map].as<int>();
+  d.q_off = q_off_dev[xcd_map].as<int>();
+  d.xcd_map = xcd_map;
+  d.beta = beta;
+  return d;
+}
+
+template <typename VT, typename XT>
+void PageRankBatch::Impl::launch_step(const BatchDev<VT, XT> &d, int par, int unroll, int wg_per_cu,
+                                      hipStream_t s) const {
+  if (n_work == 0) return;
+  const unsigned grid = static_cast<unsigned>(std::max(kQueues, (cus * wg_per_cu) / kQueues * kQueues));
+  if (unroll == 4)
+    hipLaunchKernelGGL((batch_step<VT, XT, 4>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+  else
+    hipLaunchKernelGGL((batch_step<VT, XT, 8>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+}
+
+template <typename VT, typename XT>
+void PageRankBatch::Impl::enqueue_iteration(const BatchDev<VT, XT> &d, int par, const BatchParams &p,
+                                            hipStream_t s) const {
+  launch_step<VT, XT>(d, par, cfg.unroll, cfg.wg_per_cu, s);
+  hipLaunchKernelGGL(batch_finalize_max, dim3(n_rhs_total), dim3(64), 0, s, d.rs, d.slots);
+  if (n_items)
+    hipLaunchKernelGGL((batch_pass<VT, XT, 0>), dim3(n_items), dim3(kTileThreads), 0, s, d, items.as<PassItem>(),
+                       parts.as<PassPart>(), out_off.as<long>(), static_cast<double *>(nullptr), par);
+  hipLaunchKernelGGL(batch_finalize_norm<0>, dim3(n_rhs_total), dim3(64), 0, s, d.rs, info.as<RhsInfo>(),
+                     parts.as<PassPart>(), p.epsilon, p.epsilon * (1.0 - p.beta) / p.beta, static_cast<int>(p.rule),
+                     p.max_iteration);
+}
+
+void PageRankBatch::Impl::init_state(const BatchParams &p, hipStream_t s) {
+  std::vector<RhsState> h(static_cast<std::size_t>(n_rhs_total));
+  for (int r = 0; r < n_rhs_total; r++) {
+    RhsState &st = h[static_cast<std::size_t>(r)];
+    std::memset(&st, 0, sizeof(st));
+    st.tele_scale = (1.0 - p.beta) / dev_nodes[static_cast<std::size_t>(r)];
+    st.sum = 0.25 * static_cast<double>(dev_rows[static_cast<std::size_t>(r)]);
+    st.tele = st.sum * st.tele_scale;
+    st.bound = 2.0;
+    st.done = p.max_iteration <= 0 ? 1 : 0;
+  }
+  TBSLA_HIP_CHECK(hipMemcpyAsync(rs.get(), h.data(), h.size() * sizeof(RhsState), hipMemcpyHostToDevice, s));
+  if (prec == Dtype::F64) {
+    if (a_rows) hipLaunchKernelGGL(batch_fill<double>, dim3(fill_grid(a_rows)), dim3(256), 0, s, xa[0].as<double>(), static_cast<long>(a_rows), 0.25);
+    if (xb_len) hipLaunchKernelGGL(batch_fill<double>, dim3(fill_grid(xb_len)), dim3(256), 0, s, xb[0].as<double>(), xb_len, 0.25);
+  } else {
+    if (a_rows) hipLaunchKernelGGL(batch_fill<float>, dim3(fill_grid(a_rows)), dim3(256), 0, s, xa[0].as<float>(), static_cast<long>(a_rows), 0.25f);
+    if (xb_len) hipLaunchKernelGGL(batch_fill<float>, dim3(fill_grid(xb_len)), dim3(256), 0, s, xb[0].as<float>(), xb_len, 0.25f);
+  }
+  const long ns = static_cast<long>(kSlots) * n_rhs_total;
+  hipLaunchKernelGGL(batch_fill<unsigned long long>, dim3(fill_grid(ns)), dim3(256), 0, s, slots.as<unsigned long long>(), ns, 0ull);
+  TBSLA_HIP_CHECK(hipGetLastError());
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));  // the state copy reads host memory
+}
+
+template <typename VT, typename XT>
+BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hipStream_t s) {
+  BatchReport rep;
+  init_state(p, s);
+  if (p.autotune && !tuned && n_work > 0) {
+    // time one step per candidate launch shape on the batch itself
+    const auto t0 = std::chrono::steady_clock::now();
+    hipEvent_t e0, e1;
+    TBSLA_HIP_CHECK(hipEventCreate(&e0));
+    TBSLA_HIP_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    BatchConfig bestc = cfg;
+    for (int xm = 0; xm < 2; xm++) {
+      const BatchDev<VT, XT> d = dev_view<VT, XT>(xm, p.beta);
+      for (int un : {4, 8})
+        for (int wg : {2, 4, 8}) {
+          launch_step<VT, XT>(d, 0, un, wg, s);
+          TBSLA_HIP_CHECK(hipEventRecord(e0, s));
+          for (int k = 0; k < 3; k++) launch_step<VT, XT>(d, 0, un, wg, s);
+          TBSLA_HIP_CHECK(hipEventRecord(e1, s));
+          TBSLA_HIP_CHECK(hipEventSynchronize(e1));
+          float ms = 0;
+          TBSLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            bestc = BatchConfig{wg, un, xm};
+          }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cfg = bestc;
+    tuned = true;
+    init_state(p, s);  // the trial steps wrote the second iterate and the slots
+    rep.tune_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  const BatchDev<VT, XT> d = dev_view<VT, XT>(cfg.xcd_map, p.beta);
+  const int C = std::max(2, (p.check_every + 1) / 2 * 2);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (p.graph_capture && p.max_iteration > 0) {
+    TBSLA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < C; k++) enqueue_iteration<VT, XT>(d, k & 1, p, s);
+    TBSLA_HIP_CHECK(hipStreamEndCapture(s, &graph));
+    TBSLA_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  std::vector<RhsState> hs(static_cast<std::size_t>(n_rhs_total));
+  hipEvent_t e0, e1;
+  TBSLA_HIP_CHECK(hipEventCreate(&e0));
+  TBSLA_HIP_CHECK(hipEventCreate(&e1));
+  TBSLA_HIP_CHECK(hipEventRecord(e0, s));
+  for (int launched = 0; launched < p.max_iteration; launched += C) {
+    if (exec)
+      TBSLA_HIP_CHECK(hipGraphLaunch(exec, s));
+    else
+      for (int k = 0; k < C; k++) enqueue_iteration<VT, XT>(d, k & 1, p, s);
+    TBSLA_HIP_CHECK(hipGetLastError());
+    TBSLA_HIP_CHECK(hipMemcpyAsync(hs.data(), rs.get(), hs.size() * sizeof(RhsState), hipMemcpyDeviceToHost, s));
+    TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+    bool all = true;
+    for (const auto &st : hs) all = all && st.done;
+    if (all) break;
+  }
+  TBSLA_HIP_CHECK(hipEventRecord(e1, s));
+  // final sum normalisation of every RHS (Matrix.cpp:105-111), compensated
+  if (n_items) {
+    hipLaunchKernelGGL((batch_pass<VT, XT, 1>), dim3(n_items), dim3(kTileThreads), 0, s, d, items.as<PassItem>(),
+                       parts.as<PassPart>(), out_off.as<long>(), out, 0);
+    hipLaunchKernelGGL(batch_finalize_norm<1>, dim3(n_rhs_total), dim3(64), 0, s, d.rs, info.as<RhsInfo>(),
+                       parts.as<PassPart>(), 0.0, 0.0, 0, 0);
+    hipLaunchKernelGGL((batch_pass<VT, XT, 2>), dim3(n_items), dim3(kTileThreads), 0, s, d, items.as<PassItem>(),
+                       parts.as<PassPart>(), out_off.as<long>(), out, 0);
+  }
+  TBSLA_HIP_CHECK(hipGetLastError());
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  TBSLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  rep.seconds = ms * 1e-3;
+  rep.config = cfg;
+  rep.bytes_per_iteration = bytes_step + bytes_pass;
+  for (int u = 0; u < n_rhs_total; u++) {
+    const RhsState &st = hs[static_cast<std::size_t>(user_to_dev[static_cast<std::size_t>(u)])];
+    rep.rhs_iterations.push_back(st.it);
+    rep.rhs_error.push_back(st.err);
+    rep.iterations = std::max(rep.iterations, st.it);
+  }
+  return rep;
+}
+
+PageRankBatch::PageRankBatch(Dtype precision) : impl_(new Impl()) {
+  if (precision != Dtype::F64 && precision != Dtype::F32)
+    throw std::invalid_argument("tbsla: batched PageRank runs in F64 or F32");
+  impl_->prec = precision;
+}
+
+PageRankBatch::~PageRankBatch() = default;
+
+void PageRankBatch::add_graph(const cpp::Matrix &m, const std::vector<std::vector<int>> &rhs_nodes) {
+  if (impl_->prepared) throw std::invalid_argument("tbsla: batch already prepared");
+  if (m.format() != cpp::FORMAT_CSR) throw std::invalid_argument("tbsla: batched PageRank needs CSR graphs");
+  if (m.get_n_row() != m.get_n_col() || m.get_NR() != 1 || m.get_NC() != 1)
+    throw std::invalid_argument("tbsla: page_rank needs a square matrix held in one block");
+  if (rhs_nodes.empty()) throw std::invalid_argument("tbsla: a batch graph needs at least one teleport set");
+  for (const auto &r : rhs_nodes)
+    if (r.empty()) throw std::invalid_argument("tbsla: empty teleport node set");
+  impl_->graphs.push_back(Impl::Graph{&dynamic_cast<const cpp::MatrixCSR &>(m), rhs_nodes});
+}
+
+void PageRankBatch::prepare() {
+  if (!impl_->prepared) impl_->build();
+}
+
+BatchReport PageRankBatch::run(const BatchParams &p, double *out_dev, hipStream_t s) {
+  prepare();
+  if (impl_->n_rhs_total == 0) return BatchReport{};
+  if (!impl_->stream) TBSLA_HIP_CHECK(hipStreamCreateWithFlags(&impl_->stream, hipStreamNonBlocking));
+  // order the private stream after the caller's stream, and the caller's after the run
+  hipEvent_t ev;
+  TBSLA_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  TBSLA_HIP_CHECK(hipEventRecord(ev, s));
+  TBSLA_HIP_CHECK(hipStreamWaitEvent(impl_->stream, ev, 0));
+  BatchReport r = impl_->prec == Dtype::F64 ? impl_->run_typed<double, double>(p, out_dev, impl_->stream)
+                                            : impl_->run_typed<float, float>(p, out_dev, impl_->stream);
+  TBSLA_HIP_CHECK(hipEventRecord(ev, impl_->stream));
+  TBSLA_HIP_CHECK(hipStreamWaitEvent(s, ev, 0));
+  (void)hipEventDestroy(ev);
+  return r;
+}
+
+int PageRankBatch::n_rhs() const {
+  int n = 0;
+  for (const auto &g : impl_->graphs) n += static_cast<int>(g.rhs.size());
+  return n;
+}
+
+std::vector<long> PageRankBatch::rhs_offsets() const {
+  if (impl_->prepared) return impl_->user_offsets;
+  std::vector<long> o{0};
+  for (const auto &g : impl_->graphs)
+    for (std::size_t k = 0; k < g.rhs.size(); k++) o.push_back(o.back() + g.m->get_n_row());
+  return o;
+}
+
+const BatchConfig &PageRankBatch::config() const { return impl_->cfg; }
+
+}  // namespace hip
+}  // namespace tbsla

@@ -161,16 +161,16 @@ struct Prod<Pattern, XT> {
   }
 };
 
-// Rows of one entry tile (blockIdx.x); epi(row, sum) once per owned row by
-// the thread that sums it.
+// Rows of entry tile `tile`; epi(row, sum) once per owned row by the thread
+// that sums it.  The caller synchronises the block before reusing `lds`.
 template <typename VT, typename XT, typename Epi>
-__device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict__ x,
-                                     CsrTileLds<typename Val<VT>::A> &lds, Epi &epi) {
+__device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restrict__ x,
+                                        CsrTileLds<typename Val<VT>::A> &lds, Epi &epi, int tile) {
 #pragma clang fp contract(off)
   using AT = typename Val<VT>::A;
   constexpr int G = kCsrTileNnz / 4 / kTileThreads;  // 4-entry groups per thread
   const int tid = threadIdx.x;
-  const long e0 = static_cast<long>(blockIdx.x) * kCsrTileNnz;
+  const long e0 = static_cast<long>(tile) * kCsrTileNnz;
   const int ne = static_cast<int>(min(static_cast<long>(kCsrTileNnz), m.nnz - e0));
   int c[G][4];
   AT p[G][4];
@@ -188,7 +188,7 @@ __device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict_
         if (4 * g + q >= ne) c[k][q] = 0;
     }
   }
-  const int rlo = m.tile_row[blockIdx.x], rhi = m.tile_row[blockIdx.x + 1];
+  const int rlo = m.tile_row[tile], rhi = m.tile_row[tile + 1];
   int a = 0, b = 0;
   if (rlo + tid < rhi) {
     a = m.rowptr[rlo + tid] - m.rp0;
@@ -236,13 +236,20 @@ __device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict_
   }
 }
 
-// One long row (m.long_rows[blockIdx.x]) per workgroup: chunk products, a
+// Rows of entry tile blockIdx.x.
+template <typename VT, typename XT, typename Epi>
+__device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict__ x,
+                                     CsrTileLds<typename Val<VT>::A> &lds, Epi &epi) {
+  csr_tile_rows_at<VT, XT, Epi>(m, x, lds, epi, blockIdx.x);
+}
+
+// One long row (m.long_rows[idx]) per workgroup: chunk products, a
 // fixed-order tree per chunk, chunk totals added in order.  epi by thread 0.
 template <typename VT, typename XT, typename Epi>
-__device__ inline void csr_long_row(const CsrView<VT> &m, const XT *__restrict__ x,
-                                    CsrTileLds<typename Val<VT>::A> &lds, Epi &epi) {
+__device__ inline void csr_long_row_at(const CsrView<VT> &m, const XT *__restrict__ x,
+                                       CsrTileLds<typename Val<VT>::A> &lds, Epi &epi, int idx) {
   using AT = typename Val<VT>::A;
-  const int r = m.long_rows[blockIdx.x];
+  const int r = m.long_rows[idx];
   const long a = m.rowptr[r] - m.rp0, b = m.rowptr[r + 1] - m.rp0;
   AT total = 0;
   for (long base = a; base < b; base += kCsrTileNnz) {
@@ -252,6 +259,12 @@ __device__ inline void csr_long_row(const CsrView<VT> &m, const XT *__restrict__
     total += block_tree_sum<AT>(acc, lds.prod);
   }
   if (threadIdx.x == 0) epi(r, total);
+}
+
+template <typename VT, typename XT, typename Epi>
+__device__ inline void csr_long_row(const CsrView<VT> &m, const XT *__restrict__ x,
+                                    CsrTileLds<typename Val<VT>::A> &lds, Epi &epi) {
+  csr_long_row_at<VT, XT, Epi>(m, x, lds, epi, blockIdx.x);
 }
 
 }  // namespace dev

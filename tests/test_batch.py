@@ -1,0 +1,76 @@
+"""Batched personalised PageRank (tbsla_batch_*) against the reference
+semantics of Matrix::personalized_page_rank (Matrix.cpp:63-114), one oracle
+call per teleport set, as the reference would run them."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(tb, oracle):
+    """Five single-set graphs of uneven size (one with a 3-node set), one
+    graph with 9 one-hot sets (SpMM group), one with 70 sets (two groups)."""
+    spec = [(1024, 12, 0.3, 1000, [[5]]), (1500, 12, 0.3, 1001, [[1499]]), (3000, 8, 0.5, 7, [[0, 17, 2999]]),
+            (5000, 12, 0.3, 1002, [[2500]]), (2100, 16, 0.1, 3, [[77]]),
+            (4096, 16, 0.3, 1256, [[(37 * r) % 4096] for r in range(9)]),
+            (2048, 12, 0.3, 1003, [[(101 * r + 5) % 2048] for r in range(70)])]
+    mats, orcs = [], []
+    for (n, c, q, s, sets) in spec:
+        mats.append(tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s))
+        orcs.append(oracle.new("CSR").fill_cqmat_stochastic(n, n, c, q, s))
+    return spec, mats, orcs
+
+
+def run_batch(tb, mats, spec, precision, **kw):
+    b = tb.PageRankBatch(precision)
+    for m, (n, c, q, s, sets) in zip(mats, spec):
+        b.add_graph(m, sets)
+    out, rep, its = b.run(**kw)
+    return b.layout(), out.cpu().numpy(), rep, its
+
+
+def test_batch_f64_matches_reference_per_set(tb, oracle, gpu):
+    spec, mats, orcs = small_batch(tb, oracle)
+    off, out, rep, its = run_batch(tb, mats, spec, "f64", epsilon=1e-10, max_iteration=500)
+    k = 0
+    for o, (n, c, q, s, sets) in zip(orcs, spec):
+        for nodes in sets:
+            pr, it = o.personalized_page_rank(1e-10, 0.85, 500, np.array(nodes, dtype=np.int32))
+            got = out[off[k]:off[k + 1]]
+            assert abs(int(its[k]) - it) <= 1, (n, nodes, its[k], it)
+            assert np.abs(got - pr).sum() <= 1e-13, (n, nodes, np.abs(got - pr).sum())
+            k += 1
+    assert k == len(its) and rep["iterations"] == its.max()
+
+
+def test_batch_graph_replay_equals_plain_launches(tb, oracle, gpu):
+    """hipGraph replay, plain launches and every autotuned launch shape give
+    bit-identical results (the products and reductions are order-fixed)."""
+    spec, mats, _ = small_batch(tb, oracle)
+    _, a, _, ia = run_batch(tb, mats, spec, "f64", epsilon=1e-10, max_iteration=300, graph=True, autotune=True)
+    _, b, _, ib = run_batch(tb, mats, spec, "f64", epsilon=1e-10, max_iteration=300, graph=False, autotune=False)
+    assert np.array_equal(ia, ib)
+    assert np.array_equal(a, b)
+
+
+def test_batch_f32_within_tolerance(tb, oracle, gpu):
+    """FP32 values, iterates and products (FP64 residual and sums), stopped by
+    the sum-normalised bound at 1e-6: within 1e-5 L1 of the FP64 result."""
+    spec, mats, orcs = small_batch(tb, oracle)
+    off, out, rep, its = run_batch(tb, mats, spec, "f32", epsilon=1e-6, rule="sumnorm", max_iteration=500)
+    off64, out64, _, _ = run_batch(tb, mats, spec, "f64", epsilon=1e-13, rule="sumnorm", max_iteration=500)
+    assert np.array_equal(off, off64)
+    for k in range(len(off) - 1):
+        p32, p64 = out[off[k]:off[k + 1]], out64[off[k]:off[k + 1]]
+        assert abs(p32.sum() - 1) < 1e-12
+        assert np.abs(p32 - p64).sum() <= 1e-5, (k, np.abs(p32 - p64).sum())
+
+
+def test_batch_max_iteration_and_zero(tb, oracle, gpu):
+    spec, mats, orcs = small_batch(tb, oracle)
+    off, out, rep, its = run_batch(tb, mats[:2], spec[:2], "f64", epsilon=0.0, max_iteration=3)
+    assert list(its) == [3, 3]
+    for k, o in enumerate(orcs[:2]):
+        pr, it = o.personalized_page_rank(0.0, 0.85, 3, np.array(spec[k][4][0], dtype=np.int32))
+        assert it == 3
+        assert np.abs(out[off[k]:off[k + 1]] - pr).sum() <= 1e-14

@@ -132,7 +132,7 @@ def declared_symbols():
     names = set()
     for h in ("tbsla_hip.h", "tbsla_cpp_c.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
-        names |= set(re.findall(r"\b(tbsla_(?:matrix|hip)_\w+|tbsla_abi_version|tbsla_last_error)\s*\(", txt))
+        names |= set(re.findall(r"\b(tbsla_(?:matrix|hip|batch)_\w+|tbsla_abi_version|tbsla_last_error)\s*\(", txt))
         for F in re.findall(r"TBSLA_C_MATRIX_DECL\((\w+)\)\n", txt):
             for op in ("create", "destroy", "fill_cdiag", "fill_cqmat", "read", "write", "print", "spmv"):
                 names.add(f"C_CPP_Matrix{F}_{op}")
@@ -162,3 +162,18 @@ def test_c_abi_errors_without_device_calls(tb):
     assert not _lib.lib.C_CPP_Vector_add(v, None)
     _lib.lib.C_CPP_Vector_destroy(v)
     _lib.lib.C_CPP_MatrixCSR_destroy(None)
+
+
+def test_batch_workload_generator(tb, oracle):
+    """The batch workload's counter generator equals the value-overlay
+    generator (u01), and the graph sizes / teleport nodes are in range."""
+    from tbsla_amd import batch
+    x = oracle.uniform_vector(1000, 8)
+    assert [batch.u01(1000, i) for i in range(8)] == list(x)
+    w = batch.personalised_workload()
+    assert len(w) == 257
+    for (n, c, q, seed, sets) in w[:-1]:
+        assert 1024 <= n <= 1 << 18 and c == 12 and q == 0.3 and len(sets) == 1 and 0 <= sets[0][0] < n
+    n, c, q, seed, sets = w[-1]
+    assert (n, c, q, seed, len(sets)) == (1 << 22, 16, 0.3, 1256, 64)
+    assert all(0 <= s[0] < n for s in sets)
