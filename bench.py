@@ -5,12 +5,20 @@ fill_cqmat(2^24, 2^24, c=16, q=0.1, seed 42) -- 4,194,304 x 4,194,304,
 62,075,592 entries -- with values 1 - u01(42, i) and x_j = u01(1, j).
 A step is one FP64 CSR SpMV of that block with the inputs resident in HBM.
 
-Also reported (extra fields): the other formats on the same block, and
-PageRank on fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) in FP64, FP32 and
-BF16 value storage and value-free FP64, each stopped by the sum-normalised
-bound and checked against the FP64 fixed point.
+Extra fields:
+  formats        ELL / SCOO / COO / DENSE on the same block
+  cpu_baseline   the reference's own MatrixCSR::Ax (compiled from its sources)
+                 on the host cores, and the bit-exact check of the GPU y
+  pagerank       fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) in FP64, FP32 and
+                 BF16 value storage and value-free FP64, each stopped by the
+                 sum-normalised bound, against the FP64 fixed point
+  batch          256 personalised graphs (n log-uniform in [2^10, 2^18]) plus
+                 a 2^22-node graph with 64 one-hot teleport sets, one hipGraph
+  adjoint        gradient of sum w_i p_i on fill_cqmat_stochastic(2^20, 2^20,
+                 16, 0.2, 3), FP32 and FP64 transposed products
 
     python bench.py [--steps K] [--warmup W] [--no-pagerank] [--no-cpu]
+                    [--no-formats] [--no-batch] [--no-adjoint]
 """
 from __future__ import annotations
 
@@ -62,16 +70,19 @@ def pmc_traffic(name):
     return None
 
 
-def cpu_baseline(tb_csr, x, bytes_alg):
-    """The reference's own CSR Ax (oracle/_ref, -O3 -fopenmp) on the same block."""
+def cpu_baseline(tb_csr, x, y_gpu, bytes_alg):
+    """The reference's own CSR Ax (oracle/_ref, -O3 -fopenmp) on the same
+    block, timed on the host cores; its result is also the checker of the
+    GPU product (bit-exact)."""
     from oracle.oracle import Reference
     if not Reference.available():
         return None
     a = tb_csr.arrays()
     info = tb_csr.info()
     m = Reference.from_arrays("CSR", info["ln_row"], info["ln_col"], a["values"], a["rowptr"], a["colidx"])
-    r = np.zeros(info["ln_row"])
-    Reference.Ax_repeat(m, r, x, 1)
+    del a
+    r = Reference.Ax_repeat(m, np.zeros(info["ln_row"]), x, 1)
+    exact = bool(np.array_equal(r, y_gpu))
     reps = 10
     t = time.perf_counter()
     Reference.Ax_repeat(m, r, x, reps)
@@ -79,7 +90,8 @@ def cpu_baseline(tb_csr, x, bytes_alg):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(bytes_alg / dt / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "reference",
             "sample": f"{reps} x MatrixCSR::Ax on the full block (reference sources, g++ -O3 -fopenmp), "
-                      f"{dt * 1e3:.1f} ms each"}
+                      f"{dt * 1e3:.1f} ms each",
+            "gpu_y_bit_exact_vs_reference": exact}
 
 
 def copy_calibration():
@@ -91,6 +103,123 @@ def copy_calibration():
     log(f"copy of {src.numel() * 8} bytes: {ms * 1e3:.1f} us, {2 * src.numel() * 8 / ms / 1e6:.0f} GB/s")
 
 
+def leg_formats(torch, tb, dev, steps):
+    fm = {}
+    for fmt in ["ELL", "SCOO", "COO"]:
+        m = tb.FORMATS[fmt]().fill_cqmat(N, N, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+        mi2 = m.mirror_info("f64")
+        nx = N if fmt == "COO" else m.get_ln_col()
+        xx = torch.from_numpy(tb.u01_array(1, 0, nx)).to(dev)
+        yy = torch.zeros(m.get_n_row(), dtype=torch.float64, device=dev)
+        ms2 = time_kernel(torch, lambda: m.Ax_device(xx, yy), max(10, steps // 4), 3)
+        fm[fmt] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
+                   "bytes": mi2["algorithmic_bytes"]}
+        log(f"{fmt} {ms2 * 1e3:.1f} us {fm[fmt]['gbs']} GB/s")
+        del m, xx, yy
+        torch.cuda.empty_cache()
+    m = tb.MatrixDENSE().fill_cqmat(32768, 32768, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+    mi2 = m.mirror_info("f64")
+    xx = torch.from_numpy(tb.u01_array(1, 0, 8192)).to(dev)
+    yy = torch.zeros(32768, dtype=torch.float64, device=dev)
+    ms2 = time_kernel(torch, lambda: m.Ax_device(xx, yy), 20, 3)
+    fm["DENSE"] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
+                   "bytes": mi2["algorithmic_bytes"]}
+    log(f"DENSE {ms2 * 1e3:.1f} us {fm['DENSE']['gbs']} GB/s")
+    return fm
+
+
+def leg_pagerank(torch, tb, dev):
+    log("generating the PageRank graph")
+    g = tb.MatrixCSR().fill_cqmat_stochastic(N, N, 8, 0.5, 7)
+    n = g.get_n_col()
+    pstar = torch.zeros(n, dtype=torch.float64, device=dev)
+    g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
+    pr = {}
+    for prec in ["f64", "f32", "bf16", "pattern"]:
+        o = torch.zeros(n, dtype=torch.float64, device=dev)
+        g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")  # warm
+        rep = g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")
+        err = float((o - pstar).abs().sum().item())
+        pr[prec] = {"seconds": round(rep["seconds"], 6), "iterations": rep["iterations"],
+                    "anchors": rep["anchors"], "l1_vs_fp64_fixed_point": err}
+        log(f"pagerank {prec}: {rep['seconds'] * 1e3:.2f} ms, {rep['iterations']} it, "
+            f"{rep['anchors']} anchors, L1 {err:.2e}")
+    base = pr["f64"]["seconds"]
+    for prec in pr:
+        pr[prec]["speedup_vs_fp64"] = round(base / pr[prec]["seconds"], 3)
+    return {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7), beta 0.85",
+            "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr}
+
+
+def leg_batch(torch, tb, dev):
+    log("generating the personalised batch (256 graphs + 2^22 x 64)")
+    t = time.time()
+    spec = tb.personalised_workload()
+    mats = [tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s) for (n, c, q, s, _) in spec]
+    log(f"generated in {time.time() - t:.1f}s")
+    res = {}
+    outs = {}
+    for prec, eps in (("f32", 1e-6), ("f64", 1e-10)):
+        b = tb.PageRankBatch(prec)
+        for m, sp in zip(mats, spec):
+            b.add_graph(m, sp[4])
+        t = time.time()
+        b.prepare()
+        prep = time.time() - t
+        out, rep, its = b.run(epsilon=eps, rule="sumnorm", max_iteration=1000, check_every=16)
+        out2, rep2, _ = b.run(out=out, epsilon=eps, rule="sumnorm", max_iteration=1000, check_every=16)
+        res[prec] = {"seconds": round(rep2["seconds"], 6), "iterations_max": int(rep2["iterations"]),
+                     "iterations_total": int(its.sum()), "stop": f"sum-normalised bound <= {eps}*(1-beta)/beta",
+                     "bytes_per_iteration": rep2["bytes_per_iteration"],
+                     "autotuned": rep["config"], "tune_seconds": round(rep["tune_seconds"], 3),
+                     "prepare_seconds": round(prep, 2)}
+        outs[prec] = (b.layout(), out2.cpu().numpy())
+        log(f"batch {prec}: {rep2['seconds'] * 1e3:.1f} ms, max {rep2['iterations']} it, config {rep['config']}")
+        del b, out, out2
+        torch.cuda.empty_cache()
+    off, p32 = outs["f32"]
+    _, p64 = outs["f64"]
+    l1 = [float(np.abs(p32[off[k]:off[k + 1]] - p64[off[k]:off[k + 1]]).sum()) for k in range(len(off) - 1)]
+    res["f32"]["max_l1_vs_f64_batch"] = max(l1)
+    # how the reference runs it: one personalised call per teleport set (sampled)
+    big, (n, c, q, s, sets) = mats[-1], spec[-1]
+    o = torch.zeros(n, dtype=torch.float64, device=dev)
+    secs = []
+    for nodes in sets[:4]:
+        rep = big.page_rank_device(o, epsilon=1e-10, max_iteration=1000, precision="f64", rule="sumnorm", nodes=nodes)
+        secs.append(rep["seconds"])
+    res["single_call_per_set_f64"] = {"sample": "4 of the 64 sets of the 2^22 graph, FP64 single-graph engine",
+                                      "seconds_per_set": round(float(np.mean(secs)), 6)}
+    return {"workload": "256 x fill_cqmat_stochastic(n_g,n_g,12,0.3,1000+g), n_g log-uniform [2^10,2^18], one "
+                        "one-hot set each; fill_cqmat_stochastic(2^22,2^22,16,0.3,1256) with 64 one-hot sets",
+            "n_sets": len(off) - 1, "variants": res}
+
+
+def leg_adjoint(torch, tb, dev):
+    log("generating the adjoint graph")
+    n = 1 << 20
+    g = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 16, 0.2, 3)
+    p = torch.empty(n, dtype=torch.float64, device=dev)
+    fwd = g.page_rank_device(p, epsilon=1e-15, max_iteration=400, rule="sumnorm")
+    w = torch.from_numpy(tb.normal_array(3, n)).to(dev)
+    ne = g.n_entries()
+    res = {"forward_seconds": round(fwd["seconds"], 6), "forward_iterations": fwd["iterations"]}
+    grads = {}
+    for prec, eps in (("f32", 1e-7), ("f64", 1e-13)):
+        gv = torch.zeros(ne, dtype=torch.float64, device=dev)
+        gt = torch.zeros(n, dtype=torch.float64, device=dev)
+        g.page_rank_adjoint(p, w, gv, gt, precision=prec, epsilon=eps)  # warm (builds A^T)
+        rep = g.page_rank_adjoint(p, w, gv, gt, precision=prec, epsilon=eps)
+        res[prec] = {"seconds": round(rep["seconds"], 6), "iterations": rep["iterations"],
+                     "last_change": rep["last_change"]}
+        grads[prec] = (gv, gt)
+        log(f"adjoint {prec}: {rep['seconds'] * 1e3:.2f} ms, {rep['iterations']} it")
+    (a, b), (c, d) = grads["f32"], grads["f64"]
+    res["f32"]["rel_err_edge_grad_vs_f64"] = float(((a - c).abs().max() / c.abs().max()).item())
+    res["f32"]["rel_err_teleport_grad_vs_f64"] = float(((b - d).abs().max() / d.abs().max()).item())
+    return {"graph": "fill_cqmat_stochastic(2^20,2^20,16,0.2,3), beta 0.85, w ~ N(0,1) seed 3", **res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +228,8 @@ def main():
     ap.add_argument("--no-pagerank", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-formats", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (the reference kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -113,7 +244,6 @@ def main():
 
     import torch
     import tbsla_amd as tb
-    from oracle.oracle import Oracle
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -122,7 +252,7 @@ def main():
     csr = tb.MatrixCSR().fill_cqmat(N, N, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
     log(f"generated in {time.time() - t:.1f}s")
     info = csr.info()
-    x_host = Oracle.uniform_vector(1, info["ln_col"])
+    x_host = tb.u01_array(1, 0, info["ln_col"])
     x = torch.from_numpy(x_host).to(dev)
     y = torch.zeros(info["ln_row"], dtype=torch.float64, device=dev)
     csr.prepare("f64")
@@ -130,13 +260,11 @@ def main():
     ms = time_kernel(torch, lambda: csr.Ax_device(x, y), args.steps, args.warmup)
     gbs = mi["algorithmic_bytes"] / (ms * 1e-3) / 1e9
     log(f"CSR {ms * 1e3:.1f} us/launch {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
-    # one parity probe on a slice (the full check lives in the tests)
-    a = csr.arrays()
-    e = a["rowptr"][4096]
-    yo = Oracle.from_arrays("CSR", 4096, info["ln_col"], a["values"][:e], a["rowptr"][:4097],
-                            a["colidx"][:e]).spmv(x_host)
-    parity = bool(np.array_equal(y[:4096].cpu().numpy(), yo))
-    del a
+    # the C-ABI host-vector path (x and y cross PCIe inside the call)
+    t = time.perf_counter()
+    for _ in range(5):
+        csr.spmv(x_host)
+    host_ms = (time.perf_counter() - t) / 5 * 1e3
 
     out = {"metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
@@ -144,63 +272,27 @@ def main():
            "config": {"workload": "CSR SpMV, block (0,0) of a 4x4 grid over fill_cqmat(2^24,2^24,16,0.1,42)",
                       "n_row": info["ln_row"], "n_col": info["ln_col"], "nnz": info["n_values"],
                       "values": "1-u01(42,i)", "x": "u01(1,j)", "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
-                      "parity_first_4096_rows_bit_exact": parity},
+                      "host_vector_spmv_ms": round(host_ms, 2)},
            "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("csr_tile_kernel_cqmat2^24_block"),
                         "kernel": "csr_tile_kernel<double,double,double>",
                         "algorithmic_bytes_per_launch": mi["algorithmic_bytes"]}}
 
-    if not args.no_formats:
-        fm = {}
-        for fmt in ["ELL", "SCOO", "COO"]:
-            m = tb.FORMATS[fmt]().fill_cqmat(N, N, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
-            mi2 = m.mirror_info("f64")
-            nx = N if fmt == "COO" else m.get_ln_col()
-            xx = torch.from_numpy(Oracle.uniform_vector(1, nx)).to(dev)
-            yy = torch.zeros(m.get_n_row(), dtype=torch.float64, device=dev)
-            ms2 = time_kernel(torch, lambda: m.Ax_device(xx, yy), max(10, args.steps // 4), 3)
-            fm[fmt] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
-                       "bytes": mi2["algorithmic_bytes"]}
-            log(f"{fmt} {ms2 * 1e3:.1f} us {fm[fmt]['gbs']} GB/s")
-            del m, xx, yy
-            torch.cuda.empty_cache()
-        m = tb.MatrixDENSE().fill_cqmat(32768, 32768, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
-        mi2 = m.mirror_info("f64")
-        xx = torch.from_numpy(Oracle.uniform_vector(1, 8192)).to(dev)
-        yy = torch.zeros(32768, dtype=torch.float64, device=dev)
-        ms2 = time_kernel(torch, lambda: m.Ax_device(xx, yy), 20, 3)
-        fm["DENSE"] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
-                       "bytes": mi2["algorithmic_bytes"]}
-        log(f"DENSE {ms2 * 1e3:.1f} us {fm['DENSE']['gbs']} GB/s")
-        out["formats"] = fm
-
     if not args.no_cpu:
         log("timing the reference CPU path")
-        out["cpu_baseline"] = cpu_baseline(csr, x_host, mi["algorithmic_bytes"])
+        out["cpu_baseline"] = cpu_baseline(csr, x_host, y.cpu().numpy(), mi["algorithmic_bytes"])
     del csr, x, y
     torch.cuda.empty_cache()
-
+    if not args.no_formats:
+        out["formats"] = leg_formats(torch, tb, dev, args.steps)
     if not args.no_pagerank:
-        log("generating the PageRank graph")
-        g = tb.MatrixCSR().fill_cqmat_stochastic(N, N, 8, 0.5, 7)
-        n = g.get_n_col()
-        pstar = torch.zeros(n, dtype=torch.float64, device=dev)
-        g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
-        pr = {}
-        for prec in ["f64", "f32", "bf16", "pattern"]:
-            o = torch.zeros(n, dtype=torch.float64, device=dev)
-            g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")  # warm
-            rep = g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")
-            err = float((o - pstar).abs().sum().item())
-            pr[prec] = {"seconds": round(rep["seconds"], 6), "iterations": rep["iterations"],
-                        "anchors": rep["anchors"], "l1_vs_fp64_fixed_point": err}
-            log(f"pagerank {prec}: {rep['seconds'] * 1e3:.2f} ms, {rep['iterations']} it, "
-                f"{rep['anchors']} anchors, L1 {err:.2e}")
-        base = pr["f64"]["seconds"]
-        for prec in pr:
-            pr[prec]["speedup_vs_fp64"] = round(base / pr[prec]["seconds"], 3)
-        out["pagerank"] = {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7), beta 0.85",
-                           "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr}
+        out["pagerank"] = leg_pagerank(torch, tb, dev)
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        out["batch"] = leg_batch(torch, tb, dev)
+        torch.cuda.empty_cache()
+    if not args.no_adjoint:
+        out["adjoint"] = leg_adjoint(torch, tb, dev)
 
     print(json.dumps(out), flush=True)
 

@@ -8,7 +8,7 @@ first use of a matrix class, so ``python -m tbsla_amd.build`` works before it
 exists.
 """
 _MATRIX = ["FORMATS", "Matrix", "MatrixCOO", "MatrixSCOO", "MatrixCSR", "MatrixELL", "MatrixDENSE"]
-_BATCH = ["PageRankBatch", "personalised_workload", "u01"]
+_BATCH = ["PageRankBatch", "personalised_workload", "u01", "u01_array", "normal_array"]
 __all__ = _MATRIX + _BATCH
 
 

@@ -41,6 +41,23 @@ def u01(seed: int, i: int) -> float:
     return (_splitmix64(_splitmix64(seed) ^ i) >> 11) * 2.0 ** -53
 
 
+def u01_array(seed: int, first: int, n: int) -> np.ndarray:
+    """u01(seed, i) for i in [first, first + n), vectorised."""
+    i = np.arange(first, first + n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(_splitmix64(seed)) ^ i) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+
+
+def normal_array(seed: int, n: int) -> np.ndarray:
+    """N(0,1) by Box-Muller: sqrt(-2 ln(1 - u01(seed, 2i))) cos(2 pi u01(seed, 2i+1))."""
+    u = u01_array(seed, 0, 2 * n)
+    return np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
+
+
 def personalised_workload(n_graphs: int = 256, big_n: int = 1 << 22, big_rhs: int = 64):
     """The batch: graph g (seed 1000+g) has n_g = floor(2^(10 + 8 u01(seed, 0)) + 0.5)
     nodes, c = 12, q = 0.3 and one teleport node floor(u01(seed, 1) * n_g);

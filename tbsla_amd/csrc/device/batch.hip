@@ -135,52 +135,142 @@ struct TileEpi {
   __device__ void operator()(int row, float s) { (*this)(row, static_cast<double>(s)); }
 };
 
+// Four consecutive iterate values (the RHS quad of one lane), 16-byte loads.
+template <typename XT>
+struct Quad;
+template <>
+struct Quad<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static void load(const float *p, float (&o)[4]) {
+    const v4 a = *reinterpret_cast<const v4 *>(p);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+    o[3] = a.w;
+  }
+  __device__ static void store(float *p, const float (&o)[4]) { *reinterpret_cast<v4 *>(p) = v4{o[0], o[1], o[2], o[3]}; }
+};
+template <>
+struct Quad<double> {
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  __device__ static void load(const double *p, double (&o)[4]) {
+    const v2 a = reinterpret_cast<const v2 *>(p)[0], b = reinterpret_cast<const v2 *>(p)[1];
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+  }
+  __device__ static void store(double *p, const double (&o)[4]) {
+    reinterpret_cast<v2 *>(p)[0] = v2{o[0], o[1]};
+    reinterpret_cast<v2 *>(p)[1] = v2{o[2], o[3]};
+  }
+};
+
+// SpMM rows of one work item (64 rows): each wavefront keeps four rows in
+// flight, 16 lanes per row, lane quad q holding RHS 4q..4q+3.  A row's
+// column indices and values are loaded 16 at a time (one entry per lane) and
+// broadcast inside the 16-lane group; UNR gathers of 4 consecutive iterate
+// values are issued before they are accumulated, in entry order, so every
+// (row, RHS) sum is the reference's left-to-right sum.
 template <typename VT, typename XT, int UNR>
 __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int blk, const XT *__restrict__ xin,
                           XT *__restrict__ xout, double *red) {
 #pragma clang fp contract(off)
   using AT = typename dev::Val<VT>::A;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int rhs = G.rhs_base + lane;
-  const bool active = lane < G.n_rhs && !d.rs[rhs].done;
-  const double tele = active ? d.rs[rhs].tele : 0.0;
+  const int sub = lane >> 4, q = lane & 15;
+  const int rows = G.rows, rp0 = G.rp0, f_col = G.f_col, n_rhs = G.n_rhs, rhs0 = G.rhs_base + 4 * q;
+  const int *__restrict__ rowptr = G.rowptr;
+  const int *__restrict__ colidx = G.colidx;
+  const VT *__restrict__ vals = G.vals;
+  const unsigned long long *__restrict__ member = G.member;
+  bool act[4];
+  double tele[4], lmax[4];
+  bool any = false;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    act[t] = 4 * q + t < n_rhs && !d.rs[rhs0 + t].done;
+    tele[t] = act[t] ? d.rs[rhs0 + t].tele : 0.0;
+    lmax[t] = -INFINITY;
+    any = any || act[t];
+  }
   const XT *X = xin + G.xoff;
   XT *Y = xout + G.xoff;
-  double lmax = -INFINITY;
-  if (__ballot(active) != 0) {
-    for (int i = 0; i < kSpmmRowsPerWave; i++) {
-      const int row = __builtin_amdgcn_readfirstlane(blk * kSpmmRows + wave * kSpmmRowsPerWave + i);
-      if (row >= G.rows) break;
-      const int a0 = G.rowptr[row] - G.rp0, a1 = G.rowptr[row + 1] - G.rp0;
-      AT acc = 0;
-      for (int e = a0; e < a1; e += UNR) {
-        AT vv[UNR], xv[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; u++) {
-          if (e + u < a1) {
-            const long c = G.colidx[e + u] - G.f_col;
-            vv[u] = dev::Val<VT>::get(G.vals, e + u);
-            xv[u] = active ? static_cast<AT>(X[c * kLanes + lane]) : AT(0);
-          }
+  if (__ballot(any) != 0) {
+#pragma unroll 1
+    for (int pass = 0; pass < kSpmmRowsPerWave / 4; pass++) {
+      const int row = blk * kSpmmRows + wave * kSpmmRowsPerWave + pass * 4 + sub;
+      const bool rv = row < rows;
+      const int a0 = rv ? rowptr[row] - rp0 : 0, a1 = rv ? rowptr[row + 1] - rp0 : 0;
+      AT acc[4] = {0, 0, 0, 0};
+      for (int base = a0; __ballot(base < a1) != 0; base += 16) {
+        const int cnt = min(16, a1 - base);  // <= 0 for a finished row
+        int cq = 0;
+        AT vq = 0;
+        if (q < cnt) {
+          cq = colidx[base + q] - f_col;
+          vq = dev::Val<VT>::get(vals, base + q);
         }
+#pragma unroll 1
+        for (int k0 = 0; k0 < 16; k0 += UNR) {
+          AT xv[UNR][4], vv[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; u++)
-          if (e + u < a1) acc += vv[u] * xv[u];
+          for (int u = 0; u < UNR; u++) {
+            const int src = (sub << 4) + k0 + u;
+            const int c = __shfl(cq, src, 64);
+            vv[u] = __shfl(vq, src, 64);
+            if (k0 + u < cnt && any) {
+              XT t4[4];
+              Quad<XT>::load(X + static_cast<long>(c) * kLanes + 4 * q, t4);
+#pragma unroll
+              for (int t = 0; t < 4; t++) xv[u][t] = static_cast<AT>(t4[t]);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < UNR; u++)
+            if (k0 + u < cnt)
+#pragma unroll
+              for (int t = 0; t < 4; t++) acc[t] += vv[u] * xv[u][t];
+        }
       }
-      if (active) {
-        double bi = d.beta * static_cast<double>(acc);
-        if ((G.member[row] >> lane) & 1ull) bi = bi + tele;
-        const XT bs = static_cast<XT>(bi);
-        Y[static_cast<long>(row) * kLanes + lane] = bs;
-        lmax = fmax(lmax, static_cast<double>(bs));
+      if (rv && any) {
+        const unsigned long long mb = member[row] >> (4 * q);
+        XT o[4];
+        bool all = true;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          double bi = d.beta * static_cast<double>(acc[t]);
+          if ((mb >> t) & 1ull) bi = bi + tele[t];
+          o[t] = static_cast<XT>(bi);
+          if (act[t]) lmax[t] = fmax(lmax[t], static_cast<double>(o[t]));
+          all = all && act[t];
+        }
+        XT *yp = Y + static_cast<long>(row) * kLanes + 4 * q;
+        if (all) {
+          Quad<XT>::store(yp, o);
+        } else {
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            if (act[t]) yp[t] = o[t];  // a finished RHS keeps its final iterate
+        }
       }
     }
   }
-  red[threadIdx.x] = lmax;
+  // maxima: the four row groups of the wave, then the four wavefronts
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    lmax[t] = fmax(lmax[t], __shfl_xor(lmax[t], 16, 64));
+    lmax[t] = fmax(lmax[t], __shfl_xor(lmax[t], 32, 64));
+  }
+  if (sub == 0)
+#pragma unroll
+    for (int t = 0; t < 4; t++) red[wave * 64 + 4 * q + t] = lmax[t];
   __syncthreads();
   if (wave == 0) {
-    const double m = fmax(fmax(red[lane], red[64 + lane]), fmax(red[128 + lane], red[192 + lane]));
-    if (active && m > -INFINITY) atomicMax(d.slots + rhs * kSlots + (blk % kSlots), ord_encode(m));
+    const int j = lane;  // RHS j of the group
+    const double m = fmax(fmax(red[j], red[64 + j]), fmax(red[128 + j], red[192 + j]));
+    if (j < n_rhs && m > -INFINITY && !d.rs[G.rhs_base + j].done)
+      atomicMax(d.slots + (G.rhs_base + j) * kSlots + (blk % kSlots), ord_encode(m));
   }
   __syncthreads();
 }

@@ -53,7 +53,7 @@ struct BatchParams {
 
 struct BatchConfig {
   int wg_per_cu = 4;  // persistent workgroups per compute unit
-  int unroll = 8;     // gathers in flight per SpMM row step (4 or 8)
+  int unroll = 8;     // gathers in flight per lane in the SpMM (4 or 8)
   int xcd_map = 1;    // 1: graphs / row ranges pinned to XCD queues; 0: interleaved
   std::string str() const;
 };

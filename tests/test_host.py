@@ -170,6 +170,10 @@ def test_batch_workload_generator(tb, oracle):
     from tbsla_amd import batch
     x = oracle.uniform_vector(1000, 8)
     assert [batch.u01(1000, i) for i in range(8)] == list(x)
+    assert np.array_equal(batch.u01_array(7, 0, 1 << 16), oracle.uniform_vector(7, 1 << 16))
+    u = oracle.uniform_vector(3, 2000)
+    w = np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
+    assert np.array_equal(batch.normal_array(3, 1000), w)
     w = batch.personalised_workload()
     assert len(w) == 257
     for (n, c, q, seed, sets) in w[:-1]:
