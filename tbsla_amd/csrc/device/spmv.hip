@@ -8,7 +8,8 @@
 //                   MatrixCSR.cpp:133-139 bit for bit (product rounded, then a
 //                   left-to-right sum starting from 0).
 // csr_long_kernel   rows longer than 1024 entries, one workgroup each.
-// ell_kernel        one thread per row over the column-major slots, 4 slots
+// ell_kernel        four rows per thread over the column-major slots, 16-byte
+//                   loads of four rows' indices and values per slot, two slots
 //                   per step; same order as MatrixELL.cpp:105-113 (contraction
 //                   off, so bit-identical in FP64).
 // seg_kernel        COO/SCOO: one workgroup per segment-aligned tile of the
@@ -81,37 +82,56 @@ __global__ __launch_bounds__(kTileThreads) void csr_long_kernel(dev::CsrView<VT>
   dev::csr_long_row<VT, XT>(m, x, lds, epi);
 }
 
+// Four consecutive rows per thread: per slot one 16-byte load of the four
+// column indices and 16 bytes (F32) or 32 bytes (F64) of values, then four
+// gathers; each row still sums its slots in order 0..K-1.
+template <typename VT>
+struct Val4 {
+  __device__ static void get(const VT *p, long i, typename Val<VT>::A (&v)[4]) { Val<VT>::get4(p, i, v); }
+};
+
 template <typename VT, typename XT, typename YT>
 __global__ __launch_bounds__(256) void ell_kernel(const int *__restrict__ cols, const VT *__restrict__ vals,
                                                   const XT *__restrict__ x, YT *__restrict__ y, int rows, int K,
                                                   long ld, int f_col) {
 #pragma clang fp contract(off)
   using AT = typename Val<VT>::A;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= rows) return;
-  AT tmp = 0;
+  const long i0 = (static_cast<long>(blockIdx.x) * 256 + threadIdx.x) * 4;
+  if (i0 >= rows) return;
+  AT acc[4] = {0, 0, 0, 0};
   int j = 0;
-  for (; j + 4 <= K; j += 4) {
-    int c[4];
-    AT v[4];
+  for (; j + 2 <= K; j += 2) {
+    int c[2][4];
+    AT v[2][4], xv[2][4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      c[q] = nt_load(cols, (j + q) * ld + i) - f_col;
-      if (c[q] < 0) c[q] = 0;
-      v[q] = Val<VT>::get(vals, (j + q) * ld + i);
+    for (int u = 0; u < 2; u++) {
+      const dev::v4i ci = __builtin_nontemporal_load(reinterpret_cast<const dev::v4i *>(cols + (j + u) * ld + i0));
+      c[u][0] = ci.x - f_col;
+      c[u][1] = ci.y - f_col;
+      c[u][2] = ci.z - f_col;
+      c[u][3] = ci.w - f_col;
+      Val4<VT>::get(vals, (j + u) * ld + i0, v[u]);
     }
-    AT xv[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) xv[q] = static_cast<AT>(x[c[q]]);
+    for (int u = 0; u < 2; u++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) tmp += v[q] * xv[q];
+      for (int r = 0; r < 4; r++) xv[u][r] = static_cast<AT>(x[c[u][r] < 0 ? 0 : c[u][r]]);
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[r] += v[u][r] * xv[u][r];
   }
   for (; j < K; j++) {
-    int cc = nt_load(cols, j * ld + i) - f_col;
-    if (cc < 0) cc = 0;
-    tmp += Val<VT>::get(vals, j * ld + i) * static_cast<AT>(x[cc]);
+    const dev::v4i ci = *reinterpret_cast<const dev::v4i *>(cols + j * ld + i0);
+    const int c[4] = {ci.x - f_col, ci.y - f_col, ci.z - f_col, ci.w - f_col};
+    AT v[4];
+    Val4<VT>::get(vals, j * ld + i0, v);
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[r] += v[r] * static_cast<AT>(x[c[r] < 0 ? 0 : c[r]]);
   }
-  y[i] = static_cast<YT>(tmp);
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+    if (i0 + r < rows) y[i0 + r] = static_cast<YT>(acc[r]);
 }
 
 template <typename VT, typename XT, typename YT>
@@ -195,7 +215,24 @@ __global__ __launch_bounds__(256) void dense_kernel(const double *__restrict__ A
   if (w >= rows) return;
   const double *a = A + static_cast<long>(w) * ld;
   double acc = 0;
-  for (int j = lane; j < cols; j += 64) acc += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
+  if ((ld & 1) == 0) {
+    // 16-byte non-temporal loads, two columns per lane, four steps in flight
+    int j = 2 * lane;
+    for (; j + 3 * 128 + 1 < cols; j += 4 * 128) {
+      dev::v2d av[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) av[u] = __builtin_nontemporal_load(reinterpret_cast<const dev::v2d *>(a + j + u * 128));
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        acc += av[u].x * static_cast<double>(x[j + u * 128]) + av[u].y * static_cast<double>(x[j + u * 128 + 1]);
+    }
+    for (; j < cols; j += 128) {
+      acc += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
+      if (j + 1 < cols) acc += __builtin_nontemporal_load(a + j + 1) * static_cast<double>(x[j + 1]);
+    }
+  } else {
+    for (int j = lane; j < cols; j += 64) acc += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if (lane == 0) y[w] = static_cast<YT>(static_cast<double>(y[w]) + acc);
@@ -221,7 +258,7 @@ void run_typed(const DeviceMatrix &d, const XT *x, YT *y, int vect_incr, hipStre
     }
     case cpp::FORMAT_ELL:
       if (d.rows > 0)
-        hipLaunchKernelGGL((ell_kernel<VT, XT, YT>), dim3(blocks_for(d.rows, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL((ell_kernel<VT, XT, YT>), dim3(blocks_for((d.rows + 3) / 4, 256)), dim3(256), 0, s,
                            d.ecols.as<int>(), d.evals.as<VT>(), x, y, d.rows, d.K, d.ld, d.f_col);
       break;
     case cpp::FORMAT_COO:
