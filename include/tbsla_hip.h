@@ -91,6 +91,8 @@ typedef struct {
   double anchor_tau; /* 0: default */
   const int *nodes;  /* personalised teleport set, or NULL */
   int n_nodes;
+  int gather;        /* CSR products: 0 auto (propagation-blocked for rule 1 and n >= 2^16),
+                        1 fixed-order entry tiles, 2 propagation-blocked */
 } tbsla_pagerank_params;
 
 typedef struct {

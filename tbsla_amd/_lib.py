@@ -29,7 +29,8 @@ class TbslaError(RuntimeError):
 class PageRankParams(C.Structure):
     _fields_ = [("epsilon", C.c_double), ("beta", C.c_double), ("max_iteration", C.c_int),
                 ("precision", C.c_int), ("rule", C.c_int), ("check_every", C.c_int),
-                ("anchor_tau", C.c_double), ("nodes", C.POINTER(C.c_int)), ("n_nodes", C.c_int)]
+                ("anchor_tau", C.c_double), ("nodes", C.POINTER(C.c_int)), ("n_nodes", C.c_int),
+                ("gather", C.c_int)]
 
 
 class PageRankReport(C.Structure):

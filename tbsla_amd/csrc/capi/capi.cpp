@@ -566,6 +566,8 @@ int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, cons
     q.rule = static_cast<hip::PrRule>(p->rule);
     q.check_every = p->check_every > 0 ? p->check_every : 8;
     q.anchor_tau = p->anchor_tau;
+    if (p->gather < 0 || p->gather > 2) throw std::invalid_argument("tbsla: bad gather mode");
+    q.gather = p->gather;
     hip::PageRankReport r = hip::page_rank_device(*m->m, q, out_dev, static_cast<hipStream_t>(stream));
     if (report) {
       report->iterations = r.iterations;

@@ -331,6 +331,132 @@ const DeviceMatrix &transpose_mirror(const cpp::Matrix &m) {
   return *cache.transposed;
 }
 
+const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
+  if (m.format() != cpp::FORMAT_CSR) bad("the propagation-blocked layout needs a CSR matrix");
+  const auto &c = dynamic_cast<const cpp::MatrixCSR &>(m);
+  DeviceCache &cache = m.device_cache();
+  std::lock_guard<std::mutex> g(cache.mu);
+  if (!cache.pb) {
+    const auto &rp = c.get_rowptr();
+    const auto &ci = c.get_colidx();
+    const int n = c.get_n_col();
+    if (c.get_n_row() != n || static_cast<int>(rp.size()) != n + 1) bad("the propagation-blocked layout needs a square matrix in one block");
+    const long r0 = rp[0];
+    const long nnz = static_cast<long>(rp.back()) - r0;
+    if (static_cast<long>(ci.size()) < r0 + nnz) bad("CSR rowptr points past colidx");
+    auto pb = std::unique_ptr<PbMatrix>(new PbMatrix());
+    pb->n = n;
+    pb->nbins = (n + kPbRows - 1) / kPbRows;
+    pb->nsb = (n + kPbCols - 1) / kPbCols;
+    const int nb = pb->nbins, ns = pb->nsb;
+    // pass 1: counts per bin (far, local) and per column (far)
+    std::vector<int> colcnt(static_cast<std::size_t>(n) + 1, 0), bincnt(static_cast<std::size_t>(nb) + 1, 0),
+        loccnt(static_cast<std::size_t>(nb) + 1, 0);
+    for (int r = 0; r < n; r++) {
+      const int b = r >> kPbShift;
+      for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
+        const int col = ci[static_cast<std::size_t>(r0 + e)] - c.get_f_col();
+        if (col < 0 || col >= n) bad("CSR column index outside the matrix");
+        if ((col >> kPbShift) == b) {
+          loccnt[static_cast<std::size_t>(b) + 1]++;
+        } else {
+          colcnt[static_cast<std::size_t>(col) + 1]++;
+          bincnt[static_cast<std::size_t>(b) + 1]++;
+        }
+      }
+    }
+    for (int j = 0; j < n; j++) colcnt[static_cast<std::size_t>(j) + 1] += colcnt[static_cast<std::size_t>(j)];
+    for (int b = 0; b < nb; b++) {
+      bincnt[static_cast<std::size_t>(b) + 1] += bincnt[static_cast<std::size_t>(b)];
+      loccnt[static_cast<std::size_t>(b) + 1] += loccnt[static_cast<std::size_t>(b)];
+    }
+    pb->n_far = colcnt[static_cast<std::size_t>(n)];
+    pb->n_loc = loccnt[static_cast<std::size_t>(nb)];
+    if (pb->n_far >= (1L << 31) || pb->n_loc >= (1L << 31)) bad("propagation-blocked layout limited to 2^31 entries");
+    const std::size_t nf = static_cast<std::size_t>(pb->n_far), nl = static_cast<std::size_t>(pb->n_loc);
+    // pass 2: far entries in column order (rows ascending), local entries bin-grouped
+    std::vector<int> csc_src(nf), csc_row(nf);
+    pb->loc_src.resize(nl);
+    std::vector<unsigned short> lrow(nl), lcol(nl);
+    std::vector<int> ccur(colcnt.begin(), colcnt.end() - 1), lcur(loccnt.begin(), loccnt.end() - 1);
+    for (int r = 0; r < n; r++) {
+      const int b = r >> kPbShift;
+      for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
+        const int col = ci[static_cast<std::size_t>(r0 + e)] - c.get_f_col();
+        if ((col >> kPbShift) == b) {
+          const int k = lcur[static_cast<std::size_t>(b)]++;
+          pb->loc_src[static_cast<std::size_t>(k)] = static_cast<int>(e);
+          lrow[static_cast<std::size_t>(k)] = static_cast<unsigned short>(r & (kPbRows - 1));
+          lcol[static_cast<std::size_t>(k)] = static_cast<unsigned short>(col & (kPbRows - 1));
+        } else {
+          const int k = ccur[static_cast<std::size_t>(col)]++;
+          csc_src[static_cast<std::size_t>(k)] = static_cast<int>(e);
+          csc_row[static_cast<std::size_t>(k)] = r;
+        }
+      }
+    }
+    // bin-grouped slot of every far entry: bins in order, each in column order
+    std::vector<int> slot_csc(nf), bcur(bincnt.begin(), bincnt.end() - 1);
+    std::vector<unsigned short> frow(nf);
+    for (std::size_t k = 0; k < nf; k++) {
+      const int r = csc_row[k];
+      const int sl = bcur[static_cast<std::size_t>(r >> kPbShift)]++;
+      slot_csc[k] = sl;
+      frow[static_cast<std::size_t>(sl)] = static_cast<unsigned short>(r & (kPbRows - 1));
+    }
+    // scatter order: superblocks in order, within a superblock by (bin, column)
+    std::vector<int> spc(static_cast<std::size_t>(ns) + 1, 0);
+    for (int s2 = 0; s2 < ns; s2++)
+      spc[static_cast<std::size_t>(s2) + 1] = colcnt[static_cast<std::size_t>(std::min(n, (s2 + 1) * kPbCols))];
+    pb->far_src.resize(nf);
+    std::vector<int> fslot(nf);
+    std::vector<unsigned short> fcol(nf);
+    {
+      std::vector<int> cnt(static_cast<std::size_t>(nb) + 1);
+      for (int s2 = 0; s2 < ns; s2++) {
+        const int k0 = spc[static_cast<std::size_t>(s2)], k1 = spc[static_cast<std::size_t>(s2) + 1];
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int k = k0; k < k1; k++) cnt[static_cast<std::size_t>(csc_row[static_cast<std::size_t>(k)] >> kPbShift) + 1]++;
+        for (int b = 0; b < nb; b++) cnt[static_cast<std::size_t>(b) + 1] += cnt[static_cast<std::size_t>(b)];
+        for (int k = k0; k < k1; k++) {  // column order, so each bin's run stays in column order
+          const int b = csc_row[static_cast<std::size_t>(k)] >> kPbShift;
+          const int o = k0 + cnt[static_cast<std::size_t>(b)]++;
+          pb->far_src[static_cast<std::size_t>(o)] = csc_src[static_cast<std::size_t>(k)];
+          fslot[static_cast<std::size_t>(o)] = slot_csc[static_cast<std::size_t>(k)];
+          fcol[static_cast<std::size_t>(o)] = static_cast<unsigned short>(
+              (ci[static_cast<std::size_t>(r0 + csc_src[static_cast<std::size_t>(k)])] - c.get_f_col()) & (kPbCols - 1));
+        }
+      }
+    }
+    pb->sp.upload(spc.data(), spc.size() * sizeof(int));
+    pb->fcol.upload(fcol.data(), fcol.size() * sizeof(unsigned short));
+    pb->fslot.upload(fslot.data(), fslot.size() * sizeof(int));
+    pb->frow.upload(frow.data(), frow.size() * sizeof(unsigned short));
+    pb->bp.upload(bincnt.data(), bincnt.size() * sizeof(int));
+    pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));
+    pb->lrow.upload(lrow.data(), lrow.size() * sizeof(unsigned short));
+    pb->lcol.upload(lcol.data(), lcol.size() * sizeof(unsigned short));
+    pb->prod.alloc(sizeof(double) * std::max<std::size_t>(1, nf));
+    cache.pb = std::move(pb);
+  }
+  PbMatrix &pb = *cache.pb;
+  if (vdt != Dtype::PATTERN && !pb.fval[static_cast<int>(vdt)].get()) {
+    const auto &va = c.get_values();
+    const long r0 = c.get_rowptr()[0];
+    if (static_cast<long>(va.size()) < r0 + static_cast<long>(pb.far_src.size() + pb.loc_src.size()))
+      bad("CSR rowptr points past values");
+    std::vector<double> fv(pb.far_src.size()), lv(pb.loc_src.size());
+    for (std::size_t k = 0; k < fv.size(); k++) fv[k] = va[static_cast<std::size_t>(r0 + pb.far_src[k])];
+    for (std::size_t k = 0; k < lv.size(); k++) lv[k] = va[static_cast<std::size_t>(r0 + pb.loc_src[k])];
+    upload_values(pb.fval[static_cast<int>(vdt)], fv.data(), fv.size(), vdt);
+    upload_values(pb.lval[static_cast<int>(vdt)], lv.data(), lv.size(), vdt);
+    if (fv.empty()) pb.fval[static_cast<int>(vdt)].alloc(8);
+    if (lv.empty()) pb.lval[static_cast<int>(vdt)].alloc(8);
+  }
+  TBSLA_HIP_CHECK(hipDeviceSynchronize());
+  return pb;
+}
+
 const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt) {
   DeviceCache &cache = m.device_cache();
   std::lock_guard<std::mutex> g(cache.mu);

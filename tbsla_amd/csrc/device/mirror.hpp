@@ -60,10 +60,37 @@ struct DeviceMatrix {
   std::size_t algorithmic_bytes(std::size_t x_elem, std::size_t y_elem) const;
 };
 
+// Propagation-blocked layout of a square CSR matrix for PageRank.  Rows are
+// grouped in bins of kPbRows; an entry whose column lies in its row's bin is
+// "local", every other entry is "far".  Columns are grouped in superblocks of
+// kPbCols.
+//   far    stored twice over: in scatter order -- by (column superblock,
+//          row bin, column) -- with the column offset in the superblock, the
+//          value and the slot of its product (sp: superblock boundaries), and
+//          in bin-grouped order -- by (row bin, column) -- with the row offset
+//          in the bin (bp: bin boundaries).  Within a bin the products of one
+//          superblock are contiguous, so the scatter pass writes runs.
+//   local  bin-grouped (lp, lrow, lcol = offsets in the bin, values), summed
+//          by the gather pass with x read from the bin's own slice
+constexpr int kPbShift = 13;
+constexpr int kPbRows = 1 << kPbShift;
+constexpr int kPbColShift = 13;
+constexpr int kPbCols = 1 << kPbColShift;
+
+struct PbMatrix {
+  int n = 0, nbins = 0, nsb = 0;
+  long n_far = 0, n_loc = 0;
+  DevBuf sp, fcol, fslot, frow, bp, lp, lrow, lcol;
+  DevBuf fval[3], lval[3];  // by Dtype F64, F32, BF16 (built on demand)
+  DevBuf prod;              // 8-byte product slots, n_far
+  std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry
+};
+
 struct DeviceCache {
   std::mutex mu;
   std::unique_ptr<DeviceMatrix> mirrors[4];
   std::unique_ptr<DeviceMatrix> transposed;
+  std::unique_ptr<PbMatrix> pb;
   DevBuf x_stage, y_stage;
 };
 
@@ -75,6 +102,10 @@ const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt);
 // A in increasing row order, and perm[k] is the CSR entry index (relative to
 // rowptr[0]) of transposed entry k, so values_T[k] = values[perm[k]].
 const DeviceMatrix &transpose_mirror(const cpp::Matrix &m);
+
+// The propagation-blocked layout of a square single-block CSR matrix, with
+// values of type vdt (PATTERN: indices only).
+const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt);
 
 // Host tile tables (exposed for tests and the autotuner).
 // tile_row of the CSR entry tiles (ntiles + 1 entries, ntiles = nnz / T + 1).

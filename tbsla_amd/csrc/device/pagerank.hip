@@ -358,6 +358,177 @@ __global__ __launch_bounds__(256) void scale_into(const double *__restrict__ s, 
 
 unsigned grid_for(long n) { return static_cast<unsigned>(std::max<long>(1, std::min<long>(kNormBlocks, (n + 255) / 256))); }
 
+// ---------------------------------------------------------------------------
+// Propagation-blocked product (layout in mirror.hpp, PbMatrix).
+//   pb_scatter  one workgroup per column superblock (8192 columns): the
+//               superblock's slice of x is staged in LDS, then every far
+//               product v * x[c] is written to its slot in the bin-grouped
+//               product array; entries come in (bin, column) order, so
+//               consecutive threads write runs of consecutive slots.
+//   pb_gather   one 1024-thread workgroup per bin of 8192 rows: the bin's
+//               far products and local products (x from the bin's own slice)
+//               are added into LDS, then the row epilogue runs on the sums.
+// The LDS additions are atomic, so the summation order within a row is not
+// fixed (the result can differ in the last bits from run to run).
+template <typename VT>
+struct PbVal {
+  template <typename AT>
+  __device__ static AT prod(const VT *v, long k, AT x) {
+    return static_cast<AT>(dev::Val<VT>::get(v, k)) * x;
+  }
+};
+template <>
+struct PbVal<dev::Pattern> {
+  template <typename AT>
+  __device__ static AT prod(const dev::Pattern *, long, AT x) {
+    return x;
+  }
+};
+
+constexpr int kPbScatterThreads = 1024;
+
+template <typename VT, typename XT, typename AT>
+__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__restrict__ sp, const unsigned short *__restrict__ fcol,
+                                                  const int *__restrict__ slot, const VT *__restrict__ val,
+                                                  const XT *__restrict__ x, AT *__restrict__ prod, int nsb, int n,
+                                                  const PrState *st, int anchor) {
+#pragma clang fp contract(off)
+  if (st->done || (anchor && !st->need_anchor)) return;
+  // superblocks of one XCD (blockIdx % 8) are contiguous, so the runs that
+  // share a cache line at a run boundary are written through the same L2
+  const int per = (gridDim.x + 7) / 8;
+  const int sb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (sb >= nsb) return;
+  const int k1 = sp[sb + 1];
+  // the superblock's slice of x, staged in LDS (64 KB in FP64): read once
+  __shared__ XT xs[kPbCols];
+  const long c0 = static_cast<long>(sb) << kPbColShift;
+  const int nc = static_cast<int>(min(static_cast<long>(kPbCols), static_cast<long>(n) - c0));
+  for (int i = threadIdx.x; i < nc; i += kPbScatterThreads) xs[i] = x[c0 + i];
+  __syncthreads();
+  // four entries per thread in flight: loads first, then the products and stores
+  for (int k = sp[sb] + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
+    int sl[4];
+    AT p[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int kk = k + u * kPbScatterThreads;
+      if (kk < k1) {
+        sl[u] = slot[kk];
+        p[u] = PbVal<VT>::template prod<AT>(val, kk, static_cast<AT>(xs[fcol[kk]]));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (k + u * kPbScatterThreads < k1) prod[sl[u]] = p[u];
+  }
+}
+
+struct PbView {
+  const int *bp, *lp;
+  const unsigned short *frow, *lrow, *lcol;
+  int n;
+};
+
+// MODE 0: PageRank update epilogue (b = beta*(yhat + s) + t, block max);
+// MODE 1: the anchor product, yhat = s.
+template <typename VT, typename XT, typename AT, bool MIXED, int TM, int MODE>
+__global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict__ lval, const XT *__restrict__ x,
+                                                  const AT *__restrict__ prod, const double *__restrict__ yhat,
+                                                  Teleport tp, double *__restrict__ out, PrState *st,
+                                                  unsigned long long *slots) {
+#pragma clang fp contract(off)
+  if (MODE == 0 ? st->done : (st->done || !st->need_anchor)) return;
+  __shared__ double ybin[kPbRows];  // FP64 sums (products may be FP32)
+  __shared__ double red[16];
+  const int bin = blockIdx.x, tid = threadIdx.x;
+  const int base = bin << kPbShift;
+  const int rows = min(kPbRows, v.n - base);
+  for (int i = tid; i < kPbRows; i += 1024) ybin[i] = 0.0;
+  __syncthreads();
+  const int f1 = v.bp[bin + 1], l1 = v.lp[bin + 1];
+#pragma unroll 8
+  for (int k = v.bp[bin] + tid; k < f1; k += 1024) atomicAdd(&ybin[v.frow[k]], static_cast<double>(prod[k]));
+  // local entries are in row order: each thread sums runs of one row within
+  // its four consecutive entries and adds each run once
+  for (int k = v.lp[bin] + 4 * tid; k < l1; k += 4 * 1024) {
+    double acc = 0.0;
+    int row = v.lrow[k];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int kk = k + u;
+      if (kk < l1) {
+        const int r = v.lrow[kk];
+        if (r != row) {
+          atomicAdd(&ybin[row], acc);
+          acc = 0.0;
+          row = r;
+        }
+        const AT xv = static_cast<AT>(x[base + v.lcol[kk]]);
+        acc += static_cast<double>(PbVal<VT>::template prod<AT>(lval, kk, xv));
+      }
+    }
+    atomicAdd(&ybin[row], acc);
+  }
+  __syncthreads();
+  double lmax = -INFINITY;
+  const double beta = st->beta, tele = st->tele;
+  for (int i = tid; i < rows; i += 1024) {
+    const int r = base + i;
+    if (MODE == 1) {
+      out[r] = static_cast<double>(ybin[i]);
+    } else {
+      const double sv = MIXED ? yhat[r] + static_cast<double>(ybin[i]) : static_cast<double>(ybin[i]);
+      const double bi = add_teleport<TM>(beta * sv, tele, tp, r);
+      out[r] = bi;
+      lmax = fmax(lmax, bi);
+    }
+  }
+  if (MODE == 1) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = lmax;
+  __syncthreads();
+  if (tid == 0) {
+    double m = red[0];
+    for (int w = 1; w < 16; w++) m = fmax(m, red[w]);
+    atomicMax(slots + (bin % kMaxSlots), ord_encode(m));
+  }
+}
+
+PbView pb_view(const PbMatrix &pb) {
+  return PbView{pb.bp.as<int>(), pb.lp.as<int>(), pb.frow.as<unsigned short>(), pb.lrow.as<unsigned short>(),
+                pb.lcol.as<unsigned short>(), pb.n};
+}
+
+// One propagation-blocked product (scatter then gather) with values of type
+// VT from `pb` (dev::Pattern: value-free), x of type XT, sums of type AT.
+template <typename VT, typename XT, typename AT, bool MIXED, int TM, int MODE>
+void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const double *yhat, const Teleport &tp, double *out,
+               PrState *st, unsigned long long *slots, hipStream_t s) {
+  const VT *fval = vslot >= 0 ? pb.fval[vslot].as<VT>() : nullptr;
+  const VT *lval = vslot >= 0 ? pb.lval[vslot].as<VT>() : nullptr;
+  AT *prod = pb.prod.as<AT>();
+  if (pb.n_far > 0)
+    hipLaunchKernelGGL((pb_scatter<VT, XT, AT>), dim3((pb.nsb + 7) / 8 * 8), dim3(kPbScatterThreads), 0, s,
+                       pb.sp.as<int>(),
+                       pb.fcol.as<unsigned short>(), pb.fslot.as<int>(), fval, x, prod, pb.nsb, pb.n, st,
+                       MODE == 1 ? 1 : 0);
+  hipLaunchKernelGGL((pb_gather<VT, XT, AT, MIXED, TM, MODE>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval,
+                     x, prod, yhat, tp, out, st, slots);
+}
+
+template <typename VT, typename XT, typename AT, bool MIXED>
+void launch_pb_update(int tmode, const PbMatrix &pb, int vslot, const XT *x, const double *yhat, const Teleport &tp,
+                      double *out, PrState *st, unsigned long long *slots, hipStream_t s) {
+  if (tmode == 1)
+    launch_pb<VT, XT, AT, MIXED, 1, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
+  else if (tmode == 2)
+    launch_pb<VT, XT, AT, MIXED, 2, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
+  else
+    launch_pb<VT, XT, AT, MIXED, 0, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
+}
+
 template <typename VT, typename XT, bool MIXED, int TM>
 void launch_update(const DeviceMatrix &d, const VT *vals, const XT *x, const double *yhat, const Teleport &tp,
                    double *b, PrState *st, unsigned long long *slots, hipStream_t s) {
@@ -428,6 +599,18 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   if (p.precision == PrPrecision::F32) dlo = &mirror(m, Dtype::F32);
   if (p.precision == PrPrecision::BF16) dlo = &mirror(m, Dtype::BF16);
   const bool fused = csr && !d64.noop && d64.rows == n;
+  // propagation-blocked products: explicit choice, or by default for the
+  // sum-normalised rule on large graphs (the reference rule keeps the
+  // fixed-order tile products)
+  const bool use_pb = fused && !p.values &&
+                      (p.gather == 2 || (p.gather == 0 && p.rule == PrRule::SUMNORM && n >= (1L << 16)));
+  const PbMatrix *pbm = nullptr;
+  if (use_pb) {
+    const bool mixed_pat = mixed && m.count_stochastic();
+    pbm = &pb_mirror(m, (p.precision == PrPrecision::PATTERN || mixed_pat) ? Dtype::PATTERN : Dtype::F64);
+    if (p.precision == PrPrecision::F32) pbm = &pb_mirror(m, Dtype::F32);
+    if (p.precision == PrPrecision::BF16) pbm = &pb_mirror(m, Dtype::BF16);
+  }
   if (csr && !d64.noop && d64.rows != n)
     throw std::invalid_argument("tbsla: CSR rowptr does not cover every row");
   if (d64.x_need > n) throw std::invalid_argument("tbsla: column index beyond n_col");
@@ -537,26 +720,43 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
           double *xhat = w.xhat.as<double>(), *yhat = w.yhat.as<double>();
           hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, xhat, dcur, mixed_pattern ? Z : nullptr,
                              colscale, n, st);
-          if (mixed_pattern)
-            launch_anchor<dev::Pattern>(d64, Z, yhat, st, s);
-          else
-            launch_anchor<double>(d64, xhat, yhat, st, s);
-          if (p.precision == PrPrecision::F32)
-            launch_update<float, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
-          else
-            launch_update<dev::bf16, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
+          if (pbm) {
+            if (mixed_pattern)
+              launch_pb<dev::Pattern, double, double, false, 0, 1>(*pbm, -1, Z, nullptr, tp, yhat, st, slots, s);
+            else
+              launch_pb<double, double, double, false, 0, 1>(*pbm, 0, xhat, nullptr, tp, yhat, st, slots, s);
+            if (p.precision == PrPrecision::F32)
+              launch_pb_update<float, float, float, true>(tmode, *pbm, 1, dcur, yhat, tp, xout, st, slots, s);
+            else
+              launch_pb_update<dev::bf16, float, float, true>(tmode, *pbm, 2, dcur, yhat, tp, xout, st, slots, s);
+          } else {
+            if (mixed_pattern)
+              launch_anchor<dev::Pattern>(d64, Z, yhat, st, s);
+            else
+              launch_anchor<double>(d64, xhat, yhat, st, s);
+            if (p.precision == PrPrecision::F32)
+              launch_update<float, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
+            else
+              launch_update<dev::bf16, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
+          }
           fin_max();
           hipLaunchKernelGGL(pr_normalize<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,
                              nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
           fin_norm();
         } else if (pattern) {
-          launch_update<dev::Pattern, double, false>(tmode, d64, nullptr, Z, nullptr, tp, xout, st, slots, s);
+          if (pbm)
+            launch_pb_update<dev::Pattern, double, double, false>(tmode, *pbm, -1, Z, nullptr, tp, xout, st, slots, s);
+          else
+            launch_update<dev::Pattern, double, false>(tmode, d64, nullptr, Z, nullptr, tp, xout, st, slots, s);
           fin_max();
           hipLaunchKernelGGL(pr_normalize<2>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
                              nullptr, Z, colscale, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
           fin_norm();
         } else {
-          launch_update<double, double, false>(tmode, d64, p.values, xin, nullptr, tp, xout, st, slots, s);
+          if (pbm)
+            launch_pb_update<double, double, double, false>(tmode, *pbm, 0, xin, nullptr, tp, xout, st, slots, s);
+          else
+            launch_update<double, double, false>(tmode, d64, p.values, xin, nullptr, tp, xout, st, slots, s);
           fin_max();
           hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
                              nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),

@@ -199,18 +199,22 @@ class Matrix:
 
     def page_rank_device(self, out, epsilon=1e-10, beta=0.85, max_iteration=100, precision="f64",
                          rule="reference", check_every=8, anchor_tau=0.0, nodes=None, stream=None,
-                         values=None, teleport=None):
+                         values=None, teleport=None, gather="auto"):
         """PageRank into the float64 tensor ``out``; returns the run report.
 
         ``values`` (float64 device tensor, one per stored CSR entry) replaces
         the stored edge values; ``teleport`` (float64 device tensor, n_col)
-        is the teleport distribution v of b = beta*A b + (1-beta)(1^T b) v."""
+        is the teleport distribution v of b = beta*A b + (1-beta)(1^T b) v.
+        ``gather``: "auto", "tiles" (fixed-order entry tiles) or "blocked"
+        (propagation-blocked products; auto picks them for the sumnorm rule
+        on n >= 2^16)."""
         p = _lib.PageRankParams()
         p.epsilon, p.beta, p.max_iteration = epsilon, beta, max_iteration
         p.precision = dtype_code(precision)
         p.rule = {"reference": 0, "sumnorm": 1}[rule] if isinstance(rule, str) else int(rule)
         p.check_every = check_every
         p.anchor_tau = anchor_tau
+        p.gather = {"auto": 0, "tiles": 1, "blocked": 2}[gather] if isinstance(gather, str) else int(gather)
         keep = None
         if nodes is not None:
             keep = np.ascontiguousarray(nodes, dtype=np.int32)

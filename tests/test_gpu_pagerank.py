@@ -51,16 +51,18 @@ def test_page_rank_2_20_reference_rule(graph_2_20, digests, gpu):
     assert abs(pr.sum() - 1) < 1e-14
 
 
+@pytest.mark.parametrize("gather", ["tiles", "blocked"])
 @pytest.mark.parametrize("precision", ["f64", "pattern", "f32", "bf16"])
-def test_page_rank_precisions_reach_1e10(graph_2_20, gpu, precision):
-    """Every precision variant stops on the sum-normalised bound and lands
+def test_page_rank_precisions_reach_1e10(graph_2_20, gpu, precision, gather):
+    """Every precision variant, with fixed-order tile products or with
+    propagation-blocked products, stops on the sum-normalised bound and lands
     within 1e-10 L1 of the FP64 fixed point (BF16/FP32 values need the FP64
     anchor product to get there)."""
     import torch
     p, _, _, pstar = graph_2_20
     out = torch.zeros(p.get_n_col(), dtype=torch.float64, device=gpu)
     rep = p.page_rank_device(out, epsilon=1e-10, beta=0.85, max_iteration=500, precision=precision,
-                             rule="sumnorm")
+                             rule="sumnorm", gather=gather)
     torch.cuda.synchronize()
     err = np.abs(out.cpu().numpy() - pstar).sum()
     assert err <= 1e-10, (precision, err, rep)
@@ -74,8 +76,9 @@ def test_pattern_path_is_bit_identical(graph_2_20, gpu):
     p = graph_2_20[0]
     a = torch.zeros(p.get_n_col(), dtype=torch.float64, device=gpu)
     b = torch.zeros_like(a)
-    ra = p.page_rank_device(a, epsilon=1e-10, max_iteration=60, precision="f64", rule="reference")
-    rb = p.page_rank_device(b, epsilon=1e-10, max_iteration=60, precision="pattern", rule="reference")
+    ra = p.page_rank_device(a, epsilon=1e-10, max_iteration=60, precision="f64", rule="reference", gather="tiles")
+    rb = p.page_rank_device(b, epsilon=1e-10, max_iteration=60, precision="pattern", rule="reference",
+                            gather="tiles")
     torch.cuda.synchronize()
     assert ra["iterations"] == rb["iterations"]
     assert torch.equal(a, b)
@@ -93,6 +96,32 @@ def test_personalized_device_matches_oracle(tb, oracle, gpu):
     torch.cuda.synchronize()
     assert abs(rep["iterations"] - its) <= 1
     assert np.abs(out.cpu().numpy() - ref).sum() <= 1e-12
+    # propagation-blocked products, reference stopping rule: same run up to rounding
+    rep = p.page_rank_device(out, epsilon=1e-12, max_iteration=1000, nodes=nodes, gather="blocked")
+    assert abs(rep["iterations"] - its) <= 1
+    assert np.abs(out.cpu().numpy() - ref).sum() <= 1e-12
+
+
+def test_blocked_products_with_teleport_vector_and_bins(tb, oracle, gpu):
+    """Propagation-blocked products on a graph spanning several bins (8192
+    rows each, last one partial) and column superblocks (8192 columns) with
+    an explicit teleport distribution: equal within rounding to the
+    fixed-order tile products."""
+    import torch
+    n = 5 * 8192 + 1234
+    p = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 12, 0.4, 77)
+    v = oracle.uniform_vector(5, n) + 0.05
+    v /= v.sum()
+    vd = torch.from_numpy(v).to(gpu)
+    a = torch.zeros(n, dtype=torch.float64, device=gpu)
+    b = torch.zeros_like(a)
+    for prec in ("f64", "pattern"):
+        ra = p.page_rank_device(a, epsilon=1e-13, max_iteration=400, rule="sumnorm", teleport=vd,
+                                precision=prec, gather="tiles")
+        rb = p.page_rank_device(b, epsilon=1e-13, max_iteration=400, rule="sumnorm", teleport=vd,
+                                precision=prec, gather="blocked")
+        assert abs(ra["iterations"] - rb["iterations"]) <= 1
+        assert float((a - b).abs().sum()) <= 1e-13, prec
 
 
 def test_page_rank_zero_iterations(tb, gpu):
