@@ -404,23 +404,34 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__res
   __shared__ XT xs[kPbCols];
   const long c0 = static_cast<long>(sb) << kPbColShift;
   const int nc = static_cast<int>(min(static_cast<long>(kPbCols), static_cast<long>(n) - c0));
-  for (int i = threadIdx.x; i < nc; i += kPbScatterThreads) xs[i] = x[c0 + i];
+  {
+    constexpr int S = kPbCols / kPbScatterThreads;  // x values per thread, all loads issued first
+    XT t[S];
+#pragma unroll
+    for (int u = 0; u < S; u++) {
+      const int i = threadIdx.x + u * kPbScatterThreads;
+      t[u] = i < nc ? x[c0 + i] : XT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < S; u++) xs[threadIdx.x + u * kPbScatterThreads] = t[u];
+  }
   __syncthreads();
-  // four entries per thread in flight: loads first, then the products and stores
+  // four entries per thread in flight: every global load first, then the LDS
+  // reads of x, then the products and the stores
   for (int k = sp[sb] + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
-    int sl[4];
-    AT p[4];
+    int sl[4], fc[4];
+    AT v1[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int kk = k + u * kPbScatterThreads;
-      if (kk < k1) {
-        sl[u] = slot[kk];
-        p[u] = PbVal<VT>::template prod<AT>(val, kk, static_cast<AT>(xs[fcol[kk]]));
-      }
+      const bool ok = kk < k1;
+      fc[u] = ok ? fcol[kk] : 0;
+      sl[u] = ok ? slot[kk] : -1;
+      v1[u] = ok ? PbVal<VT>::template prod<AT>(val, kk, AT(1)) : AT(0);
     }
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (k + u * kPbScatterThreads < k1) prod[sl[u]] = p[u];
+      if (sl[u] >= 0) prod[sl[u]] = v1[u] * static_cast<AT>(xs[fc[u]]);
   }
 }
 
@@ -447,28 +458,51 @@ __global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict
   for (int i = tid; i < kPbRows; i += 1024) ybin[i] = 0.0;
   __syncthreads();
   const int f1 = v.bp[bin + 1], l1 = v.lp[bin + 1];
-#pragma unroll 8
-  for (int k = v.bp[bin] + tid; k < f1; k += 1024) atomicAdd(&ybin[v.frow[k]], static_cast<double>(prod[k]));
-  // local entries are in row order: each thread sums runs of one row within
-  // its four consecutive entries and adds each run once
+  // far products: eight loads per thread issued before their LDS additions
+  // (the compiler does not move global loads across LDS atomics)
+  constexpr int U = 8;
+  for (int k = v.bp[bin] + tid; k < f1; k += U * 1024) {
+    int r[U];
+    double pv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kk = k + u * 1024;
+      r[u] = kk < f1 ? v.frow[kk] : -1;
+      pv[u] = kk < f1 ? static_cast<double>(prod[kk]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (r[u] >= 0) atomicAdd(&ybin[r[u]], pv[u]);
+  }
+  // local entries are in row order: each thread takes four consecutive
+  // entries (all loads first), sums runs of one row and adds each run once
   for (int k = v.lp[bin] + 4 * tid; k < l1; k += 4 * 1024) {
-    double acc = 0.0;
-    int row = v.lrow[k];
+    int r[4], c[4];
+    AT lv[4], xv[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int kk = k + u;
-      if (kk < l1) {
-        const int r = v.lrow[kk];
-        if (r != row) {
-          atomicAdd(&ybin[row], acc);
-          acc = 0.0;
-          row = r;
-        }
-        const AT xv = static_cast<AT>(x[base + v.lcol[kk]]);
-        acc += static_cast<double>(PbVal<VT>::template prod<AT>(lval, kk, xv));
-      }
+      const bool ok = k + u < l1;
+      r[u] = ok ? v.lrow[k + u] : -1;
+      c[u] = ok ? v.lcol[k + u] : 0;
     }
-    atomicAdd(&ybin[row], acc);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      xv[u] = static_cast<AT>(x[base + c[u]]);
+      lv[u] = r[u] >= 0 ? PbVal<VT>::template prod<AT>(lval, k + u, AT(1)) : AT(0);
+    }
+    double acc = 0.0;
+    int row = r[0];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (r[u] < 0) break;
+      if (r[u] != row) {
+        atomicAdd(&ybin[row], acc);
+        acc = 0.0;
+        row = r[u];
+      }
+      acc += static_cast<double>(lv[u] * xv[u]);
+    }
+    if (row >= 0) atomicAdd(&ybin[row], acc);
   }
   __syncthreads();
   double lmax = -INFINITY;
