@@ -114,6 +114,12 @@ def leg_formats(torch, tb, dev, steps):
         ms2 = time_kernel(torch, lambda: m.Ax_device(xx, yy), max(10, steps // 4), 3)
         fm[fmt] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
                    "bytes": mi2["algorithmic_bytes"]}
+        if fmt in ("SCOO", "COO"):
+            # the kernel runs on the compressed-row mirror: 12 B per entry, 8 B per row
+            # segment, x once, y read and written
+            mb = mi2["entries"] * 12 + mi2["rows"] * 8 + 8 * mi2["x_need"] + 16 * (mi2["y_hi"] - mi2["y_lo"])
+            fm[fmt]["mirror_bytes"] = mb
+            fm[fmt]["mirror_gbs"] = round(mb / ms2 / 1e6, 1)
         log(f"{fmt} {ms2 * 1e3:.1f} us {fm[fmt]['gbs']} GB/s")
         del m, xx, yy
         torch.cuda.empty_cache()

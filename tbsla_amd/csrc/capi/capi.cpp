@@ -471,10 +471,10 @@ int tbsla_hip_mirror_info(tbsla_matrix *m, int value_dtype, long *out) {
     const hip::DeviceMatrix &d = hip::mirror(*m->m, dtype_of(value_dtype));
     const std::size_t xe = d.vdt == hip::Dtype::F64 ? 8 : 4;
     std::size_t bytes = d.rowptr.bytes() + d.colidx.bytes() + d.vals.bytes() + d.tiles.bytes() + d.ecols.bytes() +
-                        d.evals.bytes() + d.srow.bytes() + d.scol.bytes() + d.sval.bytes() + d.stiles.bytes() +
+                        d.evals.bytes() + d.segrow.bytes() + d.long_rows.bytes() +
                         d.dvals.bytes();
     const long v[8] = {d.rows, d.entries, d.x_need, d.y_lo, d.y_hi,
-                       d.format == cpp::FORMAT_CSR ? d.ntiles : d.nstiles, static_cast<long>(bytes),
+                       d.ntiles, static_cast<long>(bytes),
                        d.noop ? 0L : static_cast<long>(d.algorithmic_bytes(xe, xe))};
     std::memcpy(out, v, sizeof(v));
   });

@@ -193,15 +193,33 @@ __global__ __launch_bounds__(256) void adj_grad_teleport(const double *__restric
     gt[i] = f * (lam[i] - a);
 }
 
-// One thread per row of A: grad[e] = beta * (lambda_i - a)/s * p[col(e)].
+// grad[e] = beta * (lambda_i - a)/s * p[col(e)] for the entries of 256 rows
+// per workgroup: the rows' pointers are staged in LDS, each thread takes
+// consecutive entries (coalesced writes) and finds its row by binary search.
 __global__ __launch_bounds__(256) void adj_grad_values(const int *__restrict__ rowptr, const int *__restrict__ colidx,
                                                        int rows, int rp0, int f_col, const double *__restrict__ lam,
                                                        const double *__restrict__ p, double *__restrict__ gv,
                                                        const AdjState *st) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= rows) return;
-  const double gi = st->beta * (lam[i] - st->a) / st->s;
-  for (int e = rowptr[i] - rp0; e < rowptr[i + 1] - rp0; e++) gv[e] = gi * p[colidx[e] - f_col];
+  __shared__ int rpl[257];
+  __shared__ double gl[256];
+  const int r0 = blockIdx.x * 256;
+  const int nr = min(256, rows - r0);
+  const double a = st->a, f = st->beta / st->s;
+  for (int i = threadIdx.x; i <= nr; i += 256) rpl[i] = rowptr[r0 + i] - rp0;
+  if (static_cast<int>(threadIdx.x) < nr) gl[threadIdx.x] = f * (lam[r0 + threadIdx.x] - a);
+  __syncthreads();
+  const int e0 = rpl[0], e1 = rpl[nr];
+  for (int e = e0 + threadIdx.x; e < e1; e += 256) {
+    int lo = 0, hi = nr;  // the row i with rpl[i] <= e < rpl[i+1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (rpl[mid] <= e)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    gv[e] = gl[lo] * p[colidx[e] - f_col];
+  }
 }
 
 unsigned grid_for(long n) {

@@ -197,13 +197,29 @@ void build_coo_like(DeviceMatrix &d, const M &m, Dtype vdt, bool scoo) {
     pcol = scol.data();
     pval = sval.data();
   }
-  d.srow.upload(prow, n * sizeof(int));
-  d.scol.upload(pcol, n * sizeof(int));
-  upload_values(d.sval, pval, n, vdt);
-  std::vector<int> t32 = segment_tiles(prow, static_cast<long>(n), kCsrTileNnz);
-  std::vector<long> t(t32.begin(), t32.end());
-  d.nstiles = static_cast<int>(t.size()) - 1;
-  d.stiles.upload(t.data(), t.size() * sizeof(long));
+  if (n >= (1UL << 31)) bad("COO mirrors are limited to 2^31 entries");
+  // segments: maximal runs of one row in the sorted view
+  std::vector<int> segptr{0}, segrow;
+  for (std::size_t k = 0; k < n; k++) {
+    if (k > 0 && prow[k] != prow[k - 1]) segptr.push_back(static_cast<int>(k));
+    if (k == 0 || prow[k] != prow[k - 1]) segrow.push_back(prow[k]);
+  }
+  segptr.push_back(static_cast<int>(n));
+  d.nseg = static_cast<int>(segrow.size());
+  d.rows = d.nseg;
+  d.rp0 = 0;
+  d.rowptr.upload(segptr.data(), segptr.size() * sizeof(int));
+  d.segrow.upload(segrow.data(), segrow.size() * sizeof(int));
+  d.colidx.upload(pcol, n * sizeof(int));
+  upload_values(d.vals, pval, n, vdt);
+  std::vector<int> t = csr_entry_tiles(segptr.data(), d.nseg, kCsrTileNnz);
+  d.ntiles = static_cast<int>(t.size()) - 1;
+  d.tiles.upload(t.data(), t.size() * sizeof(int));
+  std::vector<int> lr;
+  for (int sidx = 0; sidx < d.nseg; sidx++)
+    if (segptr[static_cast<std::size_t>(sidx) + 1] - segptr[static_cast<std::size_t>(sidx)] > kCsrLongRow) lr.push_back(sidx);
+  d.nlong = static_cast<int>(lr.size());
+  if (d.nlong) d.long_rows.upload(lr.data(), lr.size() * sizeof(int));
 }
 
 void build_dense(DeviceMatrix &d, const cpp::MatrixDENSE &m, Dtype vdt) {
@@ -240,26 +256,6 @@ std::vector<int> csr_entry_tiles(const int *rowptr, int rows, int tile_nnz) {
   return t;
 }
 
-std::vector<int> segment_tiles(const int *row, long n, int max_nnz) {
-  std::vector<int> t;
-  t.push_back(0);
-  long s = 0;
-  while (s < n) {
-    long e = std::min(n, s + max_nnz);
-    if (e < n) {
-      long b = e;
-      while (b > s && row[b] == row[b - 1]) b--;  // back up to the start of the row cut at e
-      if (b > s) {
-        e = b;
-      } else {
-        while (e < n && row[e] == row[e - 1]) e++;  // one row longer than a tile
-      }
-    }
-    t.push_back(static_cast<int>(e));
-    s = e;
-  }
-  return t;
-}
 
 std::size_t DeviceMatrix::algorithmic_bytes(std::size_t xe, std::size_t ye) const {
   const std::size_t ve = dtype_size(vdt);

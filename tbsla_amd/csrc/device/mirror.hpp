@@ -7,8 +7,10 @@
 //   ELL   column-major (slot j of row i at j*ld + i), ld = rows rounded up to 64.
 //   COO / SCOO
 //         entries in a row-sorted view (a stable counting sort by row when the
-//         input is not already sorted) plus a tile table whose boundaries fall
-//         on row changes, so every row is summed by one thread in entry order.
+//         input is not already sorted), compressed to row segments: segment s
+//         = entries [rowptr[s], rowptr[s+1]) of row segrow[s].  The CSR tile
+//         kernels run on it, each segment's sum starting from y[row], so every
+//         row is accumulated in entry order.
 //   DENSE row-major, as stored.
 //
 // Every index the kernels will read is bounded here on the host (x_need,
@@ -47,9 +49,10 @@ struct DeviceMatrix {
   int K = 0;
   long ld = 0;
   DevBuf ecols, evals;
-  // COO / SCOO sorted view
-  DevBuf srow, scol, sval, stiles;
-  int nstiles = 0;
+  // COO / SCOO: segments of the row-sorted view (rowptr, colidx, vals, tiles
+  // and long_rows above hold the segment CSR)
+  DevBuf segrow;
+  int nseg = 0;
   // DENSE
   int dcols = 0;
   DevBuf dvals;
@@ -110,7 +113,6 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt);
 // Host tile tables (exposed for tests and the autotuner).
 // tile_row of the CSR entry tiles (ntiles + 1 entries, ntiles = nnz / T + 1).
 std::vector<int> csr_entry_tiles(const int *rowptr, int rows, int tile_nnz);
-std::vector<int> segment_tiles(const int *row_sorted, long n, int max_nnz);
 
 }  // namespace hip
 }  // namespace tbsla

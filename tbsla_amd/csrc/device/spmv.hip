@@ -12,9 +12,9 @@
 //                   loads of four rows' indices and values per slot, two slots
 //                   per step; same order as MatrixELL.cpp:105-113 (contraction
 //                   off, so bit-identical in FP64).
-// seg_kernel        COO/SCOO: one workgroup per segment-aligned tile of the
-//                   row-sorted view; segment heads found with a block scan,
-//                   each row accumulated into y in entry order
+// coo_tile_kernel   COO/SCOO: the CSR entry-tile kernel on the row segments of
+//                   the row-sorted view, each segment's sum starting from
+//                   y[row], so each row is accumulated in entry order
 //                   (MatrixCOO.cpp:113-115 / MatrixSCOO.cpp:120-122 with one
 //                   thread).
 // dense_kernel      one wavefront per row, butterfly reduction
@@ -134,77 +134,33 @@ __global__ __launch_bounds__(256) void ell_kernel(const int *__restrict__ cols, 
     if (i0 + r < rows) y[i0 + r] = static_cast<YT>(acc[r]);
 }
 
-template <typename VT, typename XT, typename YT>
-__global__ __launch_bounds__(kTileThreads) void seg_kernel(const int *__restrict__ row, const int *__restrict__ col,
-                                                           const VT *__restrict__ vals, const XT *__restrict__ x,
-                                                           YT *__restrict__ y, const long *__restrict__ tiles,
-                                                           int x_off, int y_off) {
-  using AT = typename Val<VT>::A;
-  __shared__ AT prod[kCsrTileNnz];
-  __shared__ int key[kCsrTileNnz];
-  __shared__ int seg[kCsrTileNnz + 1];
-  __shared__ int wsum[kTileThreads / 64];
-  const int tid = threadIdx.x;
-  const long e0 = tiles[blockIdx.x];
-  const int ne = static_cast<int>(tiles[blockIdx.x + 1] - e0);
-  if (ne <= kCsrTileNnz) {
-    int c[kTilePer];
-    AT v[kTilePer];
-#pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < ne) {
-        key[e] = nt_load(row, e0 + e);
-        c[k] = nt_load(col, e0 + e) + x_off;
-        v[k] = Val<VT>::get(vals, e0 + e);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < ne) prod[e] = v[k] * static_cast<AT>(x[c[k]]);
-    }
-    __syncthreads();
-    int cnt = 0;
-#pragma unroll
-    for (int j = 0; j < kTilePer; j++) {
-      const int e = tid * kTilePer + j;
-      if (e < ne && (e == 0 || key[e] != key[e - 1])) cnt++;
-    }
-    int nseg = 0;
-    int o = block_exclusive_scan(cnt, nseg, wsum);
-#pragma unroll
-    for (int j = 0; j < kTilePer; j++) {
-      const int e = tid * kTilePer + j;
-      if (e < ne && (e == 0 || key[e] != key[e - 1])) seg[o++] = e;
-    }
-    if (tid == 0) seg[nseg] = ne;
-    __syncthreads();
-    for (int s = tid; s < nseg; s += kTileThreads) {
-      const int a = seg[s], b = seg[s + 1];
-      const long r = static_cast<long>(key[a]) + y_off;
-      AT acc = static_cast<AT>(y[r]);
-      for (int e = a; e < b; e++) acc += prod[e];
-      y[r] = static_cast<YT>(acc);
-    }
-    return;
+// COO/SCOO segment epilogue: the segment's sum started from y[row], so the
+// row is accumulated y[row] += v * x in entry order (MatrixCOO.cpp:113-115).
+template <typename YT>
+struct AccRow {
+  YT *y;
+  const int *segrow;
+  long y_off;
+  template <typename AT>
+  __device__ AT init(int s) const {
+    return static_cast<AT>(y[segrow[s] + y_off]);
   }
-  // One row longer than a tile: chunked tree sums, then y[r] += total.
-  AT total = 0;
-  for (int base = 0; base < ne; base += kCsrTileNnz) {
-    const int cnt = min(kCsrTileNnz, ne - base);
-    AT acc = 0;
-#pragma unroll
-    for (int k = 0; k < kTilePer; k++) {
-      const int e = k * kTileThreads + tid;
-      if (e < cnt) acc += Val<VT>::get(vals, e0 + base + e) * static_cast<AT>(x[nt_load(col, e0 + base + e) + x_off]);
-    }
-    total += block_tree_sum<AT>(acc, prod);
+  template <typename AT>
+  __device__ void operator()(int s, AT v) const {
+    y[segrow[s] + y_off] = static_cast<YT>(v);
   }
-  if (tid == 0) {
-    const long r = static_cast<long>(row[e0]) + y_off;
-    y[r] = static_cast<YT>(static_cast<AT>(y[r]) + total);
-  }
+};
+
+template <typename VT, typename XT, typename YT, bool LONG>
+__global__ __launch_bounds__(kTileThreads) void coo_tile_kernel(dev::CsrView<VT> m, const XT *__restrict__ x,
+                                                                YT *__restrict__ y, const int *__restrict__ segrow,
+                                                                long y_off) {
+  __shared__ dev::CsrTileLds<typename Val<VT>::A> lds;
+  AccRow<YT> epi{y, segrow, y_off};
+  if (LONG)
+    dev::csr_long_row<VT, XT>(m, x, lds, epi);
+  else
+    dev::csr_tile_rows<VT, XT>(m, x, lds, epi);
 }
 
 template <typename XT, typename YT>
@@ -263,11 +219,16 @@ void run_typed(const DeviceMatrix &d, const XT *x, YT *y, int vect_incr, hipStre
       break;
     case cpp::FORMAT_COO:
     case cpp::FORMAT_SCOO: {
-      const int x_off = d.format == cpp::FORMAT_SCOO ? -d.f_col : 0;
-      const int y_off = vect_incr - (d.format == cpp::FORMAT_SCOO ? d.f_row : 0);
-      if (d.nstiles > 0)
-        hipLaunchKernelGGL((seg_kernel<VT, XT, YT>), dim3(d.nstiles), dim3(kTileThreads), 0, s, d.srow.as<int>(),
-                           d.scol.as<int>(), d.sval.as<VT>(), x, y, d.stiles.as<long>(), x_off, y_off);
+      // columns: global (COO) or minus f_col (SCOO, the view's f_col);
+      // rows: + vect_incr (- f_row for SCOO)
+      const dev::CsrView<VT> m = dev::csr_view<VT>(d);
+      const long y_off = vect_incr - (d.format == cpp::FORMAT_SCOO ? d.f_row : 0);
+      if (d.ntiles > 0)
+        hipLaunchKernelGGL((coo_tile_kernel<VT, XT, YT, false>), dim3(d.ntiles), dim3(kTileThreads), 0, s, m, x, y,
+                           d.segrow.as<int>(), y_off);
+      if (d.nlong > 0)
+        hipLaunchKernelGGL((coo_tile_kernel<VT, XT, YT, true>), dim3(d.nlong), dim3(kTileThreads), 0, s, m, x, y,
+                           d.segrow.as<int>(), y_off);
       break;
     }
     case cpp::FORMAT_DENSE:
@@ -313,7 +274,7 @@ const char *spmv_kernel_name(cpp::Format f) {
     case cpp::FORMAT_CSR: return "csr_tile_kernel";
     case cpp::FORMAT_ELL: return "ell_kernel";
     case cpp::FORMAT_COO:
-    case cpp::FORMAT_SCOO: return "seg_kernel";
+    case cpp::FORMAT_SCOO: return "coo_tile_kernel";
     case cpp::FORMAT_DENSE: return "dense_kernel";
   }
   return "";

@@ -161,6 +161,17 @@ struct Prod<Pattern, XT> {
   }
 };
 
+// A row sum starts from epi.init<AT>(row) when the epilogue has one (an
+// accumulating product, y[row] += ...), else from 0.
+template <typename AT, typename E>
+__device__ inline auto epi_init(const E &e, int r, int) -> decltype(e.template init<AT>(r)) {
+  return e.template init<AT>(r);
+}
+template <typename AT, typename E>
+__device__ inline AT epi_init(const E &, int, long) {
+  return AT(0);
+}
+
 // Rows of entry tile `tile`; epi(row, sum) once per owned row by the thread
 // that sums it.  The caller synchronises the block before reusing `lds`.
 template <typename VT, typename XT, typename Epi>
@@ -219,7 +230,7 @@ __device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restri
       b = m.rowptr[r + 1] - m.rp0;
     }
     if (b - a > kCsrLongRow) continue;  // summed by csr_long_row
-    AT s = 0;
+    AT s = epi_init<AT>(epi, r, 0);
     const int lend = static_cast<int>(min(static_cast<long>(b), e1) - e0);
     for (int e = a - static_cast<int>(e0); e < lend; e++) s += lds.prod[e];
     // the rest of a row that runs past the tile, same order, 8 loads in flight
@@ -258,7 +269,7 @@ __device__ inline void csr_long_row_at(const CsrView<VT> &m, const XT *__restric
     for (long e = threadIdx.x; e < cnt; e += kTileThreads) acc += Prod<VT, XT>::one(m, x, base + e);
     total += block_tree_sum<AT>(acc, lds.prod);
   }
-  if (threadIdx.x == 0) epi(r, total);
+  if (threadIdx.x == 0) epi(r, epi_init<AT>(epi, r, 0) + total);
 }
 
 template <typename VT, typename XT, typename Epi>
