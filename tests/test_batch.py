@@ -74,3 +74,37 @@ def test_batch_max_iteration_and_zero(tb, oracle, gpu):
         pr, it = o.personalized_page_rank(0.0, 0.85, 3, np.array(spec[k][4][0], dtype=np.int32))
         assert it == 3
         assert np.abs(out[off[k]:off[k + 1]] - pr).sum() <= 1e-14
+
+
+@pytest.mark.slow
+def test_full_size_personalised_batch(tb, gpu):
+    """The full batch (256 graphs, n log-uniform in [2^10, 2^18], plus the
+    2^22 graph with 64 one-hot sets): every set sums to 1, FP32 lands within
+    1e-5 L1 of FP64 for every set, and sampled sets equal the single-graph
+    FP64 engine run with the same stopping rule."""
+    import torch
+    spec = tb.personalised_workload()
+    mats = [tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s) for (n, c, q, s, _) in spec]
+    outs = {}
+    for prec, eps in (("f32", 1e-6), ("f64", 1e-10)):
+        b = tb.PageRankBatch(prec)
+        for m, sp in zip(mats, spec):
+            b.add_graph(m, sp[4])
+        out, rep, its = b.run(epsilon=eps, rule="sumnorm", max_iteration=1000)
+        off = b.layout()
+        outs[prec] = out.cpu().numpy()
+        del b, out
+    assert off[-1] == sum(n * len(sets) for (n, _, _, _, sets) in spec)
+    sums = np.add.reduceat(outs["f64"], off[:-1])
+    assert np.all(np.abs(sums - 1) < 1e-12)
+    for k in range(len(off) - 1):
+        assert np.abs(outs["f32"][off[k]:off[k + 1]] - outs["f64"][off[k]:off[k + 1]]).sum() <= 1e-5, k
+    # sampled sets against the single-graph engine (FP64, same rule)
+    k_of = np.cumsum([0] + [len(sp[4]) for sp in spec])
+    for g in (0, 77, 255, 256):
+        m, sets = mats[g], spec[g][4]
+        for j in (0, len(sets) - 1):
+            o = torch.zeros(m.get_n_col(), dtype=torch.float64, device=gpu)
+            m.page_rank_device(o, epsilon=1e-10, max_iteration=1000, rule="sumnorm", nodes=sets[j], gather="tiles")
+            k = k_of[g] + j
+            assert np.abs(o.cpu().numpy() - outs["f64"][off[k]:off[k + 1]]).sum() <= 1e-12, (g, j)

@@ -128,3 +128,24 @@ def test_page_rank_zero_iterations(tb, gpu):
     m = tb.MatrixCSR().fill_cqmat_stochastic(100, 100, 4, 0.5, 2)
     pr, its = m.page_rank(1e-10, 0.85, 0)
     assert its == 0 and np.allclose(pr, 1.0 / 100)
+
+
+@pytest.mark.slow
+def test_full_size_pagerank_2_24(tb, gpu):
+    """fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) at full size: every
+    precision with propagation-blocked products, and FP64 with fixed-order
+    tiles, lands within 1e-10 L1 of the FP64 fixed point (100 reference-rule
+    iterations, tile products) and sums to 1."""
+    import torch
+    n = 1 << 24
+    g = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 8, 0.5, 7)
+    pstar = torch.zeros(n, dtype=torch.float64, device=gpu)
+    g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
+    out = torch.zeros_like(pstar)
+    for prec, gather in [("f64", "tiles"), ("f64", "blocked"), ("pattern", "blocked"), ("f32", "blocked"),
+                         ("bf16", "blocked")]:
+        rep = g.page_rank_device(out, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm",
+                                 gather=gather)
+        err = float((out - pstar).abs().sum())
+        assert err <= 1e-10, (prec, gather, err, rep)
+        assert abs(float(out.sum()) - 1.0) < 1e-12
