@@ -84,50 +84,68 @@ __global__ __launch_bounds__(kTileThreads) void csr_long_kernel(dev::CsrView<VT>
 
 // Four consecutive rows per thread: per slot one 16-byte load of the four
 // column indices and 16 bytes (F32) or 32 bytes (F64) of values, then four
-// gathers; each row still sums its slots in order 0..K-1.
+// gathers; each row still sums its slots in order 0..K-1.  Slots go in pairs,
+// and the next pair's loads are issued before this pair's gathers (two
+// register sets, alternating), so each step waits on one round trip.
 template <typename VT>
-struct Val4 {
-  __device__ static void get(const VT *p, long i, typename Val<VT>::A (&v)[4]) { Val<VT>::get4(p, i, v); }
+struct EllPair {
+  using AT = typename Val<VT>::A;
+  int c[2][4];
+  AT v[2][4];
+
+  template <int S>
+  __device__ void load(const int *__restrict__ cols, const VT *__restrict__ vals, long ld, long i0, int j, int f_col) {
+#pragma unroll
+    for (int u = 0; u < S; u++) {
+      const long o = (j + u) * ld + i0;
+      const dev::v4i ci = __builtin_nontemporal_load(reinterpret_cast<const dev::v4i *>(cols + o));
+      // MatrixELL.cpp:110: x[max(col - f_col, 0)]
+      c[u][0] = max(ci.x - f_col, 0);
+      c[u][1] = max(ci.y - f_col, 0);
+      c[u][2] = max(ci.z - f_col, 0);
+      c[u][3] = max(ci.w - f_col, 0);
+      Val<VT>::get4(vals, o, v[u]);
+    }
+  }
+  template <int S, typename XT>
+  __device__ void accumulate(const XT *__restrict__ x, AT (&acc)[4]) const {
+#pragma clang fp contract(off)
+    AT xv[S][4];
+#pragma unroll
+    for (int u = 0; u < S; u++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) xv[u][r] = static_cast<AT>(x[c[u][r]]);
+#pragma unroll
+    for (int u = 0; u < S; u++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[r] += v[u][r] * xv[u][r];
+  }
 };
 
 template <typename VT, typename XT, typename YT>
 __global__ __launch_bounds__(256) void ell_kernel(const int *__restrict__ cols, const VT *__restrict__ vals,
                                                   const XT *__restrict__ x, YT *__restrict__ y, int rows, int K,
                                                   long ld, int f_col) {
-#pragma clang fp contract(off)
   using AT = typename Val<VT>::A;
   const long i0 = (static_cast<long>(blockIdx.x) * 256 + threadIdx.x) * 4;
   if (i0 >= rows) return;
   AT acc[4] = {0, 0, 0, 0};
+  EllPair<VT> A, B;
+  const int K2 = K & ~1;  // slots in whole pairs
+  if (K2 > 0) A.template load<2>(cols, vals, ld, i0, 0, f_col);
   int j = 0;
-  for (; j + 2 <= K; j += 2) {
-    int c[2][4];
-    AT v[2][4], xv[2][4];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const dev::v4i ci = __builtin_nontemporal_load(reinterpret_cast<const dev::v4i *>(cols + (j + u) * ld + i0));
-      c[u][0] = ci.x - f_col;
-      c[u][1] = ci.y - f_col;
-      c[u][2] = ci.z - f_col;
-      c[u][3] = ci.w - f_col;
-      Val4<VT>::get(vals, (j + u) * ld + i0, v[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < 2; u++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) xv[u][r] = static_cast<AT>(x[c[u][r] < 0 ? 0 : c[u][r]]);
-#pragma unroll
-    for (int u = 0; u < 2; u++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) acc[r] += v[u][r] * xv[u][r];
+  while (j < K2) {
+    if (j + 2 < K2) B.template load<2>(cols, vals, ld, i0, j + 2, f_col);
+    A.template accumulate<2>(x, acc);
+    j += 2;
+    if (j >= K2) break;
+    if (j + 2 < K2) A.template load<2>(cols, vals, ld, i0, j + 2, f_col);
+    B.template accumulate<2>(x, acc);
+    j += 2;
   }
-  for (; j < K; j++) {
-    const dev::v4i ci = *reinterpret_cast<const dev::v4i *>(cols + j * ld + i0);
-    const int c[4] = {ci.x - f_col, ci.y - f_col, ci.z - f_col, ci.w - f_col};
-    AT v[4];
-    Val4<VT>::get(vals, j * ld + i0, v);
-#pragma unroll
-    for (int r = 0; r < 4; r++) acc[r] += v[r] * static_cast<AT>(x[c[r] < 0 ? 0 : c[r]]);
+  if (j < K) {  // odd K: the last slot
+    A.template load<1>(cols, vals, ld, i0, j, f_col);
+    A.template accumulate<1>(x, acc);
   }
 #pragma unroll
   for (int r = 0; r < 4; r++)

@@ -121,19 +121,24 @@ inline CsrView<VT> csr_view(const DeviceMatrix &d) {
                      d.long_rows.as<int>(), d.entries, d.rp0, d.f_col};
 }
 
+// A row's sum reads its products from LDS kSumRun at a time, all reads issued
+// before the adds; the slack slots after the tile keep those reads in bounds
+// (their contents are never added).
+constexpr int kSumRun = 16;
+
 template <typename AT>
 struct CsrTileLds {
-  AT prod[kCsrTileNnz];
+  AT prod[kCsrTileNnz + kSumRun];
 };
 
 // Value source: stored values (VT) or value-free (every product is z[c]).
+// load4 reads four consecutive values (issued before the gathers that need
+// the column indices), apply4 gathers x and forms the rounded products.
 template <typename VT, typename XT>
 struct Prod {
   using AT = typename Val<VT>::A;
-  __device__ static void group(const CsrView<VT> &m, const XT *__restrict__ x, long e, const int (&c)[4],
-                               AT (&p)[4]) {
-    AT v[4];
-    Val<VT>::get4(m.vals, e, v);
+  __device__ static void load4(const CsrView<VT> &m, long e, AT (&v)[4]) { Val<VT>::get4(m.vals, e, v); }
+  __device__ static void apply4(const AT (&v)[4], const XT *__restrict__ x, const int (&c)[4], AT (&p)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; q++) p[q] = v[q] * static_cast<AT>(x[c[q]]);
   }
@@ -151,8 +156,8 @@ struct Val<Pattern> {
 template <typename XT>
 struct Prod<Pattern, XT> {
   using AT = double;
-  __device__ static void group(const CsrView<Pattern> &, const XT *__restrict__ z, long, const int (&c)[4],
-                               double (&p)[4]) {
+  __device__ static void load4(const CsrView<Pattern> &, long, double (&)[4]) {}
+  __device__ static void apply4(const double (&)[4], const XT *__restrict__ z, const int (&c)[4], double (&p)[4]) {
 #pragma unroll
     for (int q = 0; q < 4; q++) p[q] = static_cast<double>(z[c[q]]);
   }
@@ -172,6 +177,81 @@ __device__ inline AT epi_init(const E &, int, long) {
   return AT(0);
 }
 
+// The entries [e0, e0 + ne) of one tile held by the calling thread: groups
+// of four at 4 * (k * kTileThreads + threadIdx.x).  load() issues the index and
+// value loads; products() gathers x and writes the rounded products to
+// prod[0 .. ne).  FULL (ne == kCsrTileNnz, every tile but the matrix's last)
+// reads with no per-entry guard; otherwise entries at or past ne are not read
+// (a caller's value array holds exactly nnz values).
+template <bool FULL, typename VT, typename XT>
+struct TileEntries {
+  using AT = typename Val<VT>::A;
+  static constexpr int G = kCsrTileNnz / 4 / kTileThreads;  // 4-entry groups per thread
+  int c[G][4];
+  AT v[G][4];
+
+  __device__ void load(const CsrView<VT> &m, long e0, int ne) {
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      const int g = k * kTileThreads + threadIdx.x;
+      if (FULL || 4 * g < ne) {
+        const v4i ci = *reinterpret_cast<const v4i *>(m.colidx + e0 + 4 * g);
+        c[k][0] = ci.x - m.f_col;
+        c[k][1] = ci.y - m.f_col;
+        c[k][2] = ci.z - m.f_col;
+        c[k][3] = ci.w - m.f_col;
+        if (!FULL)  // entries past the matrix end read the zeroed padding: column 0
+          for (int q = 0; q < 4; q++)
+            if (4 * g + q >= ne) c[k][q] = 0;
+      }
+      if (FULL || 4 * g + 4 <= ne) Prod<VT, XT>::load4(m, e0 + 4 * g, v[k]);
+    }
+  }
+
+  // gathers and rounded products into p
+  __device__ void products(const CsrView<VT> &m, const XT *__restrict__ x, long e0, int ne, AT (&p)[G][4]) const {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      const int g = k * kTileThreads + threadIdx.x;
+      if (FULL || 4 * g + 4 <= ne)
+        Prod<VT, XT>::apply4(v[k], x, c[k], p[k]);
+      else if (4 * g < ne)  // the matrix's last entries: no value read past nnz
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (4 * g + q < ne) p[k][q] = Prod<VT, XT>::one(m, x, e0 + 4 * g + q);
+    }
+  }
+
+  static __device__ void store(const AT (&p)[G][4], AT *prod, int ne) {
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      const int g = k * kTileThreads + threadIdx.x;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (FULL || 4 * g + q < ne) prod[4 * g + q] = p[k][q];
+    }
+  }
+};
+
+// The bounds of the thread's first owned row, then the index/value loads,
+// gathers and products into prod.  (Issuing the row table first measured
+// faster than deferring it behind the tile loads.)
+template <bool FULL, typename VT, typename XT>
+__device__ inline void csr_tile_stage(const CsrView<VT> &m, const XT *__restrict__ x, typename Val<VT>::A *prod,
+                                      long e0, int ne, int rlo, int rhi, int &a, int &b) {
+  // loaded with no branch so they stay in flight under the tile loads
+  // (indices clamped to rhi <= rows: in bounds, and exact for every thread
+  // that owns a row)
+  a = m.rowptr[min(rlo + static_cast<int>(threadIdx.x), rhi)] - m.rp0;
+  b = m.rowptr[min(rlo + static_cast<int>(threadIdx.x) + 1, rhi)] - m.rp0;
+  TileEntries<FULL, VT, XT> t;
+  t.load(m, e0, ne);
+  typename Val<VT>::A p[TileEntries<FULL, VT, XT>::G][4];
+  t.products(m, x, e0, ne, p);
+  TileEntries<FULL, VT, XT>::store(p, prod, ne);
+}
+
 // Rows of entry tile `tile`; epi(row, sum) once per owned row by the thread
 // that sums it.  The caller synchronises the block before reusing `lds`.
 template <typename VT, typename XT, typename Epi>
@@ -179,49 +259,15 @@ __device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restri
                                         CsrTileLds<typename Val<VT>::A> &lds, Epi &epi, int tile) {
 #pragma clang fp contract(off)
   using AT = typename Val<VT>::A;
-  constexpr int G = kCsrTileNnz / 4 / kTileThreads;  // 4-entry groups per thread
   const int tid = threadIdx.x;
   const long e0 = static_cast<long>(tile) * kCsrTileNnz;
   const int ne = static_cast<int>(min(static_cast<long>(kCsrTileNnz), m.nnz - e0));
-  int c[G][4];
-  AT p[G][4];
-#pragma unroll
-  for (int k = 0; k < G; k++) {
-    const int g = k * kTileThreads + tid;
-    if (4 * g < ne) {
-      const v4i ci = *reinterpret_cast<const v4i *>(m.colidx + e0 + 4 * g);
-      c[k][0] = ci.x - m.f_col;
-      c[k][1] = ci.y - m.f_col;
-      c[k][2] = ci.z - m.f_col;
-      c[k][3] = ci.w - m.f_col;
-      // entries past the matrix end read the zeroed padding: column 0
-      for (int q = 0; q < 4; q++)
-        if (4 * g + q >= ne) c[k][q] = 0;
-    }
-  }
   const int rlo = m.tile_row[tile], rhi = m.tile_row[tile + 1];
-  int a = 0, b = 0;
-  if (rlo + tid < rhi) {
-    a = m.rowptr[rlo + tid] - m.rp0;
-    b = m.rowptr[rlo + tid + 1] - m.rp0;
-  }
-#pragma unroll
-  for (int k = 0; k < G; k++) {
-    const int g = k * kTileThreads + tid;
-    if (4 * g + 4 <= ne)
-      Prod<VT, XT>::group(m, x, e0 + 4 * g, c[k], p[k]);
-    else if (4 * g < ne)  // the matrix's last entries: no value read past nnz
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (4 * g + q < ne) p[k][q] = Prod<VT, XT>::one(m, x, e0 + 4 * g + q);
-  }
-#pragma unroll
-  for (int k = 0; k < G; k++) {
-    const int g = k * kTileThreads + tid;
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-      if (4 * g + q < ne) lds.prod[4 * g + q] = p[k][q];
-  }
+  int a, b;
+  if (ne == kCsrTileNnz)
+    csr_tile_stage<true, VT, XT>(m, x, lds.prod, e0, ne, rlo, rhi, a, b);
+  else
+    csr_tile_stage<false, VT, XT>(m, x, lds.prod, e0, ne, rlo, rhi, a, b);
   __syncthreads();
   const long e1 = e0 + ne;
   for (int r = rlo + tid; r < rhi; r += kTileThreads) {
@@ -231,8 +277,16 @@ __device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restri
     }
     if (b - a > kCsrLongRow) continue;  // summed by csr_long_row
     AT s = epi_init<AT>(epi, r, 0);
+    // a row owned by the tile starts in it: 0 <= a - e0 < kCsrTileNnz, so the
+    // reads below stay inside prod[kCsrTileNnz + kSumRun]
     const int lend = static_cast<int>(min(static_cast<long>(b), e1) - e0);
-    for (int e = a - static_cast<int>(e0); e < lend; e++) s += lds.prod[e];
+    for (int e = a - static_cast<int>(e0); e < lend; e += kSumRun) {
+      AT v[kSumRun];
+#pragma unroll
+      for (int u = 0; u < kSumRun; u++) v[u] = lds.prod[e + u];
+#pragma unroll
+      for (int u = 0; u < kSumRun; u++) s = e + u < lend ? s + v[u] : s;
+    }
     // the rest of a row that runs past the tile, same order, 8 loads in flight
     for (long e = e1; e < b; e += 8) {
       AT q8[8];
