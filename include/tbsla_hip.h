@@ -56,6 +56,11 @@ int tbsla_matrix_info(const tbsla_matrix *m, long *out16);
 int tbsla_matrix_array(const tbsla_matrix *m, int which, const void **data, size_t *n);
 int tbsla_matrix_write(tbsla_matrix *m, const char *path); /* appends, like C_CPP_Matrix<F>_write */
 int tbsla_matrix_read(tbsla_matrix *m, const char *path, long seek);
+/* MatrixMarket "coordinate real general|symmetric" into a COO handle
+ * (MatrixCOO::readMM, MatrixCOO.cpp:230-292).  TBSLA_EIO when the file cannot
+ * be opened (the reference returns silently), TBSLA_EINVAL for another format
+ * or an unsupported header (the reference throws MatrixFormatReadException). */
+int tbsla_matrix_read_mm(tbsla_matrix *m, const char *path);
 int tbsla_matrix_count_stochastic(const tbsla_matrix *m);
 
 /* ---- host-vector products on the GPU ---- */

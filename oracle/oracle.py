@@ -379,6 +379,8 @@ class _Reference(_Backend):
         L.ref_a_axpx.argtypes = [C.c_void_p, _dp, C.c_size_t, _dp]
         L.ref_page_rank.restype = C.c_int
         L.ref_page_rank.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp]
+        L.ref_read_mm.restype = C.c_int
+        L.ref_read_mm.argtypes = [C.c_void_p, C.c_char_p]
         L.ref_personalized_page_rank.restype = C.c_int
         L.ref_personalized_page_rank.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _ip,
                                                  C.c_int, _dp]
@@ -444,6 +446,14 @@ class _Reference(_Backend):
 
     def _ppr(self, h, eps, beta, it, nodes, out):
         return self.lib.ref_personalized_page_rank(h, eps, beta, it, nodes, nodes.size, out)
+
+    def read_mm(self, path: str) -> Mat:
+        """MatrixCOO::readMM (MatrixCOO.cpp:230-292); ValueError where the
+        reference throws MatrixFormatReadException."""
+        m = self.new("COO")
+        if not self.lib.ref_read_mm(m.h, path.encode()):
+            raise ValueError("MatrixFormatReadException")
+        return m
 
     def Ax_repeat(self, m: Mat, r, x, reps):
         r = _f64(r).copy()

@@ -142,4 +142,15 @@ int ref_personalized_page_rank(void *m, double epsilon, double beta, int max_it,
   return it;
 }
 
+// MatrixCOO::readMM (MatrixCOO.cpp:230-292); returns 0 when the reference
+// throws MatrixFormatReadException
+int ref_read_mm(void *m, const char *path) {
+  try {
+    dynamic_cast<MatrixCOO *>(static_cast<Matrix *>(m))->readMM(std::string(path));
+  } catch (const MatrixFormatReadException &) {
+    return 0;
+  }
+  return 1;
+}
+
 }  // extern "C"

@@ -78,6 +78,7 @@ SIGNATURES = {
     "tbsla_matrix_array": (_i, [_vp, _i, C.POINTER(_vp), C.POINTER(_sz)]),
     "tbsla_matrix_write": (_i, [_vp, C.c_char_p]),
     "tbsla_matrix_read": (_i, [_vp, C.c_char_p, _l]),
+    "tbsla_matrix_read_mm": (_i, [_vp, C.c_char_p]),
     "tbsla_matrix_count_stochastic": (_i, [_vp]),
     "tbsla_matrix_spmv": (_i, [_vp, _dp, _sz, _i, _dp, _sz]),
     "tbsla_matrix_Ax": (_i, [_vp, _dp, _sz, _dp, _sz, _i]),

@@ -404,6 +404,21 @@ int tbsla_matrix_read(tbsla_matrix *m, const char *path, long seek) {
   });
 }
 
+int tbsla_matrix_read_mm(tbsla_matrix *m, const char *path) {
+  if (!m || !path) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  auto *coo = dynamic_cast<cpp::MatrixCOO *>(m->m.get());
+  if (!coo) return fail(TBSLA_EINVAL, "tbsla: readMM needs a COO matrix");
+  if (!std::ifstream(path).is_open()) return fail(TBSLA_EIO, std::string("tbsla: cannot open ") + path);
+  try {
+    coo->readMM(path);
+  } catch (const cpp::MatrixFormatReadException &e) {
+    return fail(TBSLA_EINVAL, std::string("tbsla: ") + e.what());
+  } catch (const std::exception &e) {
+    return fail(TBSLA_EOTHER, e.what());
+  }
+  return TBSLA_OK;
+}
+
 int tbsla_matrix_count_stochastic(const tbsla_matrix *m) { return m && m->m->count_stochastic() ? 1 : 0; }
 
 int tbsla_matrix_spmv(tbsla_matrix *m, const double *x, size_t nx, int vect_incr, double *y, size_t ny) {

@@ -290,6 +290,13 @@ class MatrixCOO(Matrix):
             _handle = _from_arrays(self.FORMAT, n_row, n_col, values, row, col)
         super().__init__(_handle)
 
+    def readMM(self, path: str):
+        """MatrixMarket coordinate real general/symmetric (MatrixCOO.cpp:230-292):
+        1-based indices, symmetric off-diagonal entries mirrored right after
+        their original.  Raises ValueError for another header."""
+        check(lib.tbsla_matrix_read_mm(self._h, path.encode()))
+        return self
+
 
 class MatrixSCOO(Matrix):
     FORMAT, NAME, _IDX0, _IDX1 = _lib.SCOO, "SCOO", "row", "col"

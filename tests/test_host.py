@@ -98,6 +98,68 @@ def test_binary_io_matches_oracle_format(tb, oracle, tmp_path):
         same_matrix(p, oracle.read(fmt, f, size1), fmt)
 
 
+MM_GENERAL = """%%MatrixMarket matrix coordinate real general
+% comment lines are skipped
+%
+3 4 5
+1 1 2.5
+3 4 -1
+2 2 0.125
+1 4 7
+3 1 1e-3
+"""
+MM_SYMMETRIC = """%%MatrixMarket matrix coordinate real symmetric
+4 4 4
+1 1 1.5
+3 1 2
+4 2 -3
+4 4 0.5
+"""
+
+
+def test_read_matrix_market(tb, tmp_path):
+    """readMM (MatrixCOO.cpp:230-292): 1-based indices in file order, and for
+    'symmetric' each off-diagonal entry followed by its mirror."""
+    from tbsla_amd import _lib
+    f = tmp_path / "g.mtx"
+    f.write_text(MM_GENERAL)
+    m = tb.MatrixCOO().readMM(str(f))
+    a, info = m.arrays(), m.info()
+    assert (info["n_row"], info["n_col"], info["nnz"]) == (3, 4, 5)
+    assert a["row"].tolist() == [0, 2, 1, 0, 2] and a["col"].tolist() == [0, 3, 1, 3, 0]
+    assert a["values"].tolist() == [2.5, -1.0, 0.125, 7.0, 1e-3]
+    f = tmp_path / "s.mtx"
+    f.write_text(MM_SYMMETRIC)
+    m = tb.MatrixCOO().readMM(str(f))
+    a, info = m.arrays(), m.info()
+    assert (info["n_row"], info["n_col"], info["nnz"]) == (4, 4, 6)
+    assert a["row"].tolist() == [0, 2, 0, 3, 1, 3] and a["col"].tolist() == [0, 0, 2, 1, 3, 3]
+    assert a["values"].tolist() == [1.5, 2.0, 2.0, -3.0, -3.0, 0.5]
+    for header in ("%%MatrixMarket matrix array real general", "%%MatrixMarket matrix coordinate complex general",
+                   "%%MatrixMarket matrix coordinate real hermitian"):
+        f.write_text(header + "\n1 1 1\n1 1 1\n")
+        with pytest.raises(_lib.TbslaError) as e:
+            tb.MatrixCOO().readMM(str(f))
+        assert e.value.code == -1
+    with pytest.raises(_lib.TbslaError) as e:
+        tb.MatrixCOO().readMM(str(tmp_path / "missing.mtx"))
+    assert e.value.code == -3
+    f.write_text(MM_GENERAL)
+    assert _lib.lib.tbsla_matrix_read_mm(tb.MatrixCSR()._h, str(f).encode()) == -1
+
+
+def test_read_matrix_market_matches_reference(tb, reference, tmp_path):
+    """The same files through the compiled reference's own readMM."""
+    for name, text in (("g", MM_GENERAL), ("s", MM_SYMMETRIC)):
+        f = tmp_path / f"{name}.mtx"
+        f.write_text(text)
+        same_matrix(tb.MatrixCOO().readMM(str(f)), reference.read_mm(str(f)), "COO")
+    f = tmp_path / "bad.mtx"
+    f.write_text("%%MatrixMarket matrix coordinate real skew-symmetric\n1 1 1\n1 1 1\n")
+    with pytest.raises(ValueError):
+        reference.read_mm(str(f))
+
+
 @pytest.mark.slow
 def test_full_size_generators_match_reference_digests(tb, digests):
     cases = {
