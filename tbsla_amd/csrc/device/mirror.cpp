@@ -362,9 +362,14 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
       }
     }
     for (int j = 0; j < n; j++) colcnt[static_cast<std::size_t>(j) + 1] += colcnt[static_cast<std::size_t>(j)];
+    // each bin's local run is padded to a multiple of kPbLocalGroup entries
+    // (row kPbPadRow, column 0, value 0), so the gather reads groups with
+    // aligned vector loads
     for (int b = 0; b < nb; b++) {
       bincnt[static_cast<std::size_t>(b) + 1] += bincnt[static_cast<std::size_t>(b)];
-      loccnt[static_cast<std::size_t>(b) + 1] += loccnt[static_cast<std::size_t>(b)];
+      const int lc = loccnt[static_cast<std::size_t>(b) + 1];
+      loccnt[static_cast<std::size_t>(b) + 1] =
+          loccnt[static_cast<std::size_t>(b)] + (lc + kPbLocalGroup - 1) / kPbLocalGroup * kPbLocalGroup;
     }
     pb->n_far = colcnt[static_cast<std::size_t>(n)];
     pb->n_loc = loccnt[static_cast<std::size_t>(nb)];
@@ -372,8 +377,8 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
     const std::size_t nf = static_cast<std::size_t>(pb->n_far), nl = static_cast<std::size_t>(pb->n_loc);
     // pass 2: far entries in column order (rows ascending), local entries bin-grouped
     std::vector<int> csc_src(nf), csc_row(nf);
-    pb->loc_src.resize(nl);
-    std::vector<unsigned short> lrow(nl), lcol(nl);
+    pb->loc_src.assign(nl, -1);
+    std::vector<unsigned short> lrow(nl, static_cast<unsigned short>(kPbPadRow)), lcol(nl, 0);
     std::vector<int> ccur(colcnt.begin(), colcnt.end() - 1), lcur(loccnt.begin(), loccnt.end() - 1);
     for (int r = 0; r < n; r++) {
       const int b = r >> kPbShift;
@@ -439,11 +444,11 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
   if (vdt != Dtype::PATTERN && !pb.fval[static_cast<int>(vdt)].get()) {
     const auto &va = c.get_values();
     const long r0 = c.get_rowptr()[0];
-    if (static_cast<long>(va.size()) < r0 + static_cast<long>(pb.far_src.size() + pb.loc_src.size()))
-      bad("CSR rowptr points past values");
+    if (static_cast<long>(va.size()) < static_cast<long>(c.get_rowptr().back())) bad("CSR rowptr points past values");
     std::vector<double> fv(pb.far_src.size()), lv(pb.loc_src.size());
     for (std::size_t k = 0; k < fv.size(); k++) fv[k] = va[static_cast<std::size_t>(r0 + pb.far_src[k])];
-    for (std::size_t k = 0; k < lv.size(); k++) lv[k] = va[static_cast<std::size_t>(r0 + pb.loc_src[k])];
+    for (std::size_t k = 0; k < lv.size(); k++)
+      lv[k] = pb.loc_src[k] >= 0 ? va[static_cast<std::size_t>(r0 + pb.loc_src[k])] : 0.0;  // padding: 0
     upload_values(pb.fval[static_cast<int>(vdt)], fv.data(), fv.size(), vdt);
     upload_values(pb.lval[static_cast<int>(vdt)], lv.data(), lv.size(), vdt);
     if (fv.empty()) pb.fval[static_cast<int>(vdt)].alloc(8);

@@ -74,19 +74,23 @@ struct DeviceMatrix {
 //          in the bin (bp: bin boundaries).  Within a bin the products of one
 //          superblock are contiguous, so the scatter pass writes runs.
 //   local  bin-grouped (lp, lrow, lcol = offsets in the bin, values), summed
-//          by the gather pass with x read from the bin's own slice
+//          by the gather pass with x read from the bin's own slice; each bin's
+//          run is padded to a multiple of kPbLocalGroup entries with row
+//          kPbPadRow (no such row), column 0 and value 0
 constexpr int kPbShift = 13;
 constexpr int kPbRows = 1 << kPbShift;
 constexpr int kPbColShift = 13;
+constexpr int kPbLocalGroup = 4;
+constexpr int kPbPadRow = 0xFFFF;
 constexpr int kPbCols = 1 << kPbColShift;
 
 struct PbMatrix {
   int n = 0, nbins = 0, nsb = 0;
-  long n_far = 0, n_loc = 0;
+  long n_far = 0, n_loc = 0;  // n_loc counts the padding entries too
   DevBuf sp, fcol, fslot, frow, bp, lp, lrow, lcol;
   DevBuf fval[3], lval[3];  // by Dtype F64, F32, BF16 (built on demand)
   DevBuf prod;              // 8-byte product slots, n_far
-  std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry
+  std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry (-1: padding)
 };
 
 struct DeviceCache {

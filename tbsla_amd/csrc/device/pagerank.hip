@@ -376,11 +376,32 @@ struct PbVal {
   __device__ static AT prod(const VT *v, long k, AT x) {
     return static_cast<AT>(dev::Val<VT>::get(v, k)) * x;
   }
+  // four values from a multiple of four
+  template <typename AT>
+  __device__ static void get4(const VT *v, long k, AT (&out)[4]) {
+    typename dev::Val<VT>::A t[4];
+    dev::Val<VT>::get4(v, k, t);
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = static_cast<AT>(t[q]);
+  }
+  template <typename AT>
+  __device__ static AT mul(AT v, AT x) {
+    return v * x;
+  }
 };
 template <>
 struct PbVal<dev::Pattern> {
   template <typename AT>
   __device__ static AT prod(const dev::Pattern *, long, AT x) {
+    return x;
+  }
+  template <typename AT>
+  __device__ static void get4(const dev::Pattern *, long, AT (&out)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = AT(1);
+  }
+  template <typename AT>
+  __device__ static AT mul(AT, AT x) {
     return x;
   }
 };
@@ -474,35 +495,32 @@ __global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict
     for (int u = 0; u < U; u++)
       if (r[u] >= 0) atomicAdd(&ybin[r[u]], pv[u]);
   }
-  // local entries are in row order: each thread takes four consecutive
-  // entries (all loads first), sums runs of one row and adds each run once
-  for (int k = v.lp[bin] + 4 * tid; k < l1; k += 4 * 1024) {
-    int r[4], c[4];
-    AT lv[4], xv[4];
+  // local entries are in row order and each bin's run is padded to groups
+  // of four: one group per thread (aligned vector loads), runs of one row
+  // summed and added once
+  for (int k = v.lp[bin] + kPbLocalGroup * tid; k < l1; k += kPbLocalGroup * 1024) {
+    const dev::v4us rr = *reinterpret_cast<const dev::v4us *>(v.lrow + k);
+    const dev::v4us cc = *reinterpret_cast<const dev::v4us *>(v.lcol + k);
+    AT lv[4];
+    PbVal<VT>::template get4<AT>(lval, k, lv);
+    const int r[4] = {rr.x, rr.y, rr.z, rr.w};
+    const int c[4] = {cc.x, cc.y, cc.z, cc.w};
+    AT xv[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const bool ok = k + u < l1;
-      r[u] = ok ? v.lrow[k + u] : -1;
-      c[u] = ok ? v.lcol[k + u] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      xv[u] = static_cast<AT>(x[base + c[u]]);
-      lv[u] = r[u] >= 0 ? PbVal<VT>::template prod<AT>(lval, k + u, AT(1)) : AT(0);
-    }
+    for (int u = 0; u < 4; u++) xv[u] = static_cast<AT>(x[base + c[u]]);
     double acc = 0.0;
     int row = r[0];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      if (r[u] < 0) break;
+      if (r[u] == kPbPadRow) break;  // padding closes a bin's run
       if (r[u] != row) {
         atomicAdd(&ybin[row], acc);
         acc = 0.0;
         row = r[u];
       }
-      acc += static_cast<double>(lv[u] * xv[u]);
+      acc += static_cast<double>(PbVal<VT>::template mul<AT>(lv[u], xv[u]));
     }
-    if (row >= 0) atomicAdd(&ybin[row], acc);
+    if (row != kPbPadRow) atomicAdd(&ybin[row], acc);
   }
   __syncthreads();
   double lmax = -INFINITY;
