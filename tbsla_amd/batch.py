@@ -83,7 +83,7 @@ class PageRankBatch:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and lib is not None:  # module globals are cleared at interpreter exit
             lib.tbsla_batch_destroy(h)
             self._h = None
 

@@ -185,31 +185,60 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
   const VT *__restrict__ vals = G.vals;
   const unsigned long long *__restrict__ member = G.member;
   bool act[4];
-  double tele[4], lmax[4];
+  XT lmax[4];  // maxima of the stored (XT) iterate values
   bool any = false;
 #pragma unroll
   for (int t = 0; t < 4; t++) {
     act[t] = 4 * q + t < n_rhs && !d.rs[rhs0 + t].done;
-    tele[t] = act[t] ? d.rs[rhs0 + t].tele : 0.0;
     lmax[t] = -INFINITY;
     any = any || act[t];
   }
   const XT *X = xin + G.xoff;
   XT *Y = xout + G.xoff;
   if (__ballot(any) != 0) {
+    const int r0 = blk * kSpmmRows + wave * kSpmmRowsPerWave;
+    // bounds of the wave's rows, loaded once: lane l <= 16 holds rowptr[r0 + l]
+    // (clamped to rows, an index rowptr has)
+    const int rpl = lane <= kSpmmRowsPerWave ? rowptr[min(r0 + lane, rows)] - rp0 : 0;
+    // the next pass's row bounds, first 16 column indices and values, and
+    // membership word, read while the current pass's gathers are in flight
+    int a0n, a1n, cqn = 0;
+    AT vqn = 0;
+    unsigned long long mbn = 0;
+    auto ahead = [&](int pass) {
+      const int i = pass * 4 + sub, row = r0 + i;
+      const bool rv = row < rows;
+      a0n = __shfl(rpl, i, 64);
+      a1n = __shfl(rpl, i + 1, 64);
+      if (!rv) a0n = a1n = 0;
+      cqn = 0;
+      vqn = 0;
+      if (q < min(16, a1n - a0n)) {
+        cqn = colidx[a0n + q] - f_col;
+        vqn = dev::Val<VT>::get(vals, a0n + q);
+      }
+      mbn = rv ? member[row] : 0ull;
+    };
+    ahead(0);
 #pragma unroll 1
     for (int pass = 0; pass < kSpmmRowsPerWave / 4; pass++) {
-      const int row = blk * kSpmmRows + wave * kSpmmRowsPerWave + pass * 4 + sub;
+      const int row = r0 + pass * 4 + sub;
       const bool rv = row < rows;
-      const int a0 = rv ? rowptr[row] - rp0 : 0, a1 = rv ? rowptr[row + 1] - rp0 : 0;
+      const int a0 = a0n, a1 = a1n;
+      int cq = cqn;
+      AT vq = vqn;
+      const unsigned long long mbrow = mbn;
+      if (pass + 1 < kSpmmRowsPerWave / 4) ahead(pass + 1);
       AT acc[4] = {0, 0, 0, 0};
       for (int base = a0; __ballot(base < a1) != 0; base += 16) {
         const int cnt = min(16, a1 - base);  // <= 0 for a finished row
-        int cq = 0;
-        AT vq = 0;
-        if (q < cnt) {
-          cq = colidx[base + q] - f_col;
-          vq = dev::Val<VT>::get(vals, base + q);
+        if (base != a0) {  // rows longer than 16 entries: later chunks
+          cq = 0;
+          vq = 0;
+          if (q < cnt) {
+            cq = colidx[base + q] - f_col;
+            vq = dev::Val<VT>::get(vals, base + q);
+          }
         }
 #pragma unroll 1
         for (int k0 = 0; k0 < 16; k0 += UNR) {
@@ -234,15 +263,15 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
         }
       }
       if (rv && any) {
-        const unsigned long long mb = member[row] >> (4 * q);
+        const unsigned long long mb = mbrow >> (4 * q);
         XT o[4];
         bool all = true;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
           double bi = d.beta * static_cast<double>(acc[t]);
-          if ((mb >> t) & 1ull) bi = bi + tele[t];
+          if ((mb >> t) & 1ull) bi = bi + d.rs[rhs0 + t].tele;  // teleport rows are few: read here
           o[t] = static_cast<XT>(bi);
-          if (act[t]) lmax[t] = fmax(lmax[t], static_cast<double>(o[t]));
+          if (act[t]) lmax[t] = max(lmax[t], o[t]);
           all = all && act[t];
         }
         XT *yp = Y + static_cast<long>(row) * kLanes + 4 * q;
@@ -259,12 +288,12 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
   // maxima: the four row groups of the wave, then the four wavefronts
 #pragma unroll
   for (int t = 0; t < 4; t++) {
-    lmax[t] = fmax(lmax[t], __shfl_xor(lmax[t], 16, 64));
-    lmax[t] = fmax(lmax[t], __shfl_xor(lmax[t], 32, 64));
+    lmax[t] = max(lmax[t], __shfl_xor(lmax[t], 16, 64));
+    lmax[t] = max(lmax[t], __shfl_xor(lmax[t], 32, 64));
   }
   if (sub == 0)
 #pragma unroll
-    for (int t = 0; t < 4; t++) red[wave * 64 + 4 * q + t] = lmax[t];
+    for (int t = 0; t < 4; t++) red[wave * 64 + 4 * q + t] = static_cast<double>(lmax[t]);
   __syncthreads();
   if (wave == 0) {
     const int j = lane;  // RHS j of the group

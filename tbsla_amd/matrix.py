@@ -70,7 +70,7 @@ class Matrix:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and lib is not None:  # module globals are cleared at interpreter exit
             lib.tbsla_matrix_destroy(h)
             self._h = None
 
