@@ -5,6 +5,10 @@ fill_cqmat(2^24, 2^24, c=16, q=0.1, seed 42) -- 4,194,304 x 4,194,304,
 62,075,592 entries -- with values 1 - u01(42, i) and x_j = u01(1, j).
 A step is one FP64 CSR SpMV of that block with the inputs resident in HBM.
 
+CSR matrices are generated on the GPU (fill_cqmat(..., device=True)), which
+gives the host generators' arrays; the other formats are generated on the
+host.
+
 Extra fields:
   formats        ELL / SCOO / COO / DENSE on the same block
   cpu_baseline   the reference's own MatrixCSR::Ax (compiled from its sources)
@@ -136,7 +140,7 @@ def leg_formats(torch, tb, dev, steps):
 
 def leg_pagerank(torch, tb, dev):
     log("generating the PageRank graph")
-    g = tb.MatrixCSR().fill_cqmat_stochastic(N, N, 8, 0.5, 7)
+    g = tb.MatrixCSR().fill_cqmat_stochastic(N, N, 8, 0.5, 7, device=True)
     n = g.get_n_col()
     pstar = torch.zeros(n, dtype=torch.float64, device=dev)
     g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
@@ -161,7 +165,7 @@ def leg_batch(torch, tb, dev):
     log("generating the personalised batch (256 graphs + 2^22 x 64)")
     t = time.time()
     spec = tb.personalised_workload()
-    mats = [tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s) for (n, c, q, s, _) in spec]
+    mats = [tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s, device=True) for (n, c, q, s, _) in spec]
     log(f"generated in {time.time() - t:.1f}s")
     res = {}
     outs = {}
@@ -204,7 +208,7 @@ def leg_batch(torch, tb, dev):
 def leg_adjoint(torch, tb, dev):
     log("generating the adjoint graph")
     n = 1 << 20
-    g = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 16, 0.2, 3)
+    g = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 16, 0.2, 3, device=True)
     p = torch.empty(n, dtype=torch.float64, device=dev)
     fwd = g.page_rank_device(p, epsilon=1e-15, max_iteration=400, rule="sumnorm")
     w = torch.from_numpy(tb.normal_array(3, n)).to(dev)
@@ -255,7 +259,7 @@ def main():
     torch.cuda.set_device(dev)
     log("generating the CSR block")
     t = time.time()
-    csr = tb.MatrixCSR().fill_cqmat(N, N, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+    csr = tb.MatrixCSR().fill_cqmat(N, N, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42, device=True)
     log(f"generated in {time.time() - t:.1f}s")
     info = csr.info()
     x_host = tb.u01_array(1, 0, info["ln_col"])

@@ -45,6 +45,13 @@ int tbsla_matrix_fill_cqmat(tbsla_matrix *m, int n_row, int n_col, int c, double
                             int pc, int NR, int NC, uint64_t value_seed);
 int tbsla_matrix_fill_cqmat_stochastic(tbsla_matrix *m, int n_row, int n_col, int c, double q,
                                        unsigned int seed_mult, int pr, int pc, int NR, int NC);
+/* fill_cqmat (stochastic = 0, with the value overlay above) or
+ * fill_cqmat_stochastic (stochastic = 1, value_seed ignored) of a CSR matrix
+ * with the entries generated on the current GPU; the host arrays are those of
+ * tbsla_matrix_fill_cqmat / _stochastic (no reference counterpart: the
+ * reference generates on the host, values_generation.cpp:26-62). */
+int tbsla_matrix_fill_cqmat_gpu(tbsla_matrix *m, int n_row, int n_col, int c, double q, unsigned int seed_mult,
+                                int pr, int pc, int NR, int NC, uint64_t value_seed, int stochastic);
 int tbsla_matrix_normalize_columns(tbsla_matrix *m);
 tbsla_matrix *tbsla_matrix_convert(const tbsla_matrix *coo, int format);
 tbsla_matrix *tbsla_matrix_from_arrays(int format, int n_row, int n_col, const double *values, size_t n_values,

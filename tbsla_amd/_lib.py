@@ -70,6 +70,7 @@ SIGNATURES = {
     "tbsla_matrix_destroy": (None, [_vp]),
     "tbsla_matrix_fill_cdiag": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i]),
     "tbsla_matrix_fill_cqmat": (_i, [_vp, _i, _i, _i, _d, _u, _i, _i, _i, _i, C.c_uint64]),
+    "tbsla_matrix_fill_cqmat_gpu": (_i, [_vp, _i, _i, _i, _d, _u, _i, _i, _i, _i, C.c_uint64, _i]),
     "tbsla_matrix_fill_cqmat_stochastic": (_i, [_vp, _i, _i, _i, _d, _u, _i, _i, _i, _i]),
     "tbsla_matrix_normalize_columns": (_i, [_vp]),
     "tbsla_matrix_convert": (_vp, [_vp, _i]),

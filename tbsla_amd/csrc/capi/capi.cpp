@@ -302,6 +302,17 @@ int tbsla_matrix_fill_cqmat(tbsla_matrix *m, int n_row, int n_col, int c, double
   return guarded([&] { m->m->fill_cqmat_values(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed); });
 }
 
+int tbsla_matrix_fill_cqmat_gpu(tbsla_matrix *m, int n_row, int n_col, int c, double q, unsigned int seed_mult,
+                                int pr, int pc, int NR, int NC, uint64_t value_seed, int stochastic) {
+  if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL matrix");
+  if (NR <= 0 || NC <= 0 || pr < 0 || pc < 0 || pr >= NR || pc >= NC || n_row < 0 || n_col < 0 || c < 0 ||
+      (stochastic && c <= 0))
+    return fail(TBSLA_EINVAL, "tbsla: bad block layout");
+  auto *csr = dynamic_cast<cpp::MatrixCSR *>(m->m.get());
+  if (!csr) return fail(TBSLA_EINVAL, "tbsla: GPU generation needs a CSR matrix");
+  return guarded([&] { csr->fill_cqmat_gpu(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed, stochastic != 0); });
+}
+
 int tbsla_matrix_fill_cqmat_stochastic(tbsla_matrix *m, int n_row, int n_col, int c, double q,
                                        unsigned int seed_mult, int pr, int pc, int NR, int NC) {
   if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL matrix");

@@ -6,6 +6,7 @@
 
 #include "../device/products.hpp"
 #include "fill_common.hpp"
+#include "../device/generate.hpp"
 
 namespace tbsla {
 namespace cpp {
@@ -226,6 +227,42 @@ void MatrixCSR::fill_cqmat_values(int n_row, int n_col, int c, double q, unsigne
   for (std::size_t k = 0; k < e.counts.size(); k++) rowptr[k + 1] = rowptr[k] + e.counts[k];
   colidx = std::move(e.cols);
   values = std::move(e.vals);
+}
+
+// fill_cqmat_values / fill_cqmat_stochastic with the block's entries
+// generated on the GPU (device/generate.hip); metadata, nnz and the early
+// returns follow the host path above.
+void MatrixCSR::fill_cqmat_gpu(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc, int NR,
+                               int NC, std::uint64_t value_seed, bool stochastic) {
+  this->n_row = n_row;
+  this->n_col = n_col;
+  this->pr = pr;
+  this->pc = pc;
+  this->NR = NR;
+  this->NC = NC;
+  values.clear();
+  colidx.clear();
+  rowptr.clear();
+  count_stochastic_ = false;
+  invalidate_device();
+  ln_row = static_cast<int>(gen::lnv(n_row, pr, NR));
+  f_row = static_cast<int>(gen::pflv(n_row, pr, NR));
+  ln_col = static_cast<int>(gen::lnv(n_col, pc, NC));
+  f_col = static_cast<int>(gen::pflv(n_col, pc, NC));
+  // fill_cqmat_stochastic fills with value 1.0 (no overlay), then divides
+  host::CqmatBlock b(n_row, n_col, c, q, seed_mult, stochastic ? 0 : value_seed, f_row, ln_row, f_col, ln_col);
+  const long lim3 = b.lim3();
+  const long nrows = std::max<long>(lim3, f_row) - f_row;
+  hip::GeneratedCsr g;
+  hip::generate_cqmat_csr(b, nrows, stochastic, g);
+  const long full_end = std::min(lim3, b.s.min_);
+  const long kept_full = full_end > f_row ? g.rowptr[static_cast<std::size_t>(full_end - f_row)] : 0;
+  nnz = host::overcount_nnz(b, kept_full);
+  if (b.candidates_in_block() == 0) return;
+  rowptr = std::move(g.rowptr);
+  colidx = std::move(g.colidx);
+  values = std::move(g.values);
+  if (stochastic && !values.empty()) count_stochastic_ = true;
 }
 
 // MatrixCSR.cpp:411-421
