@@ -25,6 +25,10 @@ constexpr int kSpmmRowsPerWave = 16;
 constexpr int kSpmmRows = 4 * kSpmmRowsPerWave;  // rows per SpMM work item
 constexpr int kSlots = 16;           // atomic-max slots per RHS
 constexpr int kChunkRows = 2048;     // rows per pass item, single-RHS graphs
+// entries per tile of the single-RHS graphs: smaller than the SpMV tiles, so
+// the persistent step (tile and SpMM items in one kernel) keeps its
+// occupancy
+constexpr int kBatchTileNnz = 2048;
 constexpr int kSpmmChunkRows = 256;  // rows per pass item, SpMM groups
 constexpr int kMaxGroups = 64;
 constexpr int kQueues = 8;           // one per XCD
@@ -309,7 +313,7 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
 template <typename VT, typename XT, int UNR>
 __global__ __launch_bounds__(kTileThreads) void batch_step(BatchDev<VT, XT> d, int par) {
   using AT = typename dev::Val<VT>::A;
-  __shared__ dev::CsrTileLds<AT> lds;
+  __shared__ dev::CsrTileLds<AT, kBatchTileNnz> lds;
   __shared__ double red[kTileThreads];
   const XT *xa_in = par ? d.xa1 : d.xa0, *xb_in = par ? d.xb1 : d.xb0;
   XT *xa_out = par ? d.xa0 : d.xa1, *xb_out = par ? d.xb0 : d.xb1;
@@ -342,9 +346,9 @@ __global__ __launch_bounds__(kTileThreads) void batch_step(BatchDev<VT, XT> d, i
       TileEpi<XT> epi{xa_out, d.row_graph, d.rs, d.a_member, d.slots, d.beta, gA, gB, w % kSlots,
                       -INFINITY, -INFINITY};
       if (tile)
-        dev::csr_tile_rows_at<VT, XT>(d.a, xa_in, lds, epi, w);
+        dev::csr_tile_rows_at<VT, XT, TileEpi<XT>, kBatchTileNnz>(d.a, xa_in, lds, epi, w);
       else
-        dev::csr_long_row_at<VT, XT>(d.a, xa_in, lds, epi, w - d.a_ntiles);
+        dev::csr_long_row_at<VT, XT, TileEpi<XT>, kBatchTileNnz>(d.a, xa_in, lds, epi, w - d.a_ntiles);
       const double bA = dev::block_tree_max<double>(epi.mA, red);
       const double bB = dev::block_tree_max<double>(epi.mB, red);
       if (threadIdx.x == 0) {
@@ -717,12 +721,12 @@ void PageRankBatch::Impl::build() {
       row_graph.upload(rg.data(), rg.size() * sizeof(int));
       row_base.upload(rb.data(), rb.size() * sizeof(int));
       a_member.upload(mem.data(), mem.size());
-      std::vector<int> t = csr_entry_tiles(rp_all.data(), a_rows, kCsrTileNnz);
+      std::vector<int> t = csr_entry_tiles(rp_all.data(), a_rows, kBatchTileNnz);
       a_ntiles = static_cast<int>(t.size()) - 1;
       a_tiles.upload(t.data(), t.size() * sizeof(int));
       std::vector<int> lr;
       for (int i = 0; i < a_rows; i++)
-        if (rp_all[static_cast<std::size_t>(i) + 1] - rp_all[static_cast<std::size_t>(i)] > kCsrLongRow) lr.push_back(i);
+        if (rp_all[static_cast<std::size_t>(i) + 1] - rp_all[static_cast<std::size_t>(i)] > kBatchTileNnz / 2) lr.push_back(i);
       a_nlong = static_cast<int>(lr.size());
       if (a_nlong) a_long.upload(lr.data(), lr.size() * sizeof(int));
       for (int k = 0; k < 2; k++) xa[k].alloc(vb * static_cast<std::size_t>(a_rows));
@@ -825,8 +829,8 @@ void PageRankBatch::Impl::build() {
       TBSLA_HIP_CHECK(hipMemcpy(t.data(), a_tiles.get(), t.size() * sizeof(int), hipMemcpyDeviceToHost));
       TBSLA_HIP_CHECK(hipMemcpy(rg.data(), row_graph.get(), rg.size() * sizeof(int), hipMemcpyDeviceToHost));
       for (int k = 0; k < a_ntiles; k++) {
-        const long e0 = static_cast<long>(k) * kCsrTileNnz;
-        wgt[static_cast<std::size_t>(k)] = static_cast<double>(std::min<long>(kCsrTileNnz, a_nnz - e0)) * (vb + 4 + vb);
+        const long e0 = static_cast<long>(k) * kBatchTileNnz;
+        wgt[static_cast<std::size_t>(k)] = static_cast<double>(std::min<long>(kBatchTileNnz, a_nnz - e0)) * (vb + 4 + vb);
         const int r = std::min(t[static_cast<std::size_t>(k)], a_rows - 1);
         item_graph[static_cast<std::size_t>(k)] = rg[static_cast<std::size_t>(r)];
       }

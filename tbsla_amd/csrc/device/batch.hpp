@@ -8,8 +8,8 @@
 //
 // Device layout:
 //   single-RHS graphs  one block-diagonal CSR (graphs concatenated, columns
-//                      offset), 2048-entry tiles as in the single-graph
-//                      product; iterate vectors concatenated.
+//                      offset), entry tiles as in the single-graph product
+//                      but of 2048 entries; iterate vectors concatenated.
 //   multi-RHS graphs   one CSR per graph with up to 64 RHS; iterates stored
 //                      row-major n x 64 so a gathered row of X is 64
 //                      consecutive values (one coalesced load per wavefront).

@@ -29,7 +29,7 @@ namespace tbsla {
 namespace hip {
 
 constexpr int kTileThreads = 256;
-constexpr int kTilePer = 8;
+constexpr int kTilePer = 12;  // 3072-entry tiles: 2048 and 4096 measured slower on the cqmat block
 constexpr int kCsrTileNnz = kTileThreads * kTilePer;  // entries staged in LDS per tile
 constexpr int kCsrLongRow = kCsrTileNnz / 2;          // longer rows: one workgroup each
 

@@ -1,13 +1,13 @@
 // SpMV kernels for gfx950 (MI355X, CDNA4, 64-wide wavefronts).
 //
-// csr_tile_kernel   one workgroup per fixed 2048-entry tile.  Entries are
-//                   read with 16-byte loads (8 per thread in flight),
+// csr_tile_kernel   one workgroup per fixed 3072-entry tile.  Entries are
+//                   read with 16-byte loads (12 entries per thread),
 //                   multiplied by the gathered x, and staged in LDS; each row
 //                   that starts in the tile is then summed by one thread in
 //                   entry order.  With FP64 this reproduces the reference loop
 //                   MatrixCSR.cpp:133-139 bit for bit (product rounded, then a
 //                   left-to-right sum starting from 0).
-// csr_long_kernel   rows longer than 1024 entries, one workgroup each.
+// csr_long_kernel   rows longer than 1536 entries, one workgroup each.
 // ell_kernel        four rows per thread over the column-major slots, 16-byte
 //                   loads of four rows' indices and values per slot, two slots
 //                   per step; same order as MatrixELL.cpp:105-113 (contraction

@@ -2,14 +2,15 @@
 // include only from .hip translation units).
 //
 // CSR products use fixed entry tiles: tile t holds entries [t*T, (t+1)*T)
-// (T = kCsrTileNnz) and owns the rows whose first entry lies in it
+// (T = kCsrTileNnz, 3072, or the caller's TN) and owns the rows whose first
+// entry lies in it
 // (tile_row[t] .. tile_row[t+1]).  A tile's entries are read with 16-byte
 // loads at an aligned offset that needs no descriptor load, the products
 // v * x[c] are rounded and staged in LDS, and each owned row is summed by one
 // thread from 0, left to right, finishing a row that runs past the tile from
 // global memory in the same order.  That reproduces the sequential FP64 row
-// sum of MatrixCSR.cpp:133-139 exactly.  Rows longer than kCsrLongRow entries
-// are left to csr_long_row (one workgroup per row, fixed-order tree).
+// sum of MatrixCSR.cpp:133-139 exactly.  Rows longer than T / 2 entries are
+// left to csr_long_row (one workgroup per row, fixed-order tree).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -126,9 +127,11 @@ inline CsrView<VT> csr_view(const DeviceMatrix &d) {
 // (their contents are never added).
 constexpr int kSumRun = 16;
 
-template <typename AT>
+// TN: entries per tile (kCsrTileNnz unless a caller builds its own tile
+// table with another size); rows longer than TN / 2 go to csr_long_row.
+template <typename AT, int TN = kCsrTileNnz>
 struct CsrTileLds {
-  AT prod[kCsrTileNnz + kSumRun];
+  AT prod[TN + kSumRun];
 };
 
 // Value source: stored values (VT) or value-free (every product is z[c]).
@@ -180,13 +183,13 @@ __device__ inline AT epi_init(const E &, int, long) {
 // The entries [e0, e0 + ne) of one tile held by the calling thread: groups
 // of four at 4 * (k * kTileThreads + threadIdx.x).  load() issues the index and
 // value loads; products() gathers x and writes the rounded products to
-// prod[0 .. ne).  FULL (ne == kCsrTileNnz, every tile but the matrix's last)
+// prod[0 .. ne).  FULL (ne == TN, every tile but the matrix's last)
 // reads with no per-entry guard; otherwise entries at or past ne are not read
 // (a caller's value array holds exactly nnz values).
-template <bool FULL, typename VT, typename XT>
+template <bool FULL, typename VT, typename XT, int TN>
 struct TileEntries {
   using AT = typename Val<VT>::A;
-  static constexpr int G = kCsrTileNnz / 4 / kTileThreads;  // 4-entry groups per thread
+  static constexpr int G = TN / 4 / kTileThreads;  // 4-entry groups per thread
   int c[G][4];
   AT v[G][4];
 
@@ -237,7 +240,7 @@ struct TileEntries {
 // The bounds of the thread's first owned row, then the index/value loads,
 // gathers and products into prod.  (Issuing the row table first measured
 // faster than deferring it behind the tile loads.)
-template <bool FULL, typename VT, typename XT>
+template <bool FULL, typename VT, typename XT, int TN>
 __device__ inline void csr_tile_stage(const CsrView<VT> &m, const XT *__restrict__ x, typename Val<VT>::A *prod,
                                       long e0, int ne, int rlo, int rhi, int &a, int &b) {
   // loaded with no branch so they stay in flight under the tile loads
@@ -245,29 +248,29 @@ __device__ inline void csr_tile_stage(const CsrView<VT> &m, const XT *__restrict
   // that owns a row)
   a = m.rowptr[min(rlo + static_cast<int>(threadIdx.x), rhi)] - m.rp0;
   b = m.rowptr[min(rlo + static_cast<int>(threadIdx.x) + 1, rhi)] - m.rp0;
-  TileEntries<FULL, VT, XT> t;
+  TileEntries<FULL, VT, XT, TN> t;
   t.load(m, e0, ne);
-  typename Val<VT>::A p[TileEntries<FULL, VT, XT>::G][4];
+  typename Val<VT>::A p[TileEntries<FULL, VT, XT, TN>::G][4];
   t.products(m, x, e0, ne, p);
-  TileEntries<FULL, VT, XT>::store(p, prod, ne);
+  TileEntries<FULL, VT, XT, TN>::store(p, prod, ne);
 }
 
 // Rows of entry tile `tile`; epi(row, sum) once per owned row by the thread
 // that sums it.  The caller synchronises the block before reusing `lds`.
-template <typename VT, typename XT, typename Epi>
+template <typename VT, typename XT, typename Epi, int TN = kCsrTileNnz>
 __device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restrict__ x,
-                                        CsrTileLds<typename Val<VT>::A> &lds, Epi &epi, int tile) {
+                                        CsrTileLds<typename Val<VT>::A, TN> &lds, Epi &epi, int tile) {
 #pragma clang fp contract(off)
   using AT = typename Val<VT>::A;
   const int tid = threadIdx.x;
-  const long e0 = static_cast<long>(tile) * kCsrTileNnz;
-  const int ne = static_cast<int>(min(static_cast<long>(kCsrTileNnz), m.nnz - e0));
+  const long e0 = static_cast<long>(tile) * TN;
+  const int ne = static_cast<int>(min(static_cast<long>(TN), m.nnz - e0));
   const int rlo = m.tile_row[tile], rhi = m.tile_row[tile + 1];
   int a, b;
-  if (ne == kCsrTileNnz)
-    csr_tile_stage<true, VT, XT>(m, x, lds.prod, e0, ne, rlo, rhi, a, b);
+  if (ne == TN)
+    csr_tile_stage<true, VT, XT, TN>(m, x, lds.prod, e0, ne, rlo, rhi, a, b);
   else
-    csr_tile_stage<false, VT, XT>(m, x, lds.prod, e0, ne, rlo, rhi, a, b);
+    csr_tile_stage<false, VT, XT, TN>(m, x, lds.prod, e0, ne, rlo, rhi, a, b);
   __syncthreads();
   const long e1 = e0 + ne;
   for (int r = rlo + tid; r < rhi; r += kTileThreads) {
@@ -275,10 +278,10 @@ __device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restri
       a = m.rowptr[r] - m.rp0;
       b = m.rowptr[r + 1] - m.rp0;
     }
-    if (b - a > kCsrLongRow) continue;  // summed by csr_long_row
+    if (b - a > TN / 2) continue;  // summed by csr_long_row
     AT s = epi_init<AT>(epi, r, 0);
-    // a row owned by the tile starts in it: 0 <= a - e0 < kCsrTileNnz, so the
-    // reads below stay inside prod[kCsrTileNnz + kSumRun]
+    // a row owned by the tile starts in it: 0 <= a - e0 < TN, so the reads
+    // below stay inside prod[TN + kSumRun]
     const int lend = static_cast<int>(min(static_cast<long>(b), e1) - e0);
     for (int e = a - static_cast<int>(e0); e < lend; e += kSumRun) {
       AT v[kSumRun];
@@ -310,15 +313,15 @@ __device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict_
 
 // One long row (m.long_rows[idx]) per workgroup: chunk products, a
 // fixed-order tree per chunk, chunk totals added in order.  epi by thread 0.
-template <typename VT, typename XT, typename Epi>
+template <typename VT, typename XT, typename Epi, int TN = kCsrTileNnz>
 __device__ inline void csr_long_row_at(const CsrView<VT> &m, const XT *__restrict__ x,
-                                       CsrTileLds<typename Val<VT>::A> &lds, Epi &epi, int idx) {
+                                       CsrTileLds<typename Val<VT>::A, TN> &lds, Epi &epi, int idx) {
   using AT = typename Val<VT>::A;
   const int r = m.long_rows[idx];
   const long a = m.rowptr[r] - m.rp0, b = m.rowptr[r + 1] - m.rp0;
   AT total = 0;
-  for (long base = a; base < b; base += kCsrTileNnz) {
-    const long cnt = min(static_cast<long>(kCsrTileNnz), b - base);
+  for (long base = a; base < b; base += TN) {
+    const long cnt = min(static_cast<long>(TN), b - base);
     AT acc = 0;
     for (long e = threadIdx.x; e < cnt; e += kTileThreads) acc += Prod<VT, XT>::one(m, x, base + e);
     total += block_tree_sum<AT>(acc, lds.prod);

@@ -100,14 +100,17 @@ def test_unsorted_coo_is_exact(tb, oracle, gpu):
     assert np.array_equal(p.Ax(r0, x), o.Ax(r0, x))
 
 
+LONG_ROW = 1536  # half a 3072-entry tile: longer rows take the per-row path
+
+
 def test_long_rows(tb, oracle, gpu):
-    """Rows longer than 1024 entries take the one-workgroup-per-row path
-    (fixed-order tree, within 1e-13); rows up to 1024 entries that run past
-    their 2048-entry tile stay bit-exact."""
+    """Rows longer than half a tile (1536 entries) take the
+    one-workgroup-per-row path (fixed-order tree, within 1e-13); shorter rows
+    that run past their 3072-entry tile stay bit-exact."""
     n = 6000
     lens = np.zeros(n, dtype=np.int64)
-    lens[:12] = [5000, 1, 0, 12000, 1024, 1025, 700, 3, 1024, 2047, 2048, 1]
-    lens[12:40] = 37
+    lens[:18] = [5000, 1, 0, 12000, 1024, 1025, 700, 3, 1536, 1537, 2047, 2048, 1535, 3071, 3072, 3073, 1536, 1]
+    lens[18:40] = 37
     rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     nnz = rowptr[-1]
     rng = np.random.default_rng(2)
@@ -118,7 +121,7 @@ def test_long_rows(tb, oracle, gpu):
     o = oracle.from_arrays("CSR", n, n, val, rowptr, col)
     y, yo = p.spmv(x), o.spmv(x)
     assert np.all(np.abs(y - yo) <= 1e-13 * np.abs(yo) + 1e-300)
-    short = lens <= 1024
+    short = lens <= LONG_ROW
     assert np.array_equal(y[short], yo[short])
     rows = np.repeat(np.arange(n), np.diff(rowptr))
     coo = tb.MatrixCOO(n, n, val, rows.astype(np.int32), col)
