@@ -43,6 +43,42 @@ def test_batch_f64_matches_reference_per_set(tb, oracle, gpu):
     assert k == len(its) and rep["iterations"] == its.max()
 
 
+def test_batch_long_rows_and_empty_rows(tb, oracle, gpu):
+    """Rows of 0 to 70 entries (the SpMM reads a row 16 entries at a time;
+    the single-set tiles send rows past half a tile to the per-row path):
+    an SpMM group of 9 sets and a single-set graph, FP64, per set against the
+    reference iteration."""
+    rng = np.random.default_rng(11)
+    graphs = []
+    for n in (3000, 2500):
+        lens = rng.integers(0, 24, n)
+        lens[::37] = rng.integers(17, 71, lens[::37].size)  # rows past one 16-entry chunk
+        lens[5::101] = 0
+        if n == 2500:
+            lens[7] = 1100  # past half a 2048-entry tile
+        rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        col = rng.integers(0, n, rowptr[-1]).astype(np.int32)
+        cnt = np.bincount(col, minlength=n).astype(np.float64)
+        val = 1.0 / cnt[col]
+        graphs.append((n, rowptr, col, val))
+    sets = [[[(53 * r + 11) % 3000] for r in range(9)], [[17]]]
+    b = tb.PageRankBatch("f64")
+    for (n, rowptr, col, val), st in zip(graphs, sets):
+        b.add_graph(tb.MatrixCSR(n, n, val, rowptr, col), st)
+    out, rep, its = b.run(epsilon=1e-10, max_iteration=500, graph=True, autotune=True)
+    out = out.cpu().numpy()
+    off = b.layout()
+    k = 0
+    for (n, rowptr, col, val), st in zip(graphs, sets):
+        o = oracle.from_arrays("CSR", n, n, val, rowptr, col)
+        for nodes in st:
+            pr, it = o.personalized_page_rank(1e-10, 0.85, 500, np.array(nodes, dtype=np.int32))
+            got = out[off[k]:off[k + 1]]
+            assert abs(int(its[k]) - it) <= 1, (n, nodes, its[k], it)
+            assert np.abs(got - pr).sum() <= 1e-13, (n, nodes, np.abs(got - pr).sum())
+            k += 1
+
+
 def test_batch_graph_replay_equals_plain_launches(tb, oracle, gpu):
     """hipGraph replay, plain launches and every autotuned launch shape give
     bit-identical results (the products and reductions are order-fixed)."""
