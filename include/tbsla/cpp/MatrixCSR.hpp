@@ -16,6 +16,9 @@ class MatrixCSR : public virtual Matrix {
   friend std::ostream &operator<<(std::ostream &os, const MatrixCSR &m);
   MatrixCSR(int n_row, int n_col, std::vector<double> &values, std::vector<int> &rowptr, std::vector<int> &colidx);
   MatrixCSR(const tbsla::cpp::MatrixCOO &m);
+  // Extension, no reference counterpart: the same conversion with the
+  // (row, column) sort on the current GPU when `gpu` is true.
+  MatrixCSR(const tbsla::cpp::MatrixCOO &m, bool gpu);
   MatrixCSR() : values(0), rowptr(0), colidx(0) {}
   const std::vector<double> &get_values() const { return values; }
   const std::vector<int> &get_rowptr() const { return rowptr; }
@@ -45,6 +48,7 @@ class MatrixCSR : public virtual Matrix {
   Format format() const override { return FORMAT_CSR; }
 
  protected:
+  void from_coo(const tbsla::cpp::MatrixCOO &m, bool gpu);
   std::vector<double> values;
   std::vector<int> rowptr;
   std::vector<int> colidx;

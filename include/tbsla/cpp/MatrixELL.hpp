@@ -18,6 +18,9 @@ class MatrixELL : public virtual Matrix {
   friend std::ostream &operator<<(std::ostream &os, const MatrixELL &m);
   MatrixELL() : values(0), columns(0), max_col(0) {}
   MatrixELL(const tbsla::cpp::MatrixCOO &m);
+  // Extension, no reference counterpart: the same conversion with the
+  // (row, column) sort on the current GPU when `gpu` is true.
+  MatrixELL(const tbsla::cpp::MatrixCOO &m, bool gpu);
   const std::vector<double> &get_values() const { return values; }
   const std::vector<int> &get_columns() const { return columns; }
   int get_max_col() const { return max_col; }
@@ -41,6 +44,7 @@ class MatrixELL : public virtual Matrix {
   Format format() const override { return FORMAT_ELL; }
 
  protected:
+  void from_coo(const tbsla::cpp::MatrixCOO &m, bool gpu);
   std::vector<double> values;
   std::vector<int> columns;
   int max_col;

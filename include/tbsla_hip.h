@@ -54,6 +54,11 @@ int tbsla_matrix_fill_cqmat_gpu(tbsla_matrix *m, int n_row, int n_col, int c, do
                                 int pr, int pc, int NR, int NC, uint64_t value_seed, int stochastic);
 int tbsla_matrix_normalize_columns(tbsla_matrix *m);
 tbsla_matrix *tbsla_matrix_convert(const tbsla_matrix *coo, int format);
+/* tbsla_matrix_convert to CSR or ELL with the (row, column) sort on the
+ * current GPU: the same arrays (no reference counterpart: the reference sorts
+ * on the host, MatrixCSR.cpp:29-74, MatrixELL.cpp:12-54).  NULL and
+ * TBSLA_EINVAL for another format. */
+tbsla_matrix *tbsla_matrix_convert_gpu(const tbsla_matrix *coo, int format);
 tbsla_matrix *tbsla_matrix_from_arrays(int format, int n_row, int n_col, const double *values, size_t n_values,
                                        const int *a, size_t na, const int *b, size_t nb);
 /* out[16]: format n_row n_col f_row f_col ln_row ln_col pr pc NR NC nnz max_col

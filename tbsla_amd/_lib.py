@@ -74,6 +74,7 @@ SIGNATURES = {
     "tbsla_matrix_fill_cqmat_stochastic": (_i, [_vp, _i, _i, _i, _d, _u, _i, _i, _i, _i]),
     "tbsla_matrix_normalize_columns": (_i, [_vp]),
     "tbsla_matrix_convert": (_vp, [_vp, _i]),
+    "tbsla_matrix_convert_gpu": (_vp, [_vp, _i]),
     "tbsla_matrix_from_arrays": (_vp, [_i, _i, _i, _dp, _sz, _ip, _sz, _ip, _sz]),
     "tbsla_matrix_info": (_i, [_vp, C.POINTER(_l)]),
     "tbsla_matrix_array": (_i, [_vp, _i, C.POINTER(_vp), C.POINTER(_sz)]),

@@ -347,6 +347,27 @@ tbsla_matrix *tbsla_matrix_convert(const tbsla_matrix *coo, int format) {
   return new tbsla_matrix{std::unique_ptr<cpp::Matrix>(out)};
 }
 
+tbsla_matrix *tbsla_matrix_convert_gpu(const tbsla_matrix *coo, int format) {
+  if (!coo || coo->m->format() != cpp::FORMAT_COO) {
+    fail(TBSLA_EINVAL, "tbsla: convert needs a COO matrix");
+    return nullptr;
+  }
+  if (format != TBSLA_CSR && format != TBSLA_ELL) {
+    fail(TBSLA_EINVAL, "tbsla: GPU conversion produces CSR or ELL");
+    return nullptr;
+  }
+  const auto &c = dynamic_cast<const cpp::MatrixCOO &>(*coo->m);
+  cpp::Matrix *out = nullptr;
+  int rc = guarded([&] {
+    if (format == TBSLA_CSR)
+      out = new cpp::MatrixCSR(c, true);
+    else
+      out = new cpp::MatrixELL(c, true);
+  });
+  if (rc) return nullptr;
+  return new tbsla_matrix{std::unique_ptr<cpp::Matrix>(out)};
+}
+
 tbsla_matrix *tbsla_matrix_from_arrays(int format, int n_row, int n_col, const double *values, size_t n_values,
                                        const int *a, size_t na, const int *b, size_t nb) {
   std::vector<double> v(values, values + n_values);

@@ -6,6 +6,7 @@
 
 #include "../device/products.hpp"
 #include "fill_common.hpp"
+#include "../device/convert.hpp"
 #include "../device/generate.hpp"
 
 namespace tbsla {
@@ -33,7 +34,13 @@ MatrixCSR::MatrixCSR(int n_row, int n_col, std::vector<double> &values, std::vec
 // MatrixCSR.cpp:29-74: (row, col) order, rowptr from row counts.  Entries
 // with equal (row, col) keep their COO order (the reference's std::sort
 // leaves their order unspecified).
-MatrixCSR::MatrixCSR(const MatrixCOO &m) {
+MatrixCSR::MatrixCSR(const MatrixCOO &m) { from_coo(m, false); }
+
+MatrixCSR::MatrixCSR(const MatrixCOO &m, bool gpu) { from_coo(m, gpu); }
+
+// MatrixCSR.cpp:29-74; the (row, column) order of the entries is computed on
+// the host or, with `gpu`, on the current GPU (the same order)
+void MatrixCSR::from_coo(const MatrixCOO &m, bool gpu) {
   n_row = m.get_n_row();
   n_col = m.get_n_col();
   ln_row = m.get_n_row();
@@ -48,6 +55,12 @@ MatrixCSR::MatrixCSR(const MatrixCOO &m) {
   const auto &r = m.get_row();
   const auto &c = m.get_col();
   const auto &v = m.get_values();
+  if (gpu) {
+    if (r.size() != v.size() || c.size() != v.size())
+      throw std::invalid_argument("tbsla: GPU conversion needs row, col and values of one length");
+    hip::coo_to_csr_gpu(n_row, r, c, v, rowptr, colidx, values);
+    return;
+  }
   const std::size_t n = v.size();
   rowptr.assign(static_cast<std::size_t>(n_row) + 1, 0);
   if (n == 0) return;

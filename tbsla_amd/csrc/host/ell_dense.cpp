@@ -7,6 +7,7 @@
 
 #include "../device/products.hpp"
 #include "fill_common.hpp"
+#include "../device/convert.hpp"
 
 namespace tbsla {
 namespace cpp {
@@ -14,7 +15,13 @@ namespace cpp {
 // ------------------------------------------------------------------ ELL ----
 
 // MatrixELL.cpp:12-54: (row, col) order, row-major padding to max_col
-MatrixELL::MatrixELL(const MatrixCOO &m) {
+MatrixELL::MatrixELL(const MatrixCOO &m) { from_coo(m, false); }
+
+MatrixELL::MatrixELL(const MatrixCOO &m, bool gpu) { from_coo(m, gpu); }
+
+// MatrixELL.cpp:12-54; the (row, column) order of the entries is computed on
+// the host or, with `gpu`, on the current GPU (the same order)
+void MatrixELL::from_coo(const MatrixCOO &m, bool gpu) {
   n_row = m.get_n_row();
   n_col = m.get_n_col();
   ln_row = m.get_n_row();
@@ -32,6 +39,14 @@ MatrixELL::MatrixELL(const MatrixCOO &m) {
   if (v.empty()) {
     max_col = 0;
     nnz = 0;
+    return;
+  }
+  if (gpu) {
+    if (r.size() != v.size() || c.size() != v.size())
+      throw std::invalid_argument("tbsla: GPU conversion needs row, col and values of one length");
+    int K = 0;
+    hip::coo_to_ell_gpu(n_row, r, c, v, K, columns, values);
+    max_col = K;
     return;
   }
   std::vector<int> p = host::row_col_order(n_row, r, c);

@@ -325,10 +325,11 @@ class MatrixSCOO(Matrix):
 class MatrixCSR(Matrix):
     FORMAT, NAME, _IDX0, _IDX1 = _lib.CSR, "CSR", "rowptr", "colidx"
 
-    def __init__(self, src=None, n_col=None, values=None, rowptr=None, colidx=None, _handle=None):
-        """MatrixCSR(), MatrixCSR(coo) or MatrixCSR(n_row, n_col, values, rowptr, colidx)."""
+    def __init__(self, src=None, n_col=None, values=None, rowptr=None, colidx=None, _handle=None, device=False):
+        """MatrixCSR(), MatrixCSR(coo) or MatrixCSR(n_row, n_col, values, rowptr, colidx);
+        MatrixCSR(coo, device=True) sorts the entries on the GPU (same arrays)."""
         if _handle is None and isinstance(src, MatrixCOO):
-            _handle = _convert(src, self.FORMAT)
+            _handle = _convert(src, self.FORMAT, device)
         elif _handle is None and values is not None:
             _handle = _from_arrays(self.FORMAT, src, n_col, values, rowptr, colidx)
         super().__init__(_handle)
@@ -337,9 +338,10 @@ class MatrixCSR(Matrix):
 class MatrixELL(Matrix):
     FORMAT, NAME, _IDX0, _IDX1 = _lib.ELL, "ELL", None, "columns"
 
-    def __init__(self, src=None, _handle=None):
+    def __init__(self, src=None, _handle=None, device=False):
+        """MatrixELL() or MatrixELL(coo); device=True sorts the entries on the GPU (same arrays)."""
         if _handle is None and isinstance(src, MatrixCOO):
-            _handle = _convert(src, self.FORMAT)
+            _handle = _convert(src, self.FORMAT, device)
         super().__init__(_handle)
 
     def get_max_col(self):
@@ -358,8 +360,8 @@ class MatrixDENSE(Matrix):
 FORMATS = {"COO": MatrixCOO, "SCOO": MatrixSCOO, "CSR": MatrixCSR, "ELL": MatrixELL, "DENSE": MatrixDENSE}
 
 
-def _convert(coo: MatrixCOO, fmt: int):
-    h = lib.tbsla_matrix_convert(coo._h, fmt)
+def _convert(coo: MatrixCOO, fmt: int, device: bool = False):
+    h = (lib.tbsla_matrix_convert_gpu if device else lib.tbsla_matrix_convert)(coo._h, fmt)
     if not h:
         check(-1)
     return h

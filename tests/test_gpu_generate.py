@@ -1,4 +1,5 @@
-"""GPU generation of cqmat CSR blocks (tbsla_matrix_fill_cqmat_gpu).
+"""GPU generation of cqmat CSR blocks (tbsla_matrix_fill_cqmat_gpu) and GPU
+COO -> CSR / ELL conversion (tbsla_matrix_convert_gpu).
 
 The device restates the reference's entry stream (values_generation.cpp:26-45)
 with its own restatement of glibc's rand_r.  Every array must equal the host
@@ -86,3 +87,71 @@ def test_gpu_generation_matches_reference_digests(tb, digests, gpu):
             assert hashlib.sha256(a.tobytes()).hexdigest() == ref[f"sha256_{k}"], (key, k)
         print(f"{key}: generated on the GPU in {dt:.2f} s")
         del m
+
+
+# ---------------------------------------------------------------- conversions
+
+def _digest(m, fmt):
+    import json
+    info, arrs = m.info(), m.arrays()
+    h = hashlib.sha256()
+    h.update(json.dumps([info[k] for k in INFO_KEYS] + [info["max_col"] if fmt == "ELL" else -1]).encode())
+    for k in sorted(arrs):
+        h.update(k.encode())
+        h.update(np.ascontiguousarray(arrs[k]).tobytes())
+    return h.hexdigest()
+
+
+def test_gpu_conversion_matches_reference_digests(tb, digests, gpu):
+    """COO -> CSR / ELL with the (row, column) sort on the GPU hashes to the
+    digests of the compiled reference's own conversions (MatrixCSR.cpp:29-74,
+    MatrixELL.cpp:12-54) over its test grid."""
+    d = digests["conversions"]
+    for (nr, nc, c) in [(10, 10, 3), (30, 30, 8), (20, 30, 4), (30, 20, 6), (50, 50, 5)]:
+        for s in range(4):
+            for q in [0.0, 0.1, 0.5, 1.0]:
+                coo = tb.MatrixCOO().fill_cqmat(nr, nc, c, q, s)
+                for fmt, cls in (("CSR", tb.MatrixCSR), ("ELL", tb.MatrixELL)):
+                    assert _digest(cls(coo, device=True), fmt) == d[f"convert/{nr}x{nc}c{c}s{s}q{q}/{fmt}"], (nr, nc, c, s, q, fmt)
+
+
+def test_gpu_conversion_equals_host(tb, gpu):
+    """Unsorted COO with duplicate (row, column) pairs and distinct values:
+    ties keep their input order on both paths; empty rows; block COO slices."""
+    from tbsla_amd import _lib
+    rng = np.random.default_rng(4)
+    n = 5000
+    nnz = 60000
+    row = rng.integers(0, n, nnz).astype(np.int32)
+    col = rng.integers(0, n // 3, nnz).astype(np.int32)  # many duplicates
+    row[:500] = 7  # one long row
+    val = rng.random(nnz)
+    coo = tb.MatrixCOO(n, n, val, row, col)
+    for cls in (tb.MatrixCSR, tb.MatrixELL):
+        same(cls(coo), cls(coo, device=True))
+    for blk in [(0, 0, 2, 2), (1, 3, 2, 4)]:
+        coo = tb.MatrixCOO().fill_cqmat(4000, 4000, 9, 0.3, 5, *blk, value_seed=3)
+        for cls in (tb.MatrixCSR, tb.MatrixELL):
+            same(cls(coo), cls(coo, device=True))
+    empty = tb.MatrixCOO(10, 10, np.zeros(0), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    for cls in (tb.MatrixCSR, tb.MatrixELL):
+        same(cls(empty), cls(empty, device=True))
+    bad = tb.MatrixCOO(5, 5, np.ones(2), np.array([0, 5], np.int32), np.array([0, 1], np.int32))
+    with pytest.raises(_lib.TbslaError):  # row 5 outside [0, 5)
+        tb.MatrixCSR(bad, device=True)
+
+
+@pytest.mark.slow
+def test_gpu_conversion_full_size(tb, gpu):
+    """The COO slice of the benchmark matrix (16.8 M entries) to CSR and ELL."""
+    coo = tb.MatrixCOO().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4, value_seed=42)
+    for cls in (tb.MatrixCSR, tb.MatrixELL):
+        t = time.time()
+        h = cls(coo)
+        th = time.time() - t
+        t = time.time()
+        g = cls(coo, device=True)
+        tg = time.time() - t
+        same(h, g)
+        print(f"{cls.__name__} from a 16.8 M-entry COO: host {th:.2f} s, GPU {tg:.2f} s")
+        del h, g
