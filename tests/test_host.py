@@ -219,6 +219,16 @@ def test_c_abi_errors_without_device_calls(tb):
     m = tb.MatrixCSR()
     with pytest.raises(_lib.TbslaError):
         m.fill_cqmat(10, 10, 3, 0.1, 1, 2, 0, 2, 2)  # pr >= NR
+    # the GPU generation and conversion entry points validate before any device call
+    with pytest.raises(_lib.TbslaError):
+        m.fill_cqmat(10, 10, 3, 0.1, 1, 2, 0, 2, 2, device=True)  # pr >= NR
+    with pytest.raises(_lib.TbslaError):
+        tb.MatrixELL().fill_cqmat(10, 10, 3, 0.1, 1, device=True)  # CSR only
+    assert _lib.lib.tbsla_matrix_convert_gpu(m._h, 2) is None  # needs a COO
+    assert b"needs a COO" in _lib.lib.tbsla_last_error()
+    coo = tb.MatrixCOO().fill_cqmat(10, 10, 3, 0.1, 1)
+    assert _lib.lib.tbsla_matrix_convert_gpu(coo._h, 4) is None  # DENSE: not a GPU conversion
+    assert b"CSR or ELL" in _lib.lib.tbsla_last_error()
     v = _lib.lib.C_CPP_Vector_create()
     _lib.lib.C_CPP_Vector_fill(v, 5, 3)
     assert not _lib.lib.C_CPP_Vector_add(v, None)
