@@ -124,6 +124,37 @@ def test_blocked_products_with_teleport_vector_and_bins(tb, oracle, gpu):
         assert float((a - b).abs().sum()) <= 1e-13, prec
 
 
+def test_blocked_products_small_and_odd_sizes(tb, gpu):
+    """Propagation-blocked products where one bin holds everything (no far
+    entries), where a bin has fewer than four local entries, and on a
+    hand-made graph with empty rows and columns and a long row: equal within
+    rounding to the fixed-order tile products, in every value precision."""
+    import torch
+    rng = np.random.default_rng(8)
+    mats = [tb.MatrixCSR().fill_cqmat_stochastic(3000, 3000, 5, 0.3, 4),
+            tb.MatrixCSR().fill_cqmat_stochastic(8195, 8195, 3, 0.9, 6)]
+    n = 9000
+    lens = rng.integers(0, 9, n)
+    lens[::50] = 0
+    lens[4321] = 3000
+    rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, n, rowptr[-1]).astype(np.int32)
+    col[col % 97 == 0] = 1  # some columns never appear
+    cnt = np.bincount(col, minlength=n).astype(np.float64)
+    mats.append(tb.MatrixCSR(n, n, 1.0 / cnt[col], rowptr, col))
+    for p in mats:
+        m = p.get_n_col()
+        a = torch.zeros(m, dtype=torch.float64, device=gpu)
+        b = torch.zeros_like(a)
+        for prec in ("f64", "f32", "bf16"):
+            ra = p.page_rank_device(a, epsilon=1e-12, max_iteration=400, rule="sumnorm", precision=prec,
+                                    gather="tiles")
+            rb = p.page_rank_device(b, epsilon=1e-12, max_iteration=400, rule="sumnorm", precision=prec,
+                                    gather="blocked")
+            assert abs(ra["iterations"] - rb["iterations"]) <= 2, (m, prec)
+            assert float((a - b).abs().sum()) <= 1e-11, (m, prec, float((a - b).abs().sum()))
+
+
 def test_page_rank_zero_iterations(tb, gpu):
     m = tb.MatrixCSR().fill_cqmat_stochastic(100, 100, 4, 0.5, 2)
     pr, its = m.page_rank(1e-10, 0.85, 0)
