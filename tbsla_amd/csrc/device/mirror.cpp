@@ -129,20 +129,23 @@ void build_ell(DeviceMatrix &d, const cpp::MatrixELL &m, Dtype vdt) {
   d.K = K;
   d.entries = rows * K;
   d.f_col = m.get_f_col();
-  d.ld = (rows + 63) / 64 * 64;
-  std::vector<int> ccm(static_cast<std::size_t>(d.ld * K), d.f_col);
-  std::vector<double> vcm(static_cast<std::size_t>(d.ld * K), 0.0);
+  const long slots = (rows + kEllSlice - 1) / kEllSlice * kEllSlice * K;
+  std::vector<int> ccm(static_cast<std::size_t>(slots), 0);
+  std::vector<double> vcm(static_cast<std::size_t>(slots), 0.0);
   long hi = 0;
   std::mutex mu;
   gen::parallel_for(rows, [&](long b, long e) {
     long h = 0;
-    for (long i = b; i < e; i++)
+    for (long i = b; i < e; i++) {
+      const long o = (i / kEllSlice) * K * kEllSlice + i % kEllSlice;
       for (long j = 0; j < K; j++) {
-        const int c = cols[static_cast<std::size_t>(i * K + j)];
-        ccm[static_cast<std::size_t>(j * d.ld + i)] = c;
-        vcm[static_cast<std::size_t>(j * d.ld + i)] = vals[static_cast<std::size_t>(i * K + j)];
-        h = std::max<long>(h, std::max<long>(static_cast<long>(c) - d.f_col, 0));
+        // MatrixELL.cpp:110 reads x[max(col - f_col, 0)]
+        const long c = std::max<long>(static_cast<long>(cols[static_cast<std::size_t>(i * K + j)]) - d.f_col, 0);
+        ccm[static_cast<std::size_t>(o + j * kEllSlice)] = static_cast<int>(c);
+        vcm[static_cast<std::size_t>(o + j * kEllSlice)] = vals[static_cast<std::size_t>(i * K + j)];
+        h = std::max(h, c);
       }
+    }
     std::lock_guard<std::mutex> g(mu);
     hi = std::max(hi, h);
   });

@@ -4,7 +4,12 @@
 //         entry-tile table: tile t = entries [t*kCsrTileNnz, (t+1)*kCsrTileNnz)
 //         owns rows [tile_row[t], tile_row[t+1]) (the rows whose first entry
 //         lies in the tile), and the list of rows longer than kCsrLongRow.
-//   ELL   column-major (slot j of row i at j*ld + i), ld = rows rounded up to 64.
+//   ELL   sliced column-major: slice b = rows [b*kEllSlice, (b+1)*kEllSlice)
+//         holds its K slots contiguously, slot j of row i at
+//         ((i / kEllSlice) * K + j) * kEllSlice + i % kEllSlice, so one
+//         workgroup streams one contiguous block.  Columns are stored local
+//         and clamped, max(col - f_col, 0) (MatrixELL.cpp:110); the slice's
+//         rows past the last hold column 0, value 0.
 //   COO / SCOO
 //         entries in a row-sorted view (a stable counting sort by row when the
 //         input is not already sorted), compressed to row segments: segment s
@@ -32,6 +37,7 @@ constexpr int kTileThreads = 256;
 constexpr int kTilePer = 12;  // 3072-entry tiles: 2048 and 4096 measured slower on the cqmat block
 constexpr int kCsrTileNnz = kTileThreads * kTilePer;  // entries staged in LDS per tile
 constexpr int kCsrLongRow = kCsrTileNnz / 2;          // longer rows: one workgroup each
+constexpr int kEllSlice = 128;  // ELL rows per slice (one row per thread, one workgroup per slice)
 
 struct DeviceMatrix {
   cpp::Format format = cpp::FORMAT_CSR;
@@ -45,9 +51,8 @@ struct DeviceMatrix {
   // CSR
   DevBuf rowptr, colidx, vals, tiles, long_rows;
   int ntiles = 0, nlong = 0, rp0 = 0;
-  // ELL (column-major)
+  // ELL (sliced column-major)
   int K = 0;
-  long ld = 0;
   DevBuf ecols, evals;
   // COO / SCOO: segments of the row-sorted view (rowptr, colidx, vals, tiles
   // and long_rows above hold the segment CSR)

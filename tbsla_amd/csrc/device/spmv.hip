@@ -8,10 +8,10 @@
 //                   MatrixCSR.cpp:133-139 bit for bit (product rounded, then a
 //                   left-to-right sum starting from 0).
 // csr_long_kernel   rows longer than 1536 entries, one workgroup each.
-// ell_kernel        four rows per thread over the column-major slots, 16-byte
-//                   loads of four rows' indices and values per slot, two slots
-//                   per step; same order as MatrixELL.cpp:105-113 (contraction
-//                   off, so bit-identical in FP64).
+// ell_kernel        one row per thread over the sliced column-major slots (one
+//                   contiguous block per workgroup), eight slots per step;
+//                   same order as MatrixELL.cpp:105-113 (contraction off, so
+//                   bit-identical in FP64).
 // coo_tile_kernel   COO/SCOO: the CSR entry-tile kernel on the row segments of
 //                   the row-sorted view, each segment's sum starting from
 //                   y[row], so each row is accumulated in entry order
@@ -82,74 +82,74 @@ __global__ __launch_bounds__(kTileThreads) void csr_long_kernel(dev::CsrView<VT>
   dev::csr_long_row<VT, XT>(m, x, lds, epi);
 }
 
-// Four consecutive rows per thread: per slot one 16-byte load of the four
-// column indices and 16 bytes (F32) or 32 bytes (F64) of values, then four
-// gathers; each row still sums its slots in order 0..K-1.  Slots go in pairs,
-// and the next pair's loads are issued before this pair's gathers (two
-// register sets, alternating), so each step waits on one round trip.
-template <typename VT>
-struct EllPair {
-  using AT = typename Val<VT>::A;
-  int c[2][4];
-  AT v[2][4];
+// ELL: one row per thread, one workgroup per slice of kEllSlice rows, whose
+// K slots lie in one contiguous block (slot j of the slice's row t at
+// j * kEllSlice + t), so a workgroup streams one block of HBM and every slot
+// is one coalesced load per wavefront.  Slots go in groups of kEllGroup; the
+// next group's loads are issued before this group's gathers (two register
+// sets, alternating), so each group waits on one round trip.  Each row still
+// sums its slots in order 0..K-1.
+constexpr int kEllGroup = 8;
 
-  template <int S>
-  __device__ void load(const int *__restrict__ cols, const VT *__restrict__ vals, long ld, long i0, int j, int f_col) {
+__device__ inline double ell_value(const double *p, long i) { return __builtin_nontemporal_load(p + i); }
+__device__ inline float ell_value(const float *p, long i) { return __builtin_nontemporal_load(p + i); }
+__device__ inline float ell_value(const bf16 *p, long i) {
+  return Val<bf16>::cvt(__builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p) + i));
+}
+
+template <typename VT>
+struct EllGroup {
+  using AT = typename Val<VT>::A;
+  int c[kEllGroup];
+  AT v[kEllGroup];
+
+  // slots o, o + kEllSlice, ... (N of them)
+  template <int N>
+  __device__ void load(const int *__restrict__ cols, const VT *__restrict__ vals, long o) {
 #pragma unroll
-    for (int u = 0; u < S; u++) {
-      const long o = (j + u) * ld + i0;
-      const dev::v4i ci = __builtin_nontemporal_load(reinterpret_cast<const dev::v4i *>(cols + o));
-      // MatrixELL.cpp:110: x[max(col - f_col, 0)]
-      c[u][0] = max(ci.x - f_col, 0);
-      c[u][1] = max(ci.y - f_col, 0);
-      c[u][2] = max(ci.z - f_col, 0);
-      c[u][3] = max(ci.w - f_col, 0);
-      Val<VT>::get4(vals, o, v[u]);
+    for (int u = 0; u < N; u++) {
+      c[u] = nt_load(cols, o + u * kEllSlice);
+      v[u] = ell_value(vals, o + u * kEllSlice);
     }
   }
-  template <int S, typename XT>
-  __device__ void accumulate(const XT *__restrict__ x, AT (&acc)[4]) const {
+  template <int N, typename XT>
+  __device__ void accumulate(const XT *__restrict__ x, AT &acc) const {
 #pragma clang fp contract(off)
-    AT xv[S][4];
+    AT xv[N];
 #pragma unroll
-    for (int u = 0; u < S; u++)
+    for (int u = 0; u < N; u++) xv[u] = static_cast<AT>(x[c[u]]);
 #pragma unroll
-      for (int r = 0; r < 4; r++) xv[u][r] = static_cast<AT>(x[c[u][r]]);
-#pragma unroll
-    for (int u = 0; u < S; u++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) acc[r] += v[u][r] * xv[u][r];
+    for (int u = 0; u < N; u++) acc += v[u] * xv[u];
   }
 };
 
 template <typename VT, typename XT, typename YT>
-__global__ __launch_bounds__(256) void ell_kernel(const int *__restrict__ cols, const VT *__restrict__ vals,
-                                                  const XT *__restrict__ x, YT *__restrict__ y, int rows, int K,
-                                                  long ld, int f_col) {
+__global__ __launch_bounds__(kEllSlice) void ell_kernel(const int *__restrict__ cols, const VT *__restrict__ vals,
+                                                        const XT *__restrict__ x, YT *__restrict__ y, int rows,
+                                                        int K) {
   using AT = typename Val<VT>::A;
-  const long i0 = (static_cast<long>(blockIdx.x) * 256 + threadIdx.x) * 4;
-  if (i0 >= rows) return;
-  AT acc[4] = {0, 0, 0, 0};
-  EllPair<VT> A, B;
-  const int K2 = K & ~1;  // slots in whole pairs
-  if (K2 > 0) A.template load<2>(cols, vals, ld, i0, 0, f_col);
+  const long i = static_cast<long>(blockIdx.x) * kEllSlice + threadIdx.x;
+  if (i >= rows) return;
+  const long base = static_cast<long>(blockIdx.x) * K * kEllSlice + threadIdx.x;
+  const int KG = K - K % kEllGroup;  // slots in whole groups
+  AT acc = 0;
+  EllGroup<VT> A, B;
   int j = 0;
-  while (j < K2) {
-    if (j + 2 < K2) B.template load<2>(cols, vals, ld, i0, j + 2, f_col);
-    A.template accumulate<2>(x, acc);
-    j += 2;
-    if (j >= K2) break;
-    if (j + 2 < K2) A.template load<2>(cols, vals, ld, i0, j + 2, f_col);
-    B.template accumulate<2>(x, acc);
-    j += 2;
+  if (KG > 0) A.template load<kEllGroup>(cols, vals, base);
+  while (j < KG) {
+    if (j + kEllGroup < KG) B.template load<kEllGroup>(cols, vals, base + static_cast<long>(j + kEllGroup) * kEllSlice);
+    A.template accumulate<kEllGroup>(x, acc);
+    j += kEllGroup;
+    if (j >= KG) break;
+    if (j + kEllGroup < KG) A.template load<kEllGroup>(cols, vals, base + static_cast<long>(j + kEllGroup) * kEllSlice);
+    B.template accumulate<kEllGroup>(x, acc);
+    j += kEllGroup;
   }
-  if (j < K) {  // odd K: the last slot
-    A.template load<1>(cols, vals, ld, i0, j, f_col);
+  for (; j < K; j++) {  // the last K % kEllGroup slots
+    A.template load<1>(cols, vals, base + static_cast<long>(j) * kEllSlice);
     A.template accumulate<1>(x, acc);
   }
-#pragma unroll
-  for (int r = 0; r < 4; r++)
-    if (i0 + r < rows) y[i0 + r] = static_cast<YT>(acc[r]);
+  y[i] = static_cast<YT>(acc);
 }
 
 // COO/SCOO segment epilogue: the segment's sum started from y[row], so the
@@ -232,8 +232,8 @@ void run_typed(const DeviceMatrix &d, const XT *x, YT *y, int vect_incr, hipStre
     }
     case cpp::FORMAT_ELL:
       if (d.rows > 0)
-        hipLaunchKernelGGL((ell_kernel<VT, XT, YT>), dim3(blocks_for((d.rows + 3) / 4, 256)), dim3(256), 0, s,
-                           d.ecols.as<int>(), d.evals.as<VT>(), x, y, d.rows, d.K, d.ld, d.f_col);
+        hipLaunchKernelGGL((ell_kernel<VT, XT, YT>), dim3(blocks_for(d.rows, kEllSlice)), dim3(kEllSlice), 0, s,
+                           d.ecols.as<int>(), d.evals.as<VT>(), x, y, d.rows, d.K);
       break;
     case cpp::FORMAT_COO:
     case cpp::FORMAT_SCOO: {

@@ -93,7 +93,7 @@ std::vector<double> MatrixELL::spmv(const std::vector<double> &v, int vect_incr)
   return r;
 }
 
-// MatrixELL.cpp:101-114 on the GPU (ell_kernel, column-major mirror)
+// MatrixELL.cpp:101-114 on the GPU (ell_kernel, sliced column-major mirror)
 void MatrixELL::Ax(std::vector<double> &r, const std::vector<double> &v, int vect_incr) const {
   hip::Ax_host(*this, r, v, vect_incr);
 }

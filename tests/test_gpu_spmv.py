@@ -156,6 +156,32 @@ def test_device_vectors_and_precisions(tb, oracle, gpu):
     assert (np.abs(y32.cpu().numpy() - yo) / np.abs(yo).max()).max() < 2e-6
 
 
+def test_ell_slot_counts_and_partial_slices(tb, gpu):
+    """ELL rows of every length up to K for K around the kernel's group of
+    eight slots, on row counts around its 128-row slices; random values and
+    x.  Expected: each row's slots summed left to right from 0 with the
+    product rounded first (MatrixELL.cpp:105-113), evaluated slot by slot in
+    numpy over all rows, padding slots included."""
+    rng = np.random.default_rng(11)
+    for n in (1, 127, 128, 129, 1000):
+        for K in (1, 7, 8, 9, 16, 17, 23, 24, 25, 40):
+            lens = rng.integers(0, K + 1, n)
+            lens[rng.integers(0, n)] = K
+            row = np.repeat(np.arange(n, dtype=np.int32), lens)
+            col = rng.integers(0, n, row.size).astype(np.int32)
+            val = rng.standard_normal(row.size)
+            ell = tb.MatrixELL(tb.MatrixCOO(n, n, val, row, col))
+            arr = ell.arrays()
+            assert ell.info()["max_col"] == K
+            v = arr["values"].reshape(n, K)
+            c = arr["columns"].reshape(n, K)
+            x = rng.standard_normal(n)
+            expect = np.zeros(n)
+            for j in range(K):
+                expect = expect + v[:, j] * x[np.maximum(c[:, j], 0)]
+            assert np.array_equal(ell.spmv(x), expect), (n, K)
+
+
 def test_short_vectors_are_rejected(tb, gpu):
     """A vector shorter than the matrix's indices is an error, not a fault."""
     from tbsla_amd._lib import TbslaError
