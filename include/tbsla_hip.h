@@ -98,11 +98,18 @@ int tbsla_hip_Ax(tbsla_matrix *m, int value_dtype, const void *x, size_t nx, int
 /* y[0:n) = value (a kernel, safe inside stream capture) */
 int tbsla_hip_fill(void *y, int y_dtype, size_t n, double value, void *stream);
 
+/* PageRank precision beside the tbsla_dtype values: the mixed FP32 scheme of
+ * TBSLA_F32 (FP32 products on a delta to an FP64 anchor) without a value
+ * array, for matrices from fill_cqmat_stochastic (value(i,j) = 1/count_j):
+ * the products sum zd = fl32(colscale .* delta). */
+#define TBSLA_PR_PATTERN_F32 4
+
 typedef struct {
   double epsilon;
   double beta;
   int max_iteration;
-  int precision;     /* tbsla_dtype: F64, F32, BF16 (values), PATTERN (value-free) */
+  int precision;     /* tbsla_dtype: F64, F32, BF16 (values), PATTERN (value-free), or
+                        TBSLA_PR_PATTERN_F32 (value-free FP32 deltas on an FP64 anchor) */
   int rule;          /* 0: reference residual < epsilon; 1: sum-normalised bound */
   int check_every;   /* iterations between convergence reads */
   double anchor_tau; /* 0: default */

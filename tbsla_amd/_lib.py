@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG, "libtbsla_hip.so")
 
 COO, SCOO, CSR, ELL, DENSE = range(5)
 F64, F32, BF16, PATTERN = range(4)
+PR_PATTERN_F32 = 4  # PageRank only: value-free FP32 deltas on an FP64 anchor
 
 
 class TbslaError(RuntimeError):

@@ -607,7 +607,8 @@ int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, cons
     q.personalized = p->nodes != nullptr && p->n_nodes > 0;
     q.nodes = p->nodes;
     q.n_nodes = p->n_nodes;
-    if (p->precision < 0 || p->precision > 3) throw std::invalid_argument("tbsla: bad precision");
+    if (p->precision < TBSLA_F64 || p->precision > TBSLA_PR_PATTERN_F32)
+      throw std::invalid_argument("tbsla: bad precision");
     q.precision = static_cast<hip::PrPrecision>(p->precision);
     if (p->rule < 0 || p->rule > 1) throw std::invalid_argument("tbsla: bad stopping rule");
     q.rule = static_cast<hip::PrRule>(p->rule);

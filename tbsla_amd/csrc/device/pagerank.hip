@@ -205,28 +205,38 @@ __device__ inline void finalize_step(PrState *st, double err, double S) {
 // MODE 0: FP64, b normalised in place, previous iterate `prev`.
 // MODE 1: mixed, previous iterate xhat + dold, writes dnew = b - xhat (FP32).
 // MODE 2: FP64 + z = colscale .* b for the next value-free product.
+// MODE 3: MODE 1 + zd = colscale .* (b - xhat) in FP32 (colscale32) for the
+//         next value-free FP32 product.
 template <int MODE>
 __global__ __launch_bounds__(kTileThreads) void pr_normalize(double *__restrict__ b, const double *__restrict__ prev,
                                                              const double *__restrict__ xhat,
                                                              const float *__restrict__ dold, float *__restrict__ dnew,
                                                              double *__restrict__ z,
-                                                             const double *__restrict__ colscale, long n, PrState *st,
+                                                             const double *__restrict__ colscale,
+                                                             float *__restrict__ zd,
+                                                             const float *__restrict__ colscale32, long n, PrState *st,
                                                              double *__restrict__ perr, double *__restrict__ ps,
                                                              double *__restrict__ pc) {
 #pragma clang fp contract(off)
   if (st->done) return;
   __shared__ double red_s[kTileThreads], red_c[kTileThreads];
   const double m = st->maxv;
+  const bool moved = st->need_anchor != 0;
   double err = 0;
   CSum cs{0, 0};
   for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * kTileThreads) {
     const double bi = b[i] / m;
-    const double pi = MODE == 1 ? xhat[i] + static_cast<double>(dold[i]) : prev[i];
+    const bool mixed = MODE == 1 || MODE == 3;
+    // an iteration that moved the anchor starts from xhat itself (its delta
+    // was folded into xhat and is not read)
+    const double pi = mixed ? (moved ? xhat[i] : xhat[i] + static_cast<double>(dold[i])) : prev[i];
     err += fabs(bi - pi);
     csum_add(cs, bi);
-    if (MODE == 1) {
-      dnew[i] = static_cast<float>(bi - xhat[i]);
+    if (mixed) {
+      const double dd = bi - xhat[i];
+      dnew[i] = static_cast<float>(dd);
+      if (MODE == 3) zd[i] = static_cast<float>(static_cast<double>(colscale32[i]) * dd);
     } else {
       b[i] = bi;
       if (MODE == 2) z[i] = colscale[i] * bi;
@@ -274,14 +284,20 @@ __global__ __launch_bounds__(kTileThreads) void pr_finalize_norm(const double *_
 }
 
 // Mixed precision: move the anchor to the current iterate when requested.
-__global__ __launch_bounds__(256) void pr_anchor_move(double *__restrict__ xhat, float *__restrict__ delta,
+// xhat += delta (and z = colscale .* xhat for the value-free anchor
+// product).  With propagation-blocked products the iteration that moves the
+// anchor skips the delta product and its normalise pass starts from xhat, so
+// nothing else changes.  The tile products still add the delta product in
+// that iteration, so there its input `clear` (the delta, or the scaled delta
+// of PATTERN_F32) is zeroed.
+__global__ __launch_bounds__(256) void pr_anchor_move(double *__restrict__ xhat, const float *delta, float *clear,
                                                       double *__restrict__ z, const double *__restrict__ colscale,
                                                       long n, const PrState *st) {
   if (st->done || !st->need_anchor) return;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) {
     const double v = xhat[i] + static_cast<double>(delta[i]);
     xhat[i] = v;
-    delta[i] = 0.0f;
+    if (clear) clear[i] = 0.0f;
     if (z) z[i] = colscale[i] * v;
   }
 }
@@ -414,7 +430,10 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__res
                                                   const XT *__restrict__ x, AT *__restrict__ prod, int nsb, int n,
                                                   const PrState *st, int anchor) {
 #pragma clang fp contract(off)
-  if (st->done || (anchor && !st->need_anchor)) return;
+  // the anchor product runs only when the anchor moves; the delta product
+  // of that iteration is skipped (the delta is zero and the anchor gather
+  // writes the update)
+  if (st->done || (anchor ? !st->need_anchor : st->need_anchor)) return;
   // superblocks of one XCD (blockIdx % 8) are contiguous, so the runs that
   // share a cache line at a run boundary are written through the same L2
   const int per = (gridDim.x + 7) / 8;
@@ -463,14 +482,17 @@ struct PbView {
 };
 
 // MODE 0: PageRank update epilogue (b = beta*(yhat + s) + t, block max);
-// MODE 1: the anchor product, yhat = s.
+//         with MIXED it is skipped in an iteration that moves the anchor.
+// MODE 2: the anchor product with the update of an iteration that moves the
+//         anchor (its delta is zero): yhat_out = s, b = beta*s + t, block max.
 template <typename VT, typename XT, typename AT, bool MIXED, int TM, int MODE>
 __global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict__ lval, const XT *__restrict__ x,
                                                   const AT *__restrict__ prod, const double *__restrict__ yhat,
                                                   Teleport tp, double *__restrict__ out, PrState *st,
-                                                  unsigned long long *slots) {
+                                                  unsigned long long *slots, double *__restrict__ yhat_out) {
 #pragma clang fp contract(off)
-  if (MODE == 0 ? st->done : (st->done || !st->need_anchor)) return;
+  static_assert(MODE == 0 || (MODE == 2 && !MIXED), "pb_gather modes: 0, and 2 without MIXED");
+  if (MODE == 0 ? (st->done || (MIXED && st->need_anchor)) : (st->done || !st->need_anchor)) return;
   __shared__ double ybin[kPbRows];  // FP64 sums (products may be FP32)
   __shared__ double red[16];
   const int bin = blockIdx.x, tid = threadIdx.x;
@@ -527,16 +549,12 @@ __global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict
   const double beta = st->beta, tele = st->tele;
   for (int i = tid; i < rows; i += 1024) {
     const int r = base + i;
-    if (MODE == 1) {
-      out[r] = static_cast<double>(ybin[i]);
-    } else {
-      const double sv = MIXED ? yhat[r] + static_cast<double>(ybin[i]) : static_cast<double>(ybin[i]);
-      const double bi = add_teleport<TM>(beta * sv, tele, tp, r);
-      out[r] = bi;
-      lmax = fmax(lmax, bi);
-    }
+    if (MODE == 2) yhat_out[r] = static_cast<double>(ybin[i]);
+    const double sv = MIXED ? yhat[r] + static_cast<double>(ybin[i]) : static_cast<double>(ybin[i]);
+    const double bi = add_teleport<TM>(beta * sv, tele, tp, r);
+    out[r] = bi;
+    lmax = fmax(lmax, bi);
   }
-  if (MODE == 1) return;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off, 64));
   if ((tid & 63) == 0) red[tid >> 6] = lmax;
@@ -557,7 +575,7 @@ PbView pb_view(const PbMatrix &pb) {
 // VT from `pb` (dev::Pattern: value-free), x of type XT, sums of type AT.
 template <typename VT, typename XT, typename AT, bool MIXED, int TM, int MODE>
 void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const double *yhat, const Teleport &tp, double *out,
-               PrState *st, unsigned long long *slots, hipStream_t s) {
+               PrState *st, unsigned long long *slots, hipStream_t s, double *yhat_out = nullptr) {
   const VT *fval = vslot >= 0 ? pb.fval[vslot].as<VT>() : nullptr;
   const VT *lval = vslot >= 0 ? pb.lval[vslot].as<VT>() : nullptr;
   AT *prod = pb.prod.as<AT>();
@@ -565,9 +583,9 @@ void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const double *yhat, c
     hipLaunchKernelGGL((pb_scatter<VT, XT, AT>), dim3((pb.nsb + 7) / 8 * 8), dim3(kPbScatterThreads), 0, s,
                        pb.sp.as<int>(),
                        pb.fcol.as<unsigned short>(), pb.fslot.as<int>(), fval, x, prod, pb.nsb, pb.n, st,
-                       MODE == 1 ? 1 : 0);
+                       MODE != 0 ? 1 : 0);
   hipLaunchKernelGGL((pb_gather<VT, XT, AT, MIXED, TM, MODE>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval,
-                     x, prod, yhat, tp, out, st, slots);
+                     x, prod, yhat, tp, out, st, slots, yhat_out);
 }
 
 template <typename VT, typename XT, typename AT, bool MIXED>
@@ -579,6 +597,19 @@ void launch_pb_update(int tmode, const PbMatrix &pb, int vslot, const XT *x, con
     launch_pb<VT, XT, AT, MIXED, 2, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
   else
     launch_pb<VT, XT, AT, MIXED, 0, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
+}
+
+// The anchor product of an iteration that moves the anchor, fused with its
+// update (MODE 2): yhat = A64 x, out = beta*yhat + teleport.
+template <typename VT>
+void launch_pb_anchor_update(int tmode, const PbMatrix &pb, int vslot, const double *x, double *yhat,
+                             const Teleport &tp, double *out, PrState *st, unsigned long long *slots, hipStream_t s) {
+  if (tmode == 1)
+    launch_pb<VT, double, double, false, 1, 2>(pb, vslot, x, nullptr, tp, out, st, slots, s, yhat);
+  else if (tmode == 2)
+    launch_pb<VT, double, double, false, 2, 2>(pb, vslot, x, nullptr, tp, out, st, slots, s, yhat);
+  else
+    launch_pb<VT, double, double, false, 0, 2>(pb, vslot, x, nullptr, tp, out, st, slots, s, yhat);
 }
 
 template <typename VT, typename XT, bool MIXED, int TM>
@@ -622,7 +653,7 @@ __global__ void zero_u64(unsigned long long *p, int n) {
 
 struct Scratch {
   DevBuf st, slots, perr, ps, pc, total, member;
-  DevBuf X0, X1, Z, yhat, xhat, D0, D1, colscale;
+  DevBuf X0, X1, Z, yhat, xhat, D0, D1, colscale, ZD0, ZD1, colscale32;
 };
 
 }  // namespace
@@ -635,11 +666,12 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     throw std::invalid_argument("tbsla: page_rank needs a square matrix held in one block");
   PageRankReport rep;
   if (n == 0) return rep;
-  const bool mixed = p.precision == PrPrecision::F32 || p.precision == PrPrecision::BF16;
+  const bool pat32 = p.precision == PrPrecision::PATTERN_F32;
+  const bool mixed = p.precision == PrPrecision::F32 || p.precision == PrPrecision::BF16 || pat32;
   const bool csr = m.format() == cpp::FORMAT_CSR;
   if ((mixed || p.precision == PrPrecision::PATTERN) && !csr)
     throw std::invalid_argument("tbsla: mixed-precision and value-free PageRank need a CSR matrix");
-  if (p.precision == PrPrecision::PATTERN && !m.count_stochastic())
+  if ((p.precision == PrPrecision::PATTERN || pat32) && !m.count_stochastic())
     throw std::invalid_argument("tbsla: value-free PageRank needs values from fill_cqmat_stochastic");
   if (mixed && p.rule != PrRule::SUMNORM)
     throw std::invalid_argument("tbsla: mixed-precision PageRank uses the SUMNORM stopping rule");
@@ -699,6 +731,10 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     const auto &va = csrm.get_values();
     for (std::size_t k = 0; k < va.size(); k++) cs[static_cast<std::size_t>(ci[k])] = va[k];
     w.colscale.upload(cs.data(), cs.size() * sizeof(double), s);
+    if (pat32) {
+      std::vector<float> c32(cs.begin(), cs.end());
+      w.colscale32.upload(c32.data(), c32.size() * sizeof(float), s);
+    }
   }
   PrState h{};
   h.beta = p.beta;
@@ -717,7 +753,7 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   h.mixed = mixed ? 1 : 0;
   h.need_anchor = mixed ? 1 : 0;
   h.anchors = mixed ? 1 : 0;
-  h.floor_eps = p.precision == PrPrecision::BF16 ? 4e-3 : 6e-8;
+  h.floor_eps = p.precision == PrPrecision::BF16 ? 4e-3 : 6e-8;  // FP32 and PATTERN_F32: 6e-8
   h.anchor_tau = p.anchor_tau > 0 ? p.anchor_tau : (p.precision == PrPrecision::BF16 ? 5e-2 : 3e-6);
   h.done = p.max_iteration <= 0 ? 1 : 0;
   TBSLA_HIP_CHECK(hipMemcpyAsync(w.st.get(), &h, sizeof(h), hipMemcpyHostToDevice, s));
@@ -737,6 +773,15 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.xhat.as<double>(), n, b0);
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[0], n, 0.0f);
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[1], n, 0.0f);
+  }
+  float *ZD[2] = {nullptr, nullptr};
+  if (pat32) {
+    w.ZD0.alloc(sizeof(float) * n);
+    w.ZD1.alloc(sizeof(float) * n);
+    ZD[0] = w.ZD0.as<float>();
+    ZD[1] = w.ZD1.as<float>();
+    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, ZD[0], n, 0.0f);
+    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, ZD[1], n, 0.0f);
   }
   if (pattern || (mixed && m.count_stochastic())) {
     w.Z.alloc(sizeof(double) * n);
@@ -769,15 +814,20 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
       if (fused) {
         if (mixed) {
           float *dcur = D[it & 1], *dnext = D[(it + 1) & 1];
+          float *zdcur = ZD[it & 1], *zdnext = ZD[(it + 1) & 1];
           double *xhat = w.xhat.as<double>(), *yhat = w.yhat.as<double>();
-          hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, xhat, dcur, mixed_pattern ? Z : nullptr,
-                             colscale, n, st);
+          hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, xhat, dcur,
+                             pbm ? nullptr : (pat32 ? zdcur : dcur), mixed_pattern ? Z : nullptr, colscale, n, st);
           if (pbm) {
+            // an iteration that moves the anchor runs the FP64 anchor product
+            // with the update; the others run the delta product below
             if (mixed_pattern)
-              launch_pb<dev::Pattern, double, double, false, 0, 1>(*pbm, -1, Z, nullptr, tp, yhat, st, slots, s);
+              launch_pb_anchor_update<dev::Pattern>(tmode, *pbm, -1, Z, yhat, tp, xout, st, slots, s);
             else
-              launch_pb<double, double, double, false, 0, 1>(*pbm, 0, xhat, nullptr, tp, yhat, st, slots, s);
-            if (p.precision == PrPrecision::F32)
+              launch_pb_anchor_update<double>(tmode, *pbm, 0, xhat, yhat, tp, xout, st, slots, s);
+            if (pat32)
+              launch_pb_update<dev::Pattern, float, float, true>(tmode, *pbm, -1, zdcur, yhat, tp, xout, st, slots, s);
+            else if (p.precision == PrPrecision::F32)
               launch_pb_update<float, float, float, true>(tmode, *pbm, 1, dcur, yhat, tp, xout, st, slots, s);
             else
               launch_pb_update<dev::bf16, float, float, true>(tmode, *pbm, 2, dcur, yhat, tp, xout, st, slots, s);
@@ -786,14 +836,22 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
               launch_anchor<dev::Pattern>(d64, Z, yhat, st, s);
             else
               launch_anchor<double>(d64, xhat, yhat, st, s);
-            if (p.precision == PrPrecision::F32)
+            if (pat32)
+              launch_update<dev::Pattern, float, true>(tmode, d64, nullptr, zdcur, yhat, tp, xout, st, slots, s);
+            else if (p.precision == PrPrecision::F32)
               launch_update<float, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
             else
               launch_update<dev::bf16, float, true>(tmode, *dlo, nullptr, dcur, yhat, tp, xout, st, slots, s);
           }
           fin_max();
-          hipLaunchKernelGGL(pr_normalize<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,
-                             nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
+          if (pat32)
+            hipLaunchKernelGGL(pr_normalize<3>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,
+                               nullptr, nullptr, zdnext, w.colscale32.as<float>(), n, st, w.perr.as<double>(),
+                               w.ps.as<double>(), w.pc.as<double>());
+          else
+            hipLaunchKernelGGL(pr_normalize<1>, dim3(gN), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,
+                               nullptr, nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
+                               w.pc.as<double>());
           fin_norm();
         } else if (pattern) {
           if (pbm)
@@ -802,7 +860,8 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
             launch_update<dev::Pattern, double, false>(tmode, d64, nullptr, Z, nullptr, tp, xout, st, slots, s);
           fin_max();
           hipLaunchKernelGGL(pr_normalize<2>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
-                             nullptr, Z, colscale, n, st, w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>());
+                             nullptr, Z, colscale, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
+                             w.pc.as<double>());
           fin_norm();
         } else {
           if (pbm)
@@ -811,8 +870,8 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
             launch_update<double, double, false>(tmode, d64, p.values, xin, nullptr, tp, xout, st, slots, s);
           fin_max();
           hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
-                             nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
-                             w.pc.as<double>());
+                             nullptr, nullptr, nullptr, nullptr, nullptr, n, st, w.perr.as<double>(),
+                             w.ps.as<double>(), w.pc.as<double>());
           fin_norm();
         }
       } else {
@@ -827,8 +886,8 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
           hipLaunchKernelGGL(pr_update_vec<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, n, tp, st, slots);
         fin_max();
         hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, n, st, w.perr.as<double>(), w.ps.as<double>(),
-                           w.pc.as<double>());
+                           nullptr, nullptr, nullptr, nullptr, nullptr, n, st, w.perr.as<double>(),
+                           w.ps.as<double>(), w.pc.as<double>());
         fin_norm();
       }
     }

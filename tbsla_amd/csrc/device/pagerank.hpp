@@ -19,6 +19,11 @@
 //            (value(i,j) = 1/count_j): the normalise pass writes
 //            z = colscale .* b, the next product sums gathered z.  Bit-identical
 //            products to F64.
+//   PATTERN_F32  the mixed scheme of F32 without a value array, for
+//            count-stochastic matrices: the normalise pass writes, beside the
+//            FP32 delta, zd = colscale .* delta rounded to FP32, and the FP32
+//            product sums gathered zd (no value bytes, FP32 products); the
+//            anchor product is the value-free FP64 one.
 // Stopping rules: REFERENCE (err < epsilon, the reference's quantity) or
 // SUMNORM (a bound on the L1 distance between successive sum-normalised
 // iterates, err/S_k + |S_{k-1}/S_k - 1| <= epsilon * (1-beta)/beta).
@@ -31,7 +36,7 @@
 namespace tbsla {
 namespace hip {
 
-enum class PrPrecision : int { F64 = 0, F32 = 1, BF16 = 2, PATTERN = 3 };
+enum class PrPrecision : int { F64 = 0, F32 = 1, BF16 = 2, PATTERN = 3, PATTERN_F32 = 4 };
 enum class PrRule : int { REFERENCE = 0, SUMNORM = 1 };
 
 struct PageRankParams {

@@ -28,6 +28,7 @@ from ._lib import check, dptr, iptr, lib
 _INFO_KEYS = ["format", "n_row", "n_col", "f_row", "f_col", "ln_row", "ln_col", "pr", "pc", "NR", "NC",
               "nnz", "max_col", "n_values", "n_idx0", "n_idx1"]
 _DTYPES = {"f64": _lib.F64, "float64": _lib.F64, "f32": _lib.F32, "float32": _lib.F32,
+           "pattern_f32": _lib.PR_PATTERN_F32,
            "bf16": _lib.BF16, "bfloat16": _lib.BF16, "pattern": _lib.PATTERN}
 
 

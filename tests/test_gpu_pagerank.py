@@ -52,12 +52,12 @@ def test_page_rank_2_20_reference_rule(graph_2_20, digests, gpu):
 
 
 @pytest.mark.parametrize("gather", ["tiles", "blocked"])
-@pytest.mark.parametrize("precision", ["f64", "pattern", "f32", "bf16"])
+@pytest.mark.parametrize("precision", ["f64", "pattern", "f32", "bf16", "pattern_f32"])
 def test_page_rank_precisions_reach_1e10(graph_2_20, gpu, precision, gather):
     """Every precision variant, with fixed-order tile products or with
     propagation-blocked products, stops on the sum-normalised bound and lands
-    within 1e-10 L1 of the FP64 fixed point (BF16/FP32 values need the FP64
-    anchor product to get there)."""
+    within 1e-10 L1 of the FP64 fixed point (BF16/FP32 values and the
+    value-free FP32 deltas need the FP64 anchor product to get there)."""
     import torch
     p, _, _, pstar = graph_2_20
     out = torch.zeros(p.get_n_col(), dtype=torch.float64, device=gpu)
@@ -146,13 +146,30 @@ def test_blocked_products_small_and_odd_sizes(tb, gpu):
         m = p.get_n_col()
         a = torch.zeros(m, dtype=torch.float64, device=gpu)
         b = torch.zeros_like(a)
-        for prec in ("f64", "f32", "bf16"):
+        precs = ("f64", "f32", "bf16") + (("pattern_f32",) if p.count_stochastic() else ())
+        for prec in precs:
             ra = p.page_rank_device(a, epsilon=1e-12, max_iteration=400, rule="sumnorm", precision=prec,
                                     gather="tiles")
             rb = p.page_rank_device(b, epsilon=1e-12, max_iteration=400, rule="sumnorm", precision=prec,
                                     gather="blocked")
             assert abs(ra["iterations"] - rb["iterations"]) <= 2, (m, prec)
             assert float((a - b).abs().sum()) <= 1e-11, (m, prec, float((a - b).abs().sum()))
+
+
+def test_value_free_precisions_need_count_stochastic_values(tb, gpu):
+    """PATTERN and PATTERN_F32 read no values, so they are refused for a
+    matrix whose values are not 1/count of their column; PATTERN_F32 also
+    needs the sum-normalised rule, like the other mixed schemes."""
+    import torch
+    from tbsla_amd._lib import TbslaError
+    m = tb.MatrixCSR().fill_cqmat(1000, 1000, 6, 0.3, 3)
+    out = torch.zeros(1000, dtype=torch.float64, device=gpu)
+    for prec in ("pattern", "pattern_f32"):
+        with pytest.raises(TbslaError):
+            m.page_rank_device(out, epsilon=1e-10, max_iteration=50, precision=prec, rule="sumnorm")
+    s = tb.MatrixCSR().fill_cqmat_stochastic(1000, 1000, 6, 0.3, 3)
+    with pytest.raises(TbslaError):
+        s.page_rank_device(out, epsilon=1e-10, max_iteration=50, precision="pattern_f32", rule="reference")
 
 
 def test_page_rank_zero_iterations(tb, gpu):
@@ -174,7 +191,7 @@ def test_full_size_pagerank_2_24(tb, gpu):
     g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
     out = torch.zeros_like(pstar)
     for prec, gather in [("f64", "tiles"), ("f64", "blocked"), ("pattern", "blocked"), ("f32", "blocked"),
-                         ("bf16", "blocked")]:
+                         ("bf16", "blocked"), ("pattern_f32", "blocked")]:
         rep = g.page_rank_device(out, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm",
                                  gather=gather)
         err = float((out - pstar).abs().sum())
