@@ -413,28 +413,52 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
     for (int s2 = 0; s2 < ns; s2++)
       spc[static_cast<std::size_t>(s2) + 1] = colcnt[static_cast<std::size_t>(std::min(n, (s2 + 1) * kPbCols))];
     pb->far_src.resize(nf);
-    std::vector<int> fslot(nf);
-    std::vector<unsigned short> fcol(nf);
+    // per superblock, the runs of entries bound for one bin, in bin order:
+    // entry o of a run lands in slot o + delta, with delta the run's first
+    // slot minus its first entry
+    std::vector<unsigned> fkey(nf);
+    std::vector<int> runp(static_cast<std::size_t>(ns) + 1, 0), rdelta;
+    constexpr int kNoDelta = std::numeric_limits<int>::min();
     {
       std::vector<int> cnt(static_cast<std::size_t>(nb) + 1);
       for (int s2 = 0; s2 < ns; s2++) {
         const int k0 = spc[static_cast<std::size_t>(s2)], k1 = spc[static_cast<std::size_t>(s2) + 1];
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int k = k0; k < k1; k++) cnt[static_cast<std::size_t>(csc_row[static_cast<std::size_t>(k)] >> kPbShift) + 1]++;
+        // run index of each non-empty bin, and its slot delta (the run's
+        // entries are consecutive in CSC order within the bin's slots)
+        std::vector<int> run_of(static_cast<std::size_t>(nb), -1);
+        int nr = 0;
+        for (int b = 0; b < nb; b++)
+          if (cnt[static_cast<std::size_t>(b) + 1] > 0) run_of[static_cast<std::size_t>(b)] = nr++;
+        if (nr > 65536) bad("propagation-blocked layout: more than 65536 bins reached from one superblock");
         for (int b = 0; b < nb; b++) cnt[static_cast<std::size_t>(b) + 1] += cnt[static_cast<std::size_t>(b)];
+        const std::size_t rbase = rdelta.size();
+        rdelta.resize(rbase + static_cast<std::size_t>(nr), kNoDelta);
         for (int k = k0; k < k1; k++) {  // column order, so each bin's run stays in column order
           const int b = csc_row[static_cast<std::size_t>(k)] >> kPbShift;
           const int o = k0 + cnt[static_cast<std::size_t>(b)]++;
+          const int r = run_of[static_cast<std::size_t>(b)];
           pb->far_src[static_cast<std::size_t>(o)] = csc_src[static_cast<std::size_t>(k)];
-          fslot[static_cast<std::size_t>(o)] = slot_csc[static_cast<std::size_t>(k)];
-          fcol[static_cast<std::size_t>(o)] = static_cast<unsigned short>(
-              (ci[static_cast<std::size_t>(r0 + csc_src[static_cast<std::size_t>(k)])] - c.get_f_col()) & (kPbCols - 1));
+          const int col = (ci[static_cast<std::size_t>(r0 + csc_src[static_cast<std::size_t>(k)])] - c.get_f_col()) &
+                          (kPbCols - 1);
+          fkey[static_cast<std::size_t>(o)] = static_cast<unsigned>(col) | (static_cast<unsigned>(r) << 16);
+          // a bin's slots follow CSC order and a run is one superblock's
+          // column range of that order, so slot - o is the same for the
+          // whole run
+          const int d = slot_csc[static_cast<std::size_t>(k)] - o;
+          int &rd = rdelta[rbase + static_cast<std::size_t>(r)];
+          if (rd != kNoDelta && rd != d) bad("propagation-blocked layout: run slots are not consecutive");
+          rd = d;
         }
+        runp[static_cast<std::size_t>(s2) + 1] = static_cast<int>(rdelta.size());
       }
     }
     pb->sp.upload(spc.data(), spc.size() * sizeof(int));
-    pb->fcol.upload(fcol.data(), fcol.size() * sizeof(unsigned short));
-    pb->fslot.upload(fslot.data(), fslot.size() * sizeof(int));
+    pb->fkey.upload(fkey.data(), fkey.size() * sizeof(unsigned));
+    pb->rp.upload(runp.data(), runp.size() * sizeof(int));
+    if (rdelta.empty()) rdelta.push_back(0);
+    pb->rdelta.upload(rdelta.data(), rdelta.size() * sizeof(int));
     pb->frow.upload(frow.data(), frow.size() * sizeof(unsigned short));
     pb->bp.upload(bincnt.data(), bincnt.size() * sizeof(int));
     pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));

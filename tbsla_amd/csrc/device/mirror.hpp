@@ -92,7 +92,11 @@ constexpr int kPbCols = 1 << kPbColShift;
 struct PbMatrix {
   int n = 0, nbins = 0, nsb = 0;
   long n_far = 0, n_loc = 0;  // n_loc counts the padding entries too
-  DevBuf sp, fcol, fslot, frow, bp, lp, lrow, lcol;
+  // far entries in scatter order: fkey = column within the superblock (low
+  // 16 bits) | run within the superblock (high 16 bits); a run is the far
+  // entries of one superblock that go to one bin, and entry o of run r of
+  // superblock s goes to product slot o + rdelta[rp[s] + r]
+  DevBuf sp, fkey, rp, rdelta, frow, bp, lp, lrow, lcol;
   DevBuf fval[3], lval[3];  // by Dtype F64, F32, BF16 (built on demand)
   DevBuf prod;              // 8-byte product slots, n_far
   std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry (-1: padding)

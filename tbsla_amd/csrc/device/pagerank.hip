@@ -425,10 +425,11 @@ struct PbVal<dev::Pattern> {
 constexpr int kPbScatterThreads = 1024;
 
 template <typename VT, typename XT, typename AT>
-__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__restrict__ sp, const unsigned short *__restrict__ fcol,
-                                                  const int *__restrict__ slot, const VT *__restrict__ val,
-                                                  const XT *__restrict__ x, AT *__restrict__ prod, int nsb, int n,
-                                                  const PrState *st, int anchor) {
+__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__restrict__ sp, const unsigned *__restrict__ fkey,
+                                                  const int *__restrict__ rp, const int *__restrict__ rdelta,
+                                                  const VT *__restrict__ val, const XT *__restrict__ x,
+                                                  AT *__restrict__ prod, int nsb, int n, const PrState *st,
+                                                  int anchor) {
 #pragma clang fp contract(off)
   // the anchor product runs only when the anchor moves; the delta product
   // of that iteration is skipped (the delta is zero and the anchor gather
@@ -440,6 +441,7 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__res
   const int sb = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (sb >= nsb) return;
   const int k1 = sp[sb + 1];
+  const int *__restrict__ delta = rdelta + rp[sb];  // slot - entry of each run of this superblock
   // the superblock's slice of x, staged in LDS (64 KB in FP64): read once
   __shared__ XT xs[kPbCols];
   const long c0 = static_cast<long>(sb) << kPbColShift;
@@ -456,22 +458,27 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__res
     for (int u = 0; u < S; u++) xs[threadIdx.x + u * kPbScatterThreads] = t[u];
   }
   __syncthreads();
-  // four entries per thread in flight: every global load first, then the LDS
-  // reads of x, then the products and the stores
+  // four entries per thread in flight: every global load first, then the
+  // run offsets and the LDS reads of x, then the products and the stores
   for (int k = sp[sb] + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
-    int sl[4], fc[4];
+    unsigned key[4];
+    int sl[4];
     AT v1[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int kk = k + u * kPbScatterThreads;
       const bool ok = kk < k1;
-      fc[u] = ok ? fcol[kk] : 0;
-      sl[u] = ok ? slot[kk] : -1;
+      key[u] = ok ? fkey[kk] : 0u;
       v1[u] = ok ? PbVal<VT>::template prod<AT>(val, kk, AT(1)) : AT(0);
     }
 #pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int kk = k + u * kPbScatterThreads;
+      sl[u] = kk < k1 ? kk + delta[key[u] >> 16] : -1;
+    }
+#pragma unroll
     for (int u = 0; u < 4; u++)
-      if (sl[u] >= 0) prod[sl[u]] = v1[u] * static_cast<AT>(xs[fc[u]]);
+      if (sl[u] >= 0) prod[sl[u]] = v1[u] * static_cast<AT>(xs[key[u] & 0xFFFFu]);
   }
 }
 
@@ -582,7 +589,7 @@ void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const double *yhat, c
   if (pb.n_far > 0)
     hipLaunchKernelGGL((pb_scatter<VT, XT, AT>), dim3((pb.nsb + 7) / 8 * 8), dim3(kPbScatterThreads), 0, s,
                        pb.sp.as<int>(),
-                       pb.fcol.as<unsigned short>(), pb.fslot.as<int>(), fval, x, prod, pb.nsb, pb.n, st,
+                       pb.fkey.as<unsigned>(), pb.rp.as<int>(), pb.rdelta.as<int>(), fval, x, prod, pb.nsb, pb.n, st,
                        MODE != 0 ? 1 : 0);
   hipLaunchKernelGGL((pb_gather<VT, XT, AT, MIXED, TM, MODE>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval,
                      x, prod, yhat, tp, out, st, slots, yhat_out);
