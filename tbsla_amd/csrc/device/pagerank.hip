@@ -527,29 +527,44 @@ __global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict
   // local entries are in row order and each bin's run is padded to groups
   // of four: one group per thread (aligned vector loads), runs of one row
   // summed and added once
-  for (int k = v.lp[bin] + kPbLocalGroup * tid; k < l1; k += kPbLocalGroup * 1024) {
-    const dev::v4us rr = *reinterpret_cast<const dev::v4us *>(v.lrow + k);
-    const dev::v4us cc = *reinterpret_cast<const dev::v4us *>(v.lcol + k);
-    AT lv[4];
-    PbVal<VT>::template get4<AT>(lval, k, lv);
-    const int r[4] = {rr.x, rr.y, rr.z, rr.w};
-    const int c[4] = {cc.x, cc.y, cc.z, cc.w};
-    AT xv[4];
+  constexpr int LU = 2;  // groups per thread in flight (4 measured slower)
+  for (int k = v.lp[bin] + kPbLocalGroup * tid; k < l1; k += LU * kPbLocalGroup * 1024) {
+    int r[LU][4], c[LU][4];
+    AT lv[LU][4], xv[LU][4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) xv[u] = static_cast<AT>(x[base + c[u]]);
-    double acc = 0.0;
-    int row = r[0];
+    for (int g = 0; g < LU; g++) {
+      const int kg = k + g * kPbLocalGroup * 1024;
+      if (g == 0 || kg < l1) {
+        const dev::v4us rr = *reinterpret_cast<const dev::v4us *>(v.lrow + kg);
+        const dev::v4us cc = *reinterpret_cast<const dev::v4us *>(v.lcol + kg);
+        PbVal<VT>::template get4<AT>(lval, kg, lv[g]);
+        r[g][0] = rr.x, r[g][1] = rr.y, r[g][2] = rr.z, r[g][3] = rr.w;
+        c[g][0] = cc.x, c[g][1] = cc.y, c[g][2] = cc.z, c[g][3] = cc.w;
+      } else {  // past the bin's run: a group of padding
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      if (r[u] == kPbPadRow) break;  // padding closes a bin's run
-      if (r[u] != row) {
-        atomicAdd(&ybin[row], acc);
-        acc = 0.0;
-        row = r[u];
+        for (int u = 0; u < 4; u++) r[g][u] = kPbPadRow, c[g][u] = 0, lv[g][u] = AT(0);
       }
-      acc += static_cast<double>(PbVal<VT>::template mul<AT>(lv[u], xv[u]));
     }
-    if (row != kPbPadRow) atomicAdd(&ybin[row], acc);
+#pragma unroll
+    for (int g = 0; g < LU; g++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) xv[g][u] = static_cast<AT>(x[base + c[g][u]]);
+#pragma unroll
+    for (int g = 0; g < LU; g++) {
+      double acc = 0.0;
+      int row = r[g][0];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (r[g][u] == kPbPadRow) break;  // padding closes a bin's run
+        if (r[g][u] != row) {
+          atomicAdd(&ybin[row], acc);
+          acc = 0.0;
+          row = r[g][u];
+        }
+        acc += static_cast<double>(PbVal<VT>::template mul<AT>(lv[g][u], xv[g][u]));
+      }
+      if (row != kPbPadRow) atomicAdd(&ybin[row], acc);
+    }
   }
   __syncthreads();
   double lmax = -INFINITY;
