@@ -37,6 +37,7 @@ struct PrState {
   double tele;          // teleport term used by the next update
   double maxv;          // max of the last update
   double err;           // reference residual sum|b_k - b_{k-1}| of the last iteration
+  double err_prev;      // ... and of the one before
   double sum, sum_prev; // sum of the last / previous normalised iterate (compensated)
   double bound, bound_prev, bound_anchor;
   double anchor_tau, floor_eps;
@@ -173,6 +174,7 @@ __global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict
 // Decision logic of the last block of pr_normalize (one thread).
 __device__ inline void finalize_step(PrState *st, double err, double S) {
   st->it += 1;
+  st->err_prev = st->err;
   st->err = err;
   st->sum_prev = st->sum;
   st->sum = S;
@@ -680,6 +682,20 @@ struct Scratch {
 
 }  // namespace
 
+// Iterations to launch before the next state read: the iterations the last
+// two residuals predict are left (geometric decay of the quantity the rule
+// tests), at least 1 and at most `check`.  Kernels launched past convergence
+// return at once, so this only trims those launches.
+int next_batch(const PrState &h, int check) {
+  const bool sumnorm = h.rule != 0;
+  const double cur = sumnorm ? h.bound : h.err, prev = sumnorm ? h.bound_prev : h.err_prev;
+  const double target = sumnorm ? h.bound_target : h.eps;
+  if (!(cur > 0.0) || !(prev > cur) || !(target > 0.0) || cur <= target) return check;
+  const double left = std::log(target / cur) / std::log(cur / prev);
+  if (!(left > 0.0)) return check;
+  return static_cast<int>(std::min<double>(check, std::max(1.0, std::ceil(left))));
+}
+
 PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, double *out_dev, hipStream_t s) {
   const long n = m.get_n_col();
   // (COO keeps NR/NC in ln_row/ln_col, MatrixCOO.cpp:344-345, so the block
@@ -818,6 +834,7 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   TBSLA_HIP_CHECK(hipEventCreate(&e1));
   TBSLA_HIP_CHECK(hipEventRecord(e0, s));
   const int check = fused ? std::max(1, p.check_every) : 1;
+  int next = check;  // iterations in the next launch batch
   PrState hs{};
   int launched = 0;
   const bool mixed_pattern = mixed && m.count_stochastic();
@@ -829,7 +846,7 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   double *Z = w.Z.as<double>();
   const double *colscale = w.colscale.as<double>();
   while (launched < p.max_iteration) {
-    const int batch = std::min(check, p.max_iteration - launched);
+    const int batch = std::min(next, p.max_iteration - launched);
     for (int k = 0; k < batch; k++) {
       const int it = launched + k;
       double *xin = X[it & 1], *xout = X[(it + 1) & 1];
@@ -918,6 +935,7 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     TBSLA_HIP_CHECK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
     TBSLA_HIP_CHECK(hipStreamSynchronize(s));
     if (hs.done) break;
+    next = next_batch(hs, check);
   }
   TBSLA_HIP_CHECK(hipEventRecord(e1, s));
   const int it = hs.it;
