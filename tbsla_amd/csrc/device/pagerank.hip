@@ -285,13 +285,13 @@ __global__ __launch_bounds__(kTileThreads) void pr_finalize_norm(const double *_
   if (threadIdx.x == 0) finalize_step(st, e, t.s + t.c);
 }
 
-// Mixed precision: move the anchor to the current iterate when requested.
-// xhat += delta (and z = colscale .* xhat for the value-free anchor
-// product).  With propagation-blocked products the iteration that moves the
-// anchor skips the delta product and its normalise pass starts from xhat, so
-// nothing else changes.  The tile products still add the delta product in
-// that iteration, so there its input `clear` (the delta, or the scaled delta
-// of PATTERN_F32) is zeroed.
+// Mixed precision with tile products: move the anchor to the current iterate
+// when requested, xhat += delta (and z = colscale .* xhat for the value-free
+// anchor product).  The tile products still add the delta product in that
+// iteration, so its input `clear` (the delta, or the scaled delta of
+// PATTERN_F32) is zeroed; the normalise pass of that iteration starts from
+// xhat.  (With propagation-blocked products the move is part of the scatter,
+// and that iteration has no delta product.)
 __global__ __launch_bounds__(256) void pr_anchor_move(double *__restrict__ xhat, const float *delta, float *clear,
                                                       double *__restrict__ z, const double *__restrict__ colscale,
                                                       long n, const PrState *st) {
@@ -426,35 +426,64 @@ struct PbVal<dev::Pattern> {
 
 constexpr int kPbScatterThreads = 1024;
 
-template <typename VT, typename XT, typename AT>
-__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__restrict__ sp, const unsigned *__restrict__ fkey,
-                                                  const int *__restrict__ rp, const int *__restrict__ rdelta,
-                                                  const VT *__restrict__ val, const XT *__restrict__ x,
-                                                  AT *__restrict__ prod, int nsb, int n, const PrState *st,
-                                                  int anchor) {
+struct PbScatterArgs {
+  const int *sp;        // far entries of superblock s: [sp[s], sp[s+1])
+  const unsigned *fkey; // column within the superblock | run << 16
+  const int *rp;        // runs of superblock s: [rp[s], rp[s+1])
+  const int *rdelta;    // slot - entry of each run
+  int nsb, n;
+};
+
+// The anchor move, folded into the scatter of the iteration that makes it:
+// over the superblock's columns c, xhat[c] += delta[c], and the product's
+// input is z[c] = colscale[c] * xhat[c] (value-free anchor product; z is
+// written for the gather) or xhat[c] itself (z == nullptr).
+struct AnchorMove {
+  double *xhat;
+  const float *delta;
+  double *z;
+  const double *colscale;
+};
+
+// The far products of one superblock (picked from blockIdx): its slice of x
+// staged in xs (LDS), then every far product v * x[c] written to its slot.
+// With MOVE the slice is the anchor move's result instead of x.
+template <typename VT, typename XT, typename AT, bool MOVE>
+__device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restrict__ val, const XT *__restrict__ x,
+                                     AT *__restrict__ prod, XT *xs, const AnchorMove &am) {
 #pragma clang fp contract(off)
-  // the anchor product runs only when the anchor moves; the delta product
-  // of that iteration is skipped (the delta is zero and the anchor gather
-  // writes the update)
-  if (st->done || (anchor ? !st->need_anchor : st->need_anchor)) return;
   // superblocks of one XCD (blockIdx % 8) are contiguous, so the runs that
   // share a cache line at a run boundary are written through the same L2
   const int per = (gridDim.x + 7) / 8;
   const int sb = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (sb >= nsb) return;
-  const int k1 = sp[sb + 1];
-  const int *__restrict__ delta = rdelta + rp[sb];  // slot - entry of each run of this superblock
-  // the superblock's slice of x, staged in LDS (64 KB in FP64): read once
-  __shared__ XT xs[kPbCols];
+  if (sb >= a.nsb) return;
+  const int k1 = a.sp[sb + 1];
+  const int *__restrict__ delta = a.rdelta + a.rp[sb];  // slot - entry of each run of this superblock
   const long c0 = static_cast<long>(sb) << kPbColShift;
-  const int nc = static_cast<int>(min(static_cast<long>(kPbCols), static_cast<long>(n) - c0));
+  const int nc = static_cast<int>(min(static_cast<long>(kPbCols), static_cast<long>(a.n) - c0));
   {
     constexpr int S = kPbCols / kPbScatterThreads;  // x values per thread, all loads issued first
     XT t[S];
 #pragma unroll
     for (int u = 0; u < S; u++) {
       const int i = threadIdx.x + u * kPbScatterThreads;
-      t[u] = i < nc ? x[c0 + i] : XT(0);
+      if (MOVE) {
+        if (i < nc) {
+          const long c = c0 + i;
+          const double v = am.xhat[c] + static_cast<double>(am.delta[c]);
+          am.xhat[c] = v;
+          double xv = v;
+          if (am.z) {
+            xv = am.colscale[c] * v;
+            am.z[c] = xv;
+          }
+          t[u] = static_cast<XT>(xv);
+        } else {
+          t[u] = XT(0);
+        }
+      } else {
+        t[u] = i < nc ? x[c0 + i] : XT(0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < S; u++) xs[threadIdx.x + u * kPbScatterThreads] = t[u];
@@ -462,7 +491,7 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__res
   __syncthreads();
   // four entries per thread in flight: every global load first, then the
   // run offsets and the LDS reads of x, then the products and the stores
-  for (int k = sp[sb] + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
+  for (int k = a.sp[sb] + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
     unsigned key[4];
     int sl[4];
     AT v1[4];
@@ -470,7 +499,7 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__res
     for (int u = 0; u < 4; u++) {
       const int kk = k + u * kPbScatterThreads;
       const bool ok = kk < k1;
-      key[u] = ok ? fkey[kk] : 0u;
+      key[u] = ok ? a.fkey[kk] : 0u;
       v1[u] = ok ? PbVal<VT>::template prod<AT>(val, kk, AT(1)) : AT(0);
     }
 #pragma unroll
@@ -484,26 +513,49 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(const int *__res
   }
 }
 
+// FP64 runs: the far products of x.
+template <typename VT, typename XT, typename AT>
+__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a, const VT *__restrict__ val,
+                                                                const XT *__restrict__ x, AT *__restrict__ prod,
+                                                                const PrState *st) {
+  if (st->done) return;
+  __shared__ XT xs[kPbCols];
+  pb_scatter_sb<VT, XT, AT, false>(a, val, x, prod, xs, AnchorMove{});
+}
+
+// Mixed runs, one launch per iteration: in an iteration that moves the
+// anchor, the anchor move and the FP64 anchor product's far products (values
+// VA, FP64); otherwise the delta product's (values VD, delta XD, sums AD).
+template <typename VA, typename VD, typename XD, typename AD>
+__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mixed(PbScatterArgs a, const VA *__restrict__ vala,
+                                                                      const VD *__restrict__ vald,
+                                                                      const XD *__restrict__ xd, void *prod,
+                                                                      AnchorMove am, const PrState *st) {
+  if (st->done) return;
+  __shared__ double xs[kPbCols];
+  if (st->need_anchor)
+    pb_scatter_sb<VA, double, double, true>(a, vala, nullptr, static_cast<double *>(prod), xs, am);
+  else
+    pb_scatter_sb<VD, XD, AD, false>(a, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am);
+}
+
 struct PbView {
   const int *bp, *lp;
   const unsigned short *frow, *lrow, *lcol;
   int n;
 };
 
-// MODE 0: PageRank update epilogue (b = beta*(yhat + s) + t, block max);
-//         with MIXED it is skipped in an iteration that moves the anchor.
+// The sums of one bin (blockIdx) and its row epilogue.
+// MODE 0: b = beta*(yhat + s) + t (MIXED) or beta*s + t, block max.
 // MODE 2: the anchor product with the update of an iteration that moves the
 //         anchor (its delta is zero): yhat_out = s, b = beta*s + t, block max.
 template <typename VT, typename XT, typename AT, bool MIXED, int TM, int MODE>
-__global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict__ lval, const XT *__restrict__ x,
-                                                  const AT *__restrict__ prod, const double *__restrict__ yhat,
-                                                  Teleport tp, double *__restrict__ out, PrState *st,
-                                                  unsigned long long *slots, double *__restrict__ yhat_out) {
+__device__ inline void pb_gather_bin(const PbView &v, const VT *__restrict__ lval, const XT *__restrict__ x,
+                                     const AT *__restrict__ prod, const double *__restrict__ yhat, const Teleport &tp,
+                                     double *__restrict__ out, const PrState *st, unsigned long long *slots,
+                                     double *__restrict__ yhat_out, double *ybin, double *red) {
 #pragma clang fp contract(off)
   static_assert(MODE == 0 || (MODE == 2 && !MIXED), "pb_gather modes: 0, and 2 without MIXED");
-  if (MODE == 0 ? (st->done || (MIXED && st->need_anchor)) : (st->done || !st->need_anchor)) return;
-  __shared__ double ybin[kPbRows];  // FP64 sums (products may be FP32)
-  __shared__ double red[16];
   const int bin = blockIdx.x, tid = threadIdx.x;
   const int base = bin << kPbShift;
   const int rows = min(kPbRows, v.n - base);
@@ -590,50 +642,99 @@ __global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict
   }
 }
 
+// FP64 runs.
+template <typename VT, typename XT, typename AT, int TM>
+__global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict__ lval, const XT *__restrict__ x,
+                                                  const AT *__restrict__ prod, Teleport tp, double *__restrict__ out,
+                                                  PrState *st, unsigned long long *slots) {
+  if (st->done) return;
+  __shared__ double ybin[kPbRows];  // FP64 sums (products may be FP32)
+  __shared__ double red[16];
+  pb_gather_bin<VT, XT, AT, false, TM, 0>(v, lval, x, prod, nullptr, tp, out, st, slots, nullptr, ybin, red);
+}
+
+// Mixed runs, the partner of pb_scatter_mixed: the anchor product with the
+// update (MODE 2, x = xa) or the delta product with the update (x = xd).
+template <typename VA, typename VD, typename XD, typename AD, int TM>
+__global__ __launch_bounds__(1024) void pb_gather_mixed(PbView v, const VA *__restrict__ lvala,
+                                                        const double *__restrict__ xa, const VD *__restrict__ lvald,
+                                                        const XD *__restrict__ xd, const void *prod, double *yhat,
+                                                        Teleport tp, double *__restrict__ out, PrState *st,
+                                                        unsigned long long *slots) {
+  if (st->done) return;
+  __shared__ double ybin[kPbRows];
+  __shared__ double red[16];
+  if (st->need_anchor)
+    pb_gather_bin<VA, double, double, false, TM, 2>(v, lvala, xa, static_cast<const double *>(prod), nullptr, tp, out,
+                                                    st, slots, yhat, ybin, red);
+  else
+    pb_gather_bin<VD, XD, AD, true, TM, 0>(v, lvald, xd, static_cast<const AD *>(prod), yhat, tp, out, st, slots,
+                                           nullptr, ybin, red);
+}
+
 PbView pb_view(const PbMatrix &pb) {
   return PbView{pb.bp.as<int>(), pb.lp.as<int>(), pb.frow.as<unsigned short>(), pb.lrow.as<unsigned short>(),
                 pb.lcol.as<unsigned short>(), pb.n};
 }
 
-// One propagation-blocked product (scatter then gather) with values of type
-// VT from `pb` (dev::Pattern: value-free), x of type XT, sums of type AT.
-template <typename VT, typename XT, typename AT, bool MIXED, int TM, int MODE>
-void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const double *yhat, const Teleport &tp, double *out,
-               PrState *st, unsigned long long *slots, hipStream_t s, double *yhat_out = nullptr) {
+PbScatterArgs pb_args(const PbMatrix &pb) {
+  return PbScatterArgs{pb.sp.as<int>(), pb.fkey.as<unsigned>(), pb.rp.as<int>(), pb.rdelta.as<int>(), pb.nsb, pb.n};
+}
+
+// One propagation-blocked product with the update (FP64 runs): values of
+// type VT from `pb` (dev::Pattern: value-free), x of type XT, sums of type AT.
+template <typename VT, typename XT, typename AT, int TM>
+void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const Teleport &tp, double *out, PrState *st,
+               unsigned long long *slots, hipStream_t s) {
   const VT *fval = vslot >= 0 ? pb.fval[vslot].as<VT>() : nullptr;
   const VT *lval = vslot >= 0 ? pb.lval[vslot].as<VT>() : nullptr;
   AT *prod = pb.prod.as<AT>();
   if (pb.n_far > 0)
     hipLaunchKernelGGL((pb_scatter<VT, XT, AT>), dim3((pb.nsb + 7) / 8 * 8), dim3(kPbScatterThreads), 0, s,
-                       pb.sp.as<int>(),
-                       pb.fkey.as<unsigned>(), pb.rp.as<int>(), pb.rdelta.as<int>(), fval, x, prod, pb.nsb, pb.n, st,
-                       MODE != 0 ? 1 : 0);
-  hipLaunchKernelGGL((pb_gather<VT, XT, AT, MIXED, TM, MODE>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval,
-                     x, prod, yhat, tp, out, st, slots, yhat_out);
+                       pb_args(pb), fval, x, prod, st);
+  hipLaunchKernelGGL((pb_gather<VT, XT, AT, TM>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval, x, prod, tp,
+                     out, st, slots);
 }
 
-template <typename VT, typename XT, typename AT, bool MIXED>
-void launch_pb_update(int tmode, const PbMatrix &pb, int vslot, const XT *x, const double *yhat, const Teleport &tp,
-                      double *out, PrState *st, unsigned long long *slots, hipStream_t s) {
+template <typename VT, typename XT, typename AT>
+void launch_pb_update(int tmode, const PbMatrix &pb, int vslot, const XT *x, const Teleport &tp, double *out,
+                      PrState *st, unsigned long long *slots, hipStream_t s) {
   if (tmode == 1)
-    launch_pb<VT, XT, AT, MIXED, 1, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
+    launch_pb<VT, XT, AT, 1>(pb, vslot, x, tp, out, st, slots, s);
   else if (tmode == 2)
-    launch_pb<VT, XT, AT, MIXED, 2, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
+    launch_pb<VT, XT, AT, 2>(pb, vslot, x, tp, out, st, slots, s);
   else
-    launch_pb<VT, XT, AT, MIXED, 0, 0>(pb, vslot, x, yhat, tp, out, st, slots, s);
+    launch_pb<VT, XT, AT, 0>(pb, vslot, x, tp, out, st, slots, s);
 }
 
-// The anchor product of an iteration that moves the anchor, fused with its
-// update (MODE 2): yhat = A64 x, out = beta*yhat + teleport.
-template <typename VT>
-void launch_pb_anchor_update(int tmode, const PbMatrix &pb, int vslot, const double *x, double *yhat,
-                             const Teleport &tp, double *out, PrState *st, unsigned long long *slots, hipStream_t s) {
+// One iteration's product and update of a mixed run: the anchor move and
+// the FP64 anchor product (values VA at slot va, x = z or xhat) when the
+// state asks for it, else the delta product (values VD at slot vd, delta
+// xd).  The scatter always runs: it also makes the anchor move.
+template <typename VA, typename VD, typename XD, typename AD, int TM>
+void launch_pb_mixed(const PbMatrix &pb, int va, int vd, const AnchorMove &am, const XD *xd, double *yhat,
+                     const Teleport &tp, double *out, PrState *st, unsigned long long *slots, hipStream_t s) {
+  const VA *fa = va >= 0 ? pb.fval[va].as<VA>() : nullptr;
+  const VA *la = va >= 0 ? pb.lval[va].as<VA>() : nullptr;
+  const VD *fd = vd >= 0 ? pb.fval[vd].as<VD>() : nullptr;
+  const VD *ld = vd >= 0 ? pb.lval[vd].as<VD>() : nullptr;
+  const double *xa = am.z ? am.z : am.xhat;
+  hipLaunchKernelGGL((pb_scatter_mixed<VA, VD, XD, AD>), dim3((pb.nsb + 7) / 8 * 8), dim3(kPbScatterThreads), 0, s,
+                     pb_args(pb), fa, fd, xd, pb.prod.get(), am, st);
+  hipLaunchKernelGGL((pb_gather_mixed<VA, VD, XD, AD, TM>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), la, xa,
+                     ld, xd, pb.prod.get(), yhat, tp, out, st, slots);
+}
+
+template <typename VA, typename VD, typename XD, typename AD>
+void launch_pb_mixed_update(int tmode, const PbMatrix &pb, int va, int vd, const AnchorMove &am, const XD *xd,
+                            double *yhat, const Teleport &tp, double *out, PrState *st, unsigned long long *slots,
+                            hipStream_t s) {
   if (tmode == 1)
-    launch_pb<VT, double, double, false, 1, 2>(pb, vslot, x, nullptr, tp, out, st, slots, s, yhat);
+    launch_pb_mixed<VA, VD, XD, AD, 1>(pb, va, vd, am, xd, yhat, tp, out, st, slots, s);
   else if (tmode == 2)
-    launch_pb<VT, double, double, false, 2, 2>(pb, vslot, x, nullptr, tp, out, st, slots, s, yhat);
+    launch_pb_mixed<VA, VD, XD, AD, 2>(pb, va, vd, am, xd, yhat, tp, out, st, slots, s);
   else
-    launch_pb<VT, double, double, false, 0, 2>(pb, vslot, x, nullptr, tp, out, st, slots, s, yhat);
+    launch_pb_mixed<VA, VD, XD, AD, 0>(pb, va, vd, am, xd, yhat, tp, out, st, slots, s);
 }
 
 template <typename VT, typename XT, bool MIXED, int TM>
@@ -855,22 +956,29 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
           float *dcur = D[it & 1], *dnext = D[(it + 1) & 1];
           float *zdcur = ZD[it & 1], *zdnext = ZD[(it + 1) & 1];
           double *xhat = w.xhat.as<double>(), *yhat = w.yhat.as<double>();
-          hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, xhat, dcur,
-                             pbm ? nullptr : (pat32 ? zdcur : dcur), mixed_pattern ? Z : nullptr, colscale, n, st);
           if (pbm) {
-            // an iteration that moves the anchor runs the FP64 anchor product
-            // with the update; the others run the delta product below
-            if (mixed_pattern)
-              launch_pb_anchor_update<dev::Pattern>(tmode, *pbm, -1, Z, yhat, tp, xout, st, slots, s);
-            else
-              launch_pb_anchor_update<double>(tmode, *pbm, 0, xhat, yhat, tp, xout, st, slots, s);
+            // one scatter and one gather per iteration: the anchor move and
+            // the FP64 anchor product with the update when the state asks for
+            // an anchor, else the delta product with the update
+            const AnchorMove am{xhat, dcur, mixed_pattern ? Z : nullptr, colscale};
             if (pat32)
-              launch_pb_update<dev::Pattern, float, float, true>(tmode, *pbm, -1, zdcur, yhat, tp, xout, st, slots, s);
+              launch_pb_mixed_update<dev::Pattern, dev::Pattern, float, float>(tmode, *pbm, -1, -1, am, zdcur, yhat,
+                                                                                tp, xout, st, slots, s);
+            else if (mixed_pattern && p.precision == PrPrecision::F32)
+              launch_pb_mixed_update<dev::Pattern, float, float, float>(tmode, *pbm, -1, 1, am, dcur, yhat, tp, xout,
+                                                                        st, slots, s);
+            else if (mixed_pattern)
+              launch_pb_mixed_update<dev::Pattern, dev::bf16, float, float>(tmode, *pbm, -1, 2, am, dcur, yhat, tp,
+                                                                            xout, st, slots, s);
             else if (p.precision == PrPrecision::F32)
-              launch_pb_update<float, float, float, true>(tmode, *pbm, 1, dcur, yhat, tp, xout, st, slots, s);
+              launch_pb_mixed_update<double, float, float, float>(tmode, *pbm, 0, 1, am, dcur, yhat, tp, xout, st,
+                                                                  slots, s);
             else
-              launch_pb_update<dev::bf16, float, float, true>(tmode, *pbm, 2, dcur, yhat, tp, xout, st, slots, s);
+              launch_pb_mixed_update<double, dev::bf16, float, float>(tmode, *pbm, 0, 2, am, dcur, yhat, tp, xout,
+                                                                      st, slots, s);
           } else {
+            hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, xhat, dcur, pat32 ? zdcur : dcur,
+                               mixed_pattern ? Z : nullptr, colscale, n, st);
             if (mixed_pattern)
               launch_anchor<dev::Pattern>(d64, Z, yhat, st, s);
             else
@@ -894,7 +1002,7 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
           fin_norm();
         } else if (pattern) {
           if (pbm)
-            launch_pb_update<dev::Pattern, double, double, false>(tmode, *pbm, -1, Z, nullptr, tp, xout, st, slots, s);
+            launch_pb_update<dev::Pattern, double, double>(tmode, *pbm, -1, Z, tp, xout, st, slots, s);
           else
             launch_update<dev::Pattern, double, false>(tmode, d64, nullptr, Z, nullptr, tp, xout, st, slots, s);
           fin_max();
@@ -904,7 +1012,7 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
           fin_norm();
         } else {
           if (pbm)
-            launch_pb_update<double, double, double, false>(tmode, *pbm, 0, xin, nullptr, tp, xout, st, slots, s);
+            launch_pb_update<double, double, double>(tmode, *pbm, 0, xin, tp, xout, st, slots, s);
           else
             launch_update<double, double, false>(tmode, d64, p.values, xin, nullptr, tp, xout, st, slots, s);
           fin_max();
