@@ -374,6 +374,12 @@ __global__ __launch_bounds__(256) void scale_into(const double *__restrict__ s, 
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) z[i] = s[i] * x[i];
 }
 
+__global__ __launch_bounds__(256) void scale_into_f32(const double *__restrict__ s, double v, float *__restrict__ z,
+                                                      long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256)
+    z[i] = static_cast<float>(s[i] * v);
+}
+
 unsigned grid_for(long n) { return static_cast<unsigned>(std::max<long>(1, std::min<long>(kNormBlocks, (n + 255) / 256))); }
 
 // ---------------------------------------------------------------------------
@@ -890,8 +896,12 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   h.max_it = p.max_iteration;
   h.rule = static_cast<int>(p.rule);
   h.mixed = mixed ? 1 : 0;
-  h.need_anchor = mixed ? 1 : 0;
-  h.anchors = mixed ? 1 : 0;
+  // mixed runs start from the anchor xhat = 0 (so yhat = A xhat = 0 with no
+  // product) and delta = b0: the first phase carries the whole iterate in the
+  // low precision, and the first anchor move comes when it has converged to
+  // that precision's floor
+  h.need_anchor = 0;
+  h.anchors = 0;
   h.floor_eps = p.precision == PrPrecision::BF16 ? 4e-3 : 6e-8;  // FP32 and PATTERN_F32: 6e-8
   h.anchor_tau = p.anchor_tau > 0 ? p.anchor_tau : (p.precision == PrPrecision::BF16 ? 5e-2 : 3e-6);
   h.done = p.max_iteration <= 0 ? 1 : 0;
@@ -909,8 +919,9 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     w.D1.alloc(sizeof(float) * n);
     D[0] = w.D0.as<float>();
     D[1] = w.D1.as<float>();
-    hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.xhat.as<double>(), n, b0);
-    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[0], n, 0.0f);
+    hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.xhat.as<double>(), n, 0.0);
+    hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.yhat.as<double>(), n, 0.0);
+    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[0], n, static_cast<float>(b0));
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[1], n, 0.0f);
   }
   float *ZD[2] = {nullptr, nullptr};
@@ -919,7 +930,7 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     w.ZD1.alloc(sizeof(float) * n);
     ZD[0] = w.ZD0.as<float>();
     ZD[1] = w.ZD1.as<float>();
-    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, ZD[0], n, 0.0f);
+    hipLaunchKernelGGL(scale_into_f32, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), b0, ZD[0], n);
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, ZD[1], n, 0.0f);
   }
   if (pattern || (mixed && m.count_stochastic())) {
