@@ -14,8 +14,9 @@ Extra fields:
   cpu_baseline   the reference's own MatrixCSR::Ax (compiled from its sources)
                  on the host cores, and the bit-exact check of the GPU y
   pagerank       fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) in FP64, FP32 and
-                 BF16 value storage and value-free FP64, each stopped by the
-                 sum-normalised bound, against the FP64 fixed point
+                 BF16 value storage, value-free FP64 and value-free FP32, each
+                 stopped by the sum-normalised bound, against the FP64 fixed
+                 point (median of three runs each)
   batch          256 personalised graphs (n log-uniform in [2^10, 2^18]) plus
                  a 2^22-node graph with 64 one-hot teleport sets, one hipGraph
   adjoint        gradient of sum w_i p_i on fill_cqmat_stochastic(2^20, 2^20,
@@ -148,9 +149,12 @@ def leg_pagerank(torch, tb, dev):
     for prec in ["f64", "f32", "bf16", "pattern", "pattern_f32"]:
         o = torch.zeros(n, dtype=torch.float64, device=dev)
         g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")  # warm
-        rep = g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")
+        # the median of three timed runs (one run varies by 1-2%)
+        reps = [g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")
+                for _ in range(3)]
+        rep = sorted(reps, key=lambda r: r["seconds"])[1]
         err = float((o - pstar).abs().sum().item())
-        pr[prec] = {"seconds": round(rep["seconds"], 6), "iterations": rep["iterations"],
+        pr[prec] = {"seconds": round(rep["seconds"], 6), "runs": 3, "iterations": rep["iterations"],
                     "anchors": rep["anchors"], "l1_vs_fp64_fixed_point": err}
         log(f"pagerank {prec}: {rep['seconds'] * 1e3:.2f} ms, {rep['iterations']} it, "
             f"{rep['anchors']} anchors, L1 {err:.2e}")
