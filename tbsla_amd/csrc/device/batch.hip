@@ -310,8 +310,11 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
 
 // Persistent step: every workgroup walks its queue with a static stride
 // (queue = XCD of the workgroup when xcd_map is set, else one shared list).
+// (sixteen gathers in flight fit two waves per SIMD in FP32 only with the
+// register budget held to 256)
 template <typename VT, typename XT, int UNR>
-__global__ __launch_bounds__(kTileThreads) void batch_step(BatchDev<VT, XT> d, int par) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(UNR == 16 ? 2 : 1)))
+void batch_step(BatchDev<VT, XT> d, int par) {
   using AT = typename dev::Val<VT>::A;
   __shared__ dev::CsrTileLds<AT, kBatchTileNnz> lds;
   __shared__ double red[kTileThreads];
@@ -946,8 +949,10 @@ void PageRankBatch::Impl::launch_step(const BatchDev<VT, XT> &d, int par, int un
   const unsigned grid = static_cast<unsigned>(std::max(kQueues, (cus * wg_per_cu) / kQueues * kQueues));
   if (unroll == 4)
     hipLaunchKernelGGL((batch_step<VT, XT, 4>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
-  else
+  else if (unroll == 8)
     hipLaunchKernelGGL((batch_step<VT, XT, 8>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+  else
+    hipLaunchKernelGGL((batch_step<VT, XT, 16>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
 }
 
 template <typename VT, typename XT>
@@ -1002,7 +1007,7 @@ BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hi
     BatchConfig bestc = cfg;
     for (int xm = 0; xm < 2; xm++) {
       const BatchDev<VT, XT> d = dev_view<VT, XT>(xm, p.beta);
-      for (int un : {4, 8})
+      for (int un : {4, 8, 16})
         for (int wg : {2, 4, 8}) {
           launch_step<VT, XT>(d, 0, un, wg, s);
           TBSLA_HIP_CHECK(hipEventRecord(e0, s));
