@@ -124,6 +124,32 @@ def test_blocked_products_with_teleport_vector_and_bins(tb, oracle, gpu):
         assert float((a - b).abs().sum()) <= 1e-13, prec
 
 
+def test_mixed_precisions_with_teleport_vector_and_node_set(tb, oracle, gpu):
+    """The mixed schemes (FP32 and BF16 values, value-free FP32) with an
+    explicit teleport distribution and with a personalised node set, on both
+    product structures: within 1e-10 L1 of the FP64 run of the same problem
+    (the anchor products are FP64 in every teleport mode)."""
+    import torch
+    n = 5 * 8192 + 1234
+    p = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 12, 0.4, 77)
+    v = oracle.uniform_vector(5, n) + 0.05
+    v /= v.sum()
+    vd = torch.from_numpy(v).to(gpu)
+    nodes = np.array([17, 1000, 30000, n - 1], dtype=np.int32)
+    ref = torch.zeros(n, dtype=torch.float64, device=gpu)
+    out = torch.zeros_like(ref)
+    for kw in ({"teleport": vd}, {"nodes": nodes}):
+        p.page_rank_device(ref, epsilon=1e-13, max_iteration=500, rule="sumnorm", precision="f64", gather="tiles",
+                           **kw)
+        for prec in ("f32", "bf16", "pattern_f32"):
+            for gather in ("tiles", "blocked"):
+                rep = p.page_rank_device(out, epsilon=1e-12, max_iteration=500, rule="sumnorm", precision=prec,
+                                         gather=gather, **kw)
+                err = float((out - ref).abs().sum())
+                assert err <= 1e-10, (list(kw), prec, gather, err, rep)
+                assert abs(float(out.sum()) - 1.0) < 1e-12
+
+
 def test_blocked_products_small_and_odd_sizes(tb, gpu):
     """Propagation-blocked products where one bin holds everything (no far
     entries), where a bin has fewer than four local entries, and on a
