@@ -5,13 +5,15 @@
 // conversions here use a stable order, ties in input order.  On the GPU the
 // same order comes from a stable LSD radix sort (hipCUB) of the 64-bit keys
 // row << 32 | (column with its sign bit flipped), carrying the entry index;
-// the entries are then gathered in that order and laid out as CSR or ELL.
+// the entries are then gathered in that order (CSR) and, for ELL, spread to
+// the padded layout on the host.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
+#include "../host/gen.hpp"
 #include "common.hpp"
 #include "convert.hpp"
 
@@ -44,20 +46,6 @@ __global__ __launch_bounds__(kThreads) void coo_gather(long n, const int *__rest
   const int s = idx[k];
   ocol[k] = col[s];
   oval[k] = val[s];
-}
-
-// ELL: sorted entry k is slot k - rowptr[row] of its row
-__global__ __launch_bounds__(kThreads) void coo_place_ell(long n, const unsigned long long *__restrict__ key,
-                                                          const int *__restrict__ idx, const int *__restrict__ rowptr,
-                                                          const int *__restrict__ col, const double *__restrict__ val,
-                                                          int K, int *__restrict__ ecol, double *__restrict__ eval) {
-  const long k = static_cast<long>(blockIdx.x) * kThreads + threadIdx.x;
-  if (k >= n) return;
-  const int r = static_cast<int>(key[k] >> 32);
-  const long o = static_cast<long>(r) * K + (k - rowptr[r]);
-  const int s = idx[k];
-  ecol[o] = col[s];
-  eval[o] = val[s];
 }
 
 // Sorted keys, permutation and row pointers of a COO on the device.
@@ -134,26 +122,28 @@ void coo_to_csr_gpu(int n_row, const std::vector<int> &row, const std::vector<in
 
 void coo_to_ell_gpu(int n_row, const std::vector<int> &row, const std::vector<int> &col,
                     const std::vector<double> &val, int &K, std::vector<int> &columns, std::vector<double> &values) {
-  Sorted s;
-  sort_coo(n_row, row, col, val, s);
+  // sorted on the GPU, brought back in the compact CSR form, then spread to
+  // the padded n_row x K layout on the host cores (the padded arrays are K/avg
+  // times larger than what crosses the bus)
+  std::vector<int> rowptr, colidx;
+  std::vector<double> v;
+  coo_to_csr_gpu(n_row, row, col, val, rowptr, colidx, v);
   K = 0;
   for (int r = 0; r < n_row; r++)
-    K = std::max(K, s.rowptr_h[static_cast<std::size_t>(r) + 1] - s.rowptr_h[static_cast<std::size_t>(r)]);
+    K = std::max(K, rowptr[static_cast<std::size_t>(r) + 1] - rowptr[static_cast<std::size_t>(r)]);
   const std::size_t slots = static_cast<std::size_t>(n_row) * static_cast<std::size_t>(K);
-  columns.assign(slots, 0);
-  values.assign(slots, 0.0);
-  if (s.n == 0) return;
-  DevBuf ec, ev;
-  ec.alloc(sizeof(int) * slots);
-  ev.alloc(sizeof(double) * slots);
-  TBSLA_HIP_CHECK(hipMemset(ec.get(), 0, sizeof(int) * slots));
-  TBSLA_HIP_CHECK(hipMemset(ev.get(), 0, sizeof(double) * slots));
-  hipLaunchKernelGGL(coo_place_ell, dim3(grid_of(s.n)), dim3(kThreads), 0, 0, s.n, s.key.as<unsigned long long>(),
-                     s.idx.as<int>(), s.rowptr.as<int>(), s.col.as<int>(), s.val.as<double>(), K, ec.as<int>(),
-                     ev.as<double>());
-  TBSLA_HIP_CHECK(hipGetLastError());
-  TBSLA_HIP_CHECK(hipMemcpy(columns.data(), ec.get(), slots * sizeof(int), hipMemcpyDeviceToHost));
-  TBSLA_HIP_CHECK(hipMemcpy(values.data(), ev.get(), slots * sizeof(double), hipMemcpyDeviceToHost));
+  columns.resize(slots);
+  values.resize(slots);
+  gen::parallel_for(n_row, [&](long b, long e) {
+    for (long r = b; r < e; r++) {
+      const std::size_t o = static_cast<std::size_t>(r) * static_cast<std::size_t>(K);
+      const int a = rowptr[static_cast<std::size_t>(r)], len = rowptr[static_cast<std::size_t>(r) + 1] - a;
+      for (int j = 0; j < K; j++) {
+        columns[o + static_cast<std::size_t>(j)] = j < len ? colidx[static_cast<std::size_t>(a + j)] : 0;
+        values[o + static_cast<std::size_t>(j)] = j < len ? v[static_cast<std::size_t>(a + j)] : 0.0;
+      }
+    }
+  });
 }
 
 }  // namespace hip
