@@ -229,6 +229,21 @@ def test_c_abi_errors_without_device_calls(tb):
     coo = tb.MatrixCOO().fill_cqmat(10, 10, 3, 0.1, 1)
     assert _lib.lib.tbsla_matrix_convert_gpu(coo._h, 4) is None  # DENSE: not a GPU conversion
     assert b"CSR or ELL" in _lib.lib.tbsla_last_error()
+    # PageRank parameters are checked before any device call (the output
+    # pointer is never touched): precision (TBSLA_PR_PATTERN_F32 = 4 is the
+    # last), stopping rule, product structure
+    import ctypes
+    from tbsla_amd.matrix import dtype_code
+    assert dtype_code("pattern_f32") == _lib.PR_PATTERN_F32 == 4
+    einval = -1  # TBSLA_EINVAL
+    g = tb.MatrixCSR().fill_cqmat_stochastic(50, 50, 4, 0.3, 1)
+    for field, bad in (("precision", 5), ("precision", -1), ("rule", 2), ("gather", 3)):
+        prm = _lib.PageRankParams(epsilon=1e-10, beta=0.85, max_iteration=10, precision=0, rule=1, check_every=8,
+                                  anchor_tau=0.0, nodes=None, n_nodes=0, gather=0)
+        setattr(prm, field, bad)
+        rc = _lib.lib.tbsla_hip_page_rank_ex(g._h, ctypes.byref(prm), None, 0, None, 0, ctypes.c_void_p(8), None,
+                                             None)
+        assert rc == einval, (field, bad, rc)
     v = _lib.lib.C_CPP_Vector_create()
     _lib.lib.C_CPP_Vector_fill(v, 5, 3)
     assert not _lib.lib.C_CPP_Vector_add(v, None)
