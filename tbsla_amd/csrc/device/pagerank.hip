@@ -848,9 +848,9 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
 
   Scratch w;
   const unsigned gN = grid_for(n);
-  // the mixed normalise pass (six streams) measured faster with four times
-  // the blocks; the FP64 one (three streams) slower
-  const unsigned gM = mixed ? grid_for(n, 4L * kNormBlocks) : gN;
+  // the mixed normalise pass (six streams) measured faster with twice the
+  // blocks (four and eight times: slower); the FP64 one (three streams) slower
+  const unsigned gM = mixed ? grid_for(n, 2L * kNormBlocks) : gN;
   w.st.alloc(sizeof(PrState));
   w.slots.alloc(sizeof(unsigned long long) * kMaxSlots);
   w.perr.alloc(sizeof(double) * std::max(gN, gM));
