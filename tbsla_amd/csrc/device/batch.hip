@@ -29,7 +29,7 @@ constexpr int kChunkRows = 2048;     // rows per pass item, single-RHS graphs
 // the persistent step (tile and SpMM items in one kernel) keeps its
 // occupancy
 constexpr int kBatchTileNnz = 2048;
-constexpr int kSpmmChunkRows = 256;  // rows per pass item, SpMM groups
+constexpr int kSpmmChunkRows = 1024;  // rows per pass item, SpMM groups
 constexpr int kMaxGroups = 64;
 constexpr int kQueues = 8;           // one per XCD
 
