@@ -182,6 +182,33 @@ def test_ell_slot_counts_and_partial_slices(tb, gpu):
             assert np.array_equal(ell.spmv(x), expect), (n, K)
 
 
+def test_ell_low_precision_slots(tb, gpu):
+    """The ELL kernel with FP32 and BF16 stored values (FP32 x and y), for K
+    around the 8-slot groups and row counts around the 128-row slices:
+    within the storage precision of the FP64 product, relative to
+    sum |a| |x| per row."""
+    import torch
+    rng = np.random.default_rng(12)
+    for n in (127, 129, 1000):
+        for K in (3, 8, 9, 17, 25):
+            lens = rng.integers(0, K + 1, n)
+            lens[rng.integers(0, n)] = K
+            row = np.repeat(np.arange(n, dtype=np.int32), lens)
+            col = rng.integers(0, n, row.size).astype(np.int32)
+            val = rng.standard_normal(row.size)
+            ell = tb.MatrixELL(tb.MatrixCOO(n, n, val, row, col))
+            x = rng.standard_normal(n).astype(np.float32)
+            ref = ell.spmv(x.astype(np.float64))
+            absr = tb.MatrixELL(tb.MatrixCOO(n, n, np.abs(val), row, col)).spmv(np.abs(x).astype(np.float64))
+            xd = torch.from_numpy(x).to(gpu)
+            for vd, tol in (("f32", 1e-6), ("bf16", 1e-2)):
+                y = torch.zeros(n, dtype=torch.float32, device=gpu)
+                ell.Ax_device(xd, y, value_dtype=vd)
+                torch.cuda.synchronize()
+                err = np.abs(y.cpu().numpy().astype(np.float64) - ref)
+                assert np.all(err <= tol * absr + 1e-30), (n, K, vd, float((err / (absr + 1e-300)).max()))
+
+
 def test_short_vectors_are_rejected(tb, gpu):
     """A vector shorter than the matrix's indices is an error, not a fault."""
     from tbsla_amd._lib import TbslaError
