@@ -17,7 +17,7 @@
 //                   y[row], so each row is accumulated in entry order
 //                   (MatrixCOO.cpp:113-115 / MatrixSCOO.cpp:120-122 with one
 //                   thread).
-// dense_kernel      one wavefront per row, butterfly reduction
+// dense_kernel      four rows per wavefront sharing each x load, butterfly reduction
 //                   (MatrixDENSE.cpp:67-72, different summation order).
 #include <hip/hip_runtime.h>
 
@@ -181,35 +181,69 @@ __global__ __launch_bounds__(kTileThreads) void coo_tile_kernel(dev::CsrView<VT>
     dev::csr_tile_rows<VT, XT>(m, x, lds, epi);
 }
 
+// kDenseRows consecutive rows per wavefront, so each x load serves all of
+// them; per row and lane the columns are summed in the same order as with one
+// row per wavefront (j = 2*lane + 128*k), then a butterfly.
+constexpr int kDenseRows = 4;
+
 template <typename XT, typename YT>
 __global__ __launch_bounds__(256) void dense_kernel(const double *__restrict__ A, const XT *__restrict__ x,
                                                     YT *__restrict__ y, int rows, int cols, long ld) {
+  constexpr int R = kDenseRows;
   const int w = (blockIdx.x * 256 + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (w >= rows) return;
-  const double *a = A + static_cast<long>(w) * ld;
-  double acc = 0;
+  const long r0 = static_cast<long>(w) * R;
+  if (r0 >= rows) return;
+  const int nr = static_cast<int>(min(static_cast<long>(R), rows - r0));  // rows of this wavefront
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = 0;
   if ((ld & 1) == 0) {
-    // 16-byte non-temporal loads, two columns per lane, four steps in flight
+    // 16-byte non-temporal loads, two columns per lane, two steps of R rows in flight
     int j = 2 * lane;
-    for (; j + 3 * 128 + 1 < cols; j += 4 * 128) {
-      dev::v2d av[4];
+    for (; j + 128 + 1 < cols; j += 2 * 128) {
+      dev::v2d av[2][R];
+      double xv[2][2];
 #pragma unroll
-      for (int u = 0; u < 4; u++) av[u] = __builtin_nontemporal_load(reinterpret_cast<const dev::v2d *>(a + j + u * 128));
+      for (int u = 0; u < 2; u++) {
+        xv[u][0] = static_cast<double>(x[j + u * 128]);
+        xv[u][1] = static_cast<double>(x[j + u * 128 + 1]);
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        acc += av[u].x * static_cast<double>(x[j + u * 128]) + av[u].y * static_cast<double>(x[j + u * 128 + 1]);
+        for (int r = 0; r < R; r++)
+          av[u][r] = r < nr ? __builtin_nontemporal_load(
+                                  reinterpret_cast<const dev::v2d *>(A + (r0 + r) * ld + j + u * 128))
+                            : dev::v2d{0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] += av[u][r].x * xv[u][0] + av[u][r].y * xv[u][1];
     }
-    for (; j < cols; j += 128) {
-      acc += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
-      if (j + 1 < cols) acc += __builtin_nontemporal_load(a + j + 1) * static_cast<double>(x[j + 1]);
-    }
+    for (; j < cols; j += 128)
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (r < nr) {
+          const double *a = A + (r0 + r) * ld;
+          acc[r] += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
+          if (j + 1 < cols) acc[r] += __builtin_nontemporal_load(a + j + 1) * static_cast<double>(x[j + 1]);
+        }
   } else {
-    for (int j = lane; j < cols; j += 64) acc += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (r < nr) {
+        const double *a = A + (r0 + r) * ld;
+        for (int j = lane; j < cols; j += 64) acc[r] += __builtin_nontemporal_load(a + j) * static_cast<double>(x[j]);
+      }
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if (lane == 0) y[w] = static_cast<YT>(static_cast<double>(y[w]) + acc);
+  for (int r = 0; r < R; r++) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[r] += __shfl_xor(acc[r], off, 64);
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (r < nr) y[r0 + r] = static_cast<YT>(static_cast<double>(y[r0 + r]) + acc[r]);
 }
 
 template <typename T>
@@ -251,7 +285,7 @@ void run_typed(const DeviceMatrix &d, const XT *x, YT *y, int vect_incr, hipStre
     }
     case cpp::FORMAT_DENSE:
       if (d.rows > 0)
-        hipLaunchKernelGGL((dense_kernel<XT, YT>), dim3(blocks_for(static_cast<long>(d.rows) * 64, 256)),
+        hipLaunchKernelGGL((dense_kernel<XT, YT>), dim3(blocks_for((static_cast<long>(d.rows) + kDenseRows - 1) / kDenseRows * 64, 256)),
                            dim3(256), 0, s, d.dvals.as<double>(), x, y, d.rows, d.dcols, static_cast<long>(d.dcols));
       break;
   }
