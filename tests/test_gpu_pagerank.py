@@ -198,6 +198,24 @@ def test_value_free_precisions_need_count_stochastic_values(tb, gpu):
         s.page_rank_device(out, epsilon=1e-10, max_iteration=50, precision="pattern_f32", rule="reference")
 
 
+def test_launch_batches_do_not_change_the_run(tb, gpu):
+    """How many iterations the host launches between two reads of the state
+    (check_every, and the batch sized from the residual decay) changes only
+    the launches that return at once: the iteration count and the vector are
+    the same (fixed-order tiles, reference rule, FP64)."""
+    import torch
+    n = 20000
+    p = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 8, 0.5, 7)
+    ref = torch.zeros(n, dtype=torch.float64, device=gpu)
+    r0 = p.page_rank_device(ref, epsilon=1e-12, max_iteration=500, rule="reference", gather="tiles", check_every=1)
+    out = torch.zeros_like(ref)
+    for ce in (2, 3, 8, 64):
+        r = p.page_rank_device(out, epsilon=1e-12, max_iteration=500, rule="reference", gather="tiles",
+                               check_every=ce)
+        assert r["iterations"] == r0["iterations"], (ce, r, r0)
+        assert torch.equal(out, ref), ce
+
+
 def test_page_rank_zero_iterations(tb, gpu):
     m = tb.MatrixCSR().fill_cqmat_stochastic(100, 100, 4, 0.5, 2)
     pr, its = m.page_rank(1e-10, 0.85, 0)
