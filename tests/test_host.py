@@ -268,3 +268,14 @@ def test_batch_workload_generator(tb, oracle):
     n, c, q, seed, sets = w[-1]
     assert (n, c, q, seed, len(sets)) == (1 << 22, 16, 0.3, 1256, 64)
     assert all(0 <= s[0] < n for s in sets)
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU's block; --gpus N > 1 exits with a message."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
