@@ -553,6 +553,30 @@ struct PbView {
   int n;
 };
 
+// One round of a bin's far products: U loads of (row, product) per thread,
+// 1024 entries apart, then their LDS additions. MASKED checks each entry
+// against the end of the bin's run.
+template <int U, bool MASKED, typename AT>
+__device__ inline void pb_far_round(const unsigned short *__restrict__ frow, const AT *__restrict__ prod, int k,
+                                    int f1, double *ybin) {
+  int r[U];
+  double pv[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int kk = k + u * 1024;
+    if (MASKED) {
+      r[u] = kk < f1 ? frow[kk] : -1;
+      pv[u] = kk < f1 ? static_cast<double>(prod[kk]) : 0.0;
+    } else {
+      r[u] = frow[kk];
+      pv[u] = static_cast<double>(prod[kk]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    if (!MASKED || r[u] >= 0) atomicAdd(&ybin[r[u]], pv[u]);
+}
+
 // The sums of one bin (blockIdx) and its row epilogue.
 // MODE 0: b = beta*(yhat + s) + t (MIXED) or beta*s + t, block max.
 // MODE 2: the anchor product with the update of an iteration that moves the
@@ -571,21 +595,12 @@ __device__ inline void pb_gather_bin(const PbView &v, const VT *__restrict__ lva
   __syncthreads();
   const int f1 = v.bp[bin + 1], l1 = v.lp[bin + 1];
   // far products: eight loads per thread issued before their LDS additions
-  // (the compiler does not move global loads across LDS atomics)
+  // (the compiler does not move global loads across LDS atomics); rounds
+  // with all eight in range run unmasked, and one masked round ends the run
   constexpr int U = 8;
-  for (int k = v.bp[bin] + tid; k < f1; k += U * 1024) {
-    int r[U];
-    double pv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int kk = k + u * 1024;
-      r[u] = kk < f1 ? v.frow[kk] : -1;
-      pv[u] = kk < f1 ? static_cast<double>(prod[kk]) : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (r[u] >= 0) atomicAdd(&ybin[r[u]], pv[u]);
-  }
+  int kf = v.bp[bin] + tid;
+  for (; kf + (U - 1) * 1024 < f1; kf += U * 1024) pb_far_round<U, false>(v.frow, prod, kf, f1, ybin);
+  if (kf < f1) pb_far_round<U, true>(v.frow, prod, kf, f1, ybin);
   // local entries are in row order and each bin's run is padded to groups
   // of four: one group per thread (aligned vector loads), runs of one row
   // summed and added once
