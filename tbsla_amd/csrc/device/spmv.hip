@@ -33,30 +33,6 @@ using dev::block_tree_sum;
 using dev::nt_load;
 using dev::Val;
 
-// Exclusive scan of one int per thread over a block of kTileThreads threads.
-__device__ inline int block_exclusive_scan(int v, int &total, int *wsum) {
-  constexpr int kWaves = kTileThreads / 64;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int t = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += t;
-  }
-  if (lane == 63) wsum[w] = inc;
-  __syncthreads();
-  int woff = 0, tot = 0;
-#pragma unroll
-  for (int k = 0; k < kWaves; k++) {
-    const int s = wsum[k];
-    if (k < w) woff += s;
-    tot += s;
-  }
-  total = tot;
-  __syncthreads();
-  return woff + inc - v;
-}
-
 template <typename YT>
 struct StoreRow {
   YT *y;
