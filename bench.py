@@ -77,7 +77,7 @@ def pmc_traffic(name):
 
 def cpu_baseline(tb_csr, x, y_gpu, bytes_alg):
     """The reference's own CSR Ax (oracle/_ref, -O3 -fopenmp) on the same
-    block, timed on the host cores; its result is also the checker of the
+    block, timed on the host cores; its result is also compared with the
     GPU product (bit-exact)."""
     from oracle.oracle import Reference
     if not Reference.available():

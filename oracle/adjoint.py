@@ -1,4 +1,4 @@
-"""FP64 CPU gradient of PageRank -- TEST INFRASTRUCTURE ONLY (the checker of
+"""FP64 CPU gradient of PageRank -- TEST INFRASTRUCTURE ONLY (the oracle of
 the adjoint path; the product never imports it).
 
 The reference has no adjoint.  The forward quantity is the fixed point of the

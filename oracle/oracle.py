@@ -1,4 +1,4 @@
-"""ctypes front-end to the checker libraries.
+"""ctypes front-end to the oracle libraries.
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()``
 and by the CPU-baseline leg of ``bench.py``; never by the product package.
@@ -466,7 +466,7 @@ Reference = _Reference()
 
 
 def build(with_reference: bool | None = None) -> None:
-    """Compile the checker (``make -C oracle``); the reference only if present."""
+    """Compile the oracle (``make -C oracle``); the reference only if present."""
     import subprocess
     subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
     ref_tree = os.path.isdir("/root/reference/src/cpp")

@@ -1,7 +1,7 @@
 /*
  * tbsla_oracle.c -- CPU restatement of the TBSLA SpMV / PageRank path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  See tbsla_oracle.h for what pins
+ * TEST INFRASTRUCTURE ONLY (the oracle).  See tbsla_oracle.h for what pins
  * each function.  Build: oracle/Makefile (gcc -O2 -ffp-contract=off).
  *
  * Citations are to the reference source tree (src/cpp/...).

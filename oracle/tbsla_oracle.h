@@ -1,7 +1,7 @@
 /*
  * tbsla_oracle.h -- CPU restatement of the TBSLA SpMV / PageRank path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only the test
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle: only the test
  * suite, the smoke check and the CPU-baseline leg of bench.py may load it.
  * The product (libtbsla_hip.so) never links or calls it.
  *
