@@ -416,11 +416,12 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
     // per superblock, the runs of entries bound for one bin, in bin order:
     // entry o of a run lands in slot o + delta, with delta the run's first
     // slot minus its first entry
-    std::vector<unsigned> fkey(nf);
+    std::vector<unsigned short> fkey(nf);
     std::vector<int> runp(static_cast<std::size_t>(ns) + 1, 0), rdelta;
+    std::vector<int> chp(static_cast<std::size_t>(ns) + 1, 0), cbase;
     constexpr int kNoDelta = std::numeric_limits<int>::min();
     {
-      std::vector<int> cnt(static_cast<std::size_t>(nb) + 1);
+      std::vector<int> cnt(static_cast<std::size_t>(nb) + 1), erun;
       for (int s2 = 0; s2 < ns; s2++) {
         const int k0 = spc[static_cast<std::size_t>(s2)], k1 = spc[static_cast<std::size_t>(s2) + 1];
         std::fill(cnt.begin(), cnt.end(), 0);
@@ -433,8 +434,10 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
           if (cnt[static_cast<std::size_t>(b) + 1] > 0) run_of[static_cast<std::size_t>(b)] = nr++;
         if (nr > 65536) bad("propagation-blocked layout: more than 65536 bins reached from one superblock");
         for (int b = 0; b < nb; b++) cnt[static_cast<std::size_t>(b) + 1] += cnt[static_cast<std::size_t>(b)];
+        const std::vector<int> first(cnt);  // first entry of each bin's run, from k0
         const std::size_t rbase = rdelta.size();
         rdelta.resize(rbase + static_cast<std::size_t>(nr), kNoDelta);
+        erun.assign(static_cast<std::size_t>(k1 - k0), -1);
         for (int k = k0; k < k1; k++) {  // column order, so each bin's run stays in column order
           const int b = csc_row[static_cast<std::size_t>(k)] >> kPbShift;
           const int o = k0 + cnt[static_cast<std::size_t>(b)]++;
@@ -442,7 +445,9 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
           pb->far_src[static_cast<std::size_t>(o)] = csc_src[static_cast<std::size_t>(k)];
           const int col = (ci[static_cast<std::size_t>(r0 + csc_src[static_cast<std::size_t>(k)])] - c.get_f_col()) &
                           (kPbCols - 1);
-          fkey[static_cast<std::size_t>(o)] = static_cast<unsigned>(col) | (static_cast<unsigned>(r) << 16);
+          const bool starts = o - k0 == first[static_cast<std::size_t>(b)];
+          fkey[static_cast<std::size_t>(o)] = static_cast<unsigned short>(col | (starts ? kPbRunStart : 0u));
+          erun[static_cast<std::size_t>(o - k0)] = r;
           // a bin's slots follow CSC order and a run is one superblock's
           // column range of that order, so slot - o is the same for the
           // whole run
@@ -452,13 +457,29 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
           rd = d;
         }
         runp[static_cast<std::size_t>(s2) + 1] = static_cast<int>(rdelta.size());
+        // run of the first entry of each chunk, checked against the run
+        // every entry gets from it and the run-start flags
+        const int nck = (k1 - k0 + kPbChunk - 1) / kPbChunk;
+        for (int j = 0; j < nck; j++) {
+          const int e0 = j * kPbChunk, e1 = std::min(k1 - k0, e0 + kPbChunk);
+          int run = erun[static_cast<std::size_t>(e0)];
+          cbase.push_back(run);
+          for (int e = e0; e < e1; e++) {
+            if (e > e0 && (fkey[static_cast<std::size_t>(k0 + e)] & kPbRunStart)) run++;
+            if (run != erun[static_cast<std::size_t>(e)]) bad("propagation-blocked layout: run starts do not give the runs");
+          }
+        }
+        chp[static_cast<std::size_t>(s2) + 1] = chp[static_cast<std::size_t>(s2)] + nck;
       }
     }
     pb->sp.upload(spc.data(), spc.size() * sizeof(int));
-    pb->fkey.upload(fkey.data(), fkey.size() * sizeof(unsigned));
+    pb->fkey.upload(fkey.data(), fkey.size() * sizeof(unsigned short));
     pb->rp.upload(runp.data(), runp.size() * sizeof(int));
     if (rdelta.empty()) rdelta.push_back(0);
     pb->rdelta.upload(rdelta.data(), rdelta.size() * sizeof(int));
+    if (cbase.empty()) cbase.push_back(0);
+    pb->cbase.upload(cbase.data(), cbase.size() * sizeof(int));
+    pb->cp.upload(chp.data(), chp.size() * sizeof(int));
     pb->frow.upload(frow.data(), frow.size() * sizeof(unsigned short));
     pb->bp.upload(bincnt.data(), bincnt.size() * sizeof(int));
     pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));

@@ -88,15 +88,23 @@ constexpr int kPbColShift = 13;
 constexpr int kPbLocalGroup = 4;
 constexpr int kPbPadRow = 0xFFFF;
 constexpr int kPbCols = 1 << kPbColShift;
+// scatter keys: the column within the superblock, and this flag on the first
+// entry of each run; the run of an entry is found per wavefront-sized chunk
+constexpr unsigned kPbRunStart = 0x8000u;
+constexpr int kPbChunk = 64;
+static_assert(kPbCols <= 0x2000, "scatter keys hold the column in 13 bits");
 
 struct PbMatrix {
   int n = 0, nbins = 0, nsb = 0;
   long n_far = 0, n_loc = 0;  // n_loc counts the padding entries too
-  // far entries in scatter order: fkey = column within the superblock (low
-  // 16 bits) | run within the superblock (high 16 bits); a run is the far
-  // entries of one superblock that go to one bin, and entry o of run r of
-  // superblock s goes to product slot o + rdelta[rp[s] + r]
-  DevBuf sp, fkey, rp, rdelta, frow, bp, lp, lrow, lcol;
+  // far entries in scatter order: fkey (16 bits) = column within the
+  // superblock, | kPbRunStart on the first entry of a run. A run is the far
+  // entries of one superblock that go to one bin; entry o of run r of
+  // superblock s goes to product slot o + rdelta[rp[s] + r]. Chunk j of
+  // superblock s (entries sp[s] + 64j ...) begins in run cbase[cp[s] + j],
+  // so the run of an entry is cbase plus the run starts after the chunk's
+  // first entry up to the entry itself.
+  DevBuf sp, fkey, rp, rdelta, cbase, cp, frow, bp, lp, lrow, lcol;
   DevBuf fval[3], lval[3];  // by Dtype F64, F32, BF16 (built on demand)
   DevBuf prod;              // 8-byte product slots, n_far
   std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry (-1: padding)

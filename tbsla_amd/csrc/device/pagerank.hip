@@ -435,10 +435,12 @@ struct PbVal<dev::Pattern> {
 constexpr int kPbScatterThreads = 1024;
 
 struct PbScatterArgs {
-  const int *sp;        // far entries of superblock s: [sp[s], sp[s+1])
-  const unsigned *fkey; // column within the superblock | run << 16
-  const int *rp;        // runs of superblock s: [rp[s], rp[s+1])
-  const int *rdelta;    // slot - entry of each run
+  const int *sp;              // far entries of superblock s: [sp[s], sp[s+1])
+  const unsigned short *fkey; // column within the superblock | kPbRunStart
+  const int *rp;              // runs of superblock s: [rp[s], rp[s+1])
+  const int *rdelta;          // slot - entry of each run
+  const int *cbase;           // run of the first entry of each 64-entry chunk
+  const int *cp;              // chunks of superblock s: [cp[s], cp[s+1])
   int nsb, n;
 };
 
@@ -465,8 +467,9 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
   const int per = (gridDim.x + 7) / 8;
   const int sb = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (sb >= a.nsb) return;
-  const int k1 = a.sp[sb + 1];
+  const int k0 = a.sp[sb], k1 = a.sp[sb + 1];
   const int *__restrict__ delta = a.rdelta + a.rp[sb];  // slot - entry of each run of this superblock
+  const int *__restrict__ cbase = a.cbase + a.cp[sb];   // run of the first entry of each chunk
   const long c0 = static_cast<long>(sb) << kPbColShift;
   const int nc = static_cast<int>(min(static_cast<long>(kPbCols), static_cast<long>(a.n) - c0));
   {
@@ -498,26 +501,36 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
   }
   __syncthreads();
   // four entries per thread in flight: every global load first, then the
-  // run offsets and the LDS reads of x, then the products and the stores
-  for (int k = a.sp[sb] + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
+  // run offsets and the LDS reads of x, then the products and the stores.
+  // The lanes of a wavefront hold consecutive entries of one 64-entry chunk
+  // (kPbScatterThreads is a multiple of 64), so an entry's run is the
+  // chunk's first run plus the run starts on lanes 1..lane, counted from a
+  // ballot; lanes past the end are the chunk's last and hold no run start.
+  static_assert(kPbScatterThreads % kPbChunk == 0 && kPbChunk == 64, "one chunk per wavefront");
+  const int lane = threadIdx.x & 63;
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+  for (int k = k0 + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
     unsigned key[4];
-    int sl[4];
+    int sl[4], cb[4];
     AT v1[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int kk = k + u * kPbScatterThreads;
       const bool ok = kk < k1;
       key[u] = ok ? a.fkey[kk] : 0u;
+      cb[u] = ok ? cbase[(kk - k0) >> 6] : 0;
       v1[u] = ok ? PbVal<VT>::template prod<AT>(val, kk, AT(1)) : AT(0);
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int kk = k + u * kPbScatterThreads;
-      sl[u] = kk < k1 ? kk + delta[key[u] >> 16] : -1;
+      const unsigned long long starts = __ballot((key[u] & kPbRunStart) != 0u);
+      const int run = cb[u] + __popcll(starts & upto & ~1ull);
+      sl[u] = kk < k1 ? kk + delta[run] : -1;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (sl[u] >= 0) prod[sl[u]] = v1[u] * static_cast<AT>(xs[key[u] & 0xFFFFu]);
+      if (sl[u] >= 0) prod[sl[u]] = v1[u] * static_cast<AT>(xs[key[u] & static_cast<unsigned>(kPbCols - 1)]);
   }
 }
 
@@ -701,7 +714,8 @@ PbView pb_view(const PbMatrix &pb) {
 }
 
 PbScatterArgs pb_args(const PbMatrix &pb) {
-  return PbScatterArgs{pb.sp.as<int>(), pb.fkey.as<unsigned>(), pb.rp.as<int>(), pb.rdelta.as<int>(), pb.nsb, pb.n};
+  return PbScatterArgs{pb.sp.as<int>(),    pb.fkey.as<unsigned short>(), pb.rp.as<int>(), pb.rdelta.as<int>(),
+                       pb.cbase.as<int>(), pb.cp.as<int>(),              pb.nsb,          pb.n};
 }
 
 // One propagation-blocked product with the update (FP64 runs): values of
