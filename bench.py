@@ -11,12 +11,15 @@ host.
 
 Extra fields:
   formats        ELL / SCOO / COO / DENSE on the same block
-  cpu_baseline   the reference's own MatrixCSR::Ax (compiled from its sources)
-                 on the host cores, and the bit-exact check of the GPU y
+  cpu_baseline   the reference's own MatrixCSR::Ax and spmv() (compiled from
+                 its sources) on the host cores, best and median of 10, and
+                 the bit-exact check of the GPU y; its Matrix::page_rank on
+                 the PageRank graph, per iteration and to 1e-10
   pagerank       fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) in FP64, FP32 and
                  BF16 value storage, value-free FP64 and value-free FP32, each
-                 stopped by the sum-normalised bound, against the FP64 fixed
-                 point (median of three runs each)
+                 stopped by the sum-normalised bound (median of three runs
+                 each), against the FP64 fixed point p* computed on the host
+                 by the oracle (whose digest is the reference's own vector)
   batch          256 personalised graphs (n log-uniform in [2^10, 2^18]) plus
                  a 2^22-node graph with 64 one-hot teleport sets, one hipGraph
   adjoint        gradient of sum w_i p_i on fill_cqmat_stochastic(2^20, 2^20,
@@ -47,21 +50,29 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def time_kernel(torch, fn, steps, warmup):
+def time_kernel(torch, fn, steps, warmup, batches=1):
     """Average device time per call of fn() over `steps` calls, measured with
-    HIP events on torch's current stream (the stream the kernels run on)."""
+    HIP events on torch's current stream (the stream the kernels run on).
+    With batches > 1 the calls are split into that many consecutive groups
+    with an event between groups, and the per-call time of each group is
+    returned as well: (mean over all steps, [per-group means])."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(steps):
-        fn()
-    e1.record(s)
+    nb = max(1, min(batches, steps))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(nb + 1)]
+    sizes = [steps // nb + (1 if b < steps % nb else 0) for b in range(nb)]
+    ev[0].record(s)
+    for b in range(nb):
+        for _ in range(sizes[b]):
+            fn()
+        ev[b + 1].record(s)
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps
+    total = ev[0].elapsed_time(ev[nb]) / steps
+    if batches == 1:
+        return total
+    return total, [ev[b].elapsed_time(ev[b + 1]) / sizes[b] for b in range(nb)]
 
 
 def pmc_traffic(name):
@@ -75,28 +86,63 @@ def pmc_traffic(name):
     return None
 
 
+def _dump_csr(d, arrays, **meta):
+    """Raw CSR arrays and their sizes for the CPU-baseline child process."""
+    for k in ("values", "rowptr", "colidx"):
+        arrays[k].tofile(os.path.join(d, f"{k}.bin"))
+    meta.update({"n_values": int(arrays["values"].size), "n_rowptr": int(arrays["rowptr"].size),
+                 "n_colidx": int(arrays["colidx"].size)})
+    with open(os.path.join(d, "meta.json"), "w") as f:
+        json.dump(meta, f)
+
+
+def _cpu_child(task, d):
+    """Run oracle/cpu_baseline.py in a child process with the OpenMP
+    environment of a reference run (all the host threads this job may use,
+    OMP_PLACES=cores, OMP_PROC_BIND=close)."""
+    import subprocess
+    from oracle.cpu_baseline import host_cores, omp_env
+    cores = host_cores()
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", task, d], cwd=ROOT,
+                       env=omp_env(cores["threads"]), capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu_baseline {task} failed: {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res["cores"] = cores
+    return res
+
+
 def cpu_baseline(tb_csr, x, y_gpu, bytes_alg):
-    """The reference's own CSR Ax (oracle/_ref, -O3 -fopenmp) on the same
-    block, timed on the host cores; its result is also compared with the
-    GPU product (bit-exact)."""
+    """The reference's own CSR Ax and spmv() (oracle/_ref, compiled from its
+    sources, -O3 -fopenmp) on the same block, timed on the host cores in a
+    child process; its result is compared with the GPU product (bit-exact)."""
+    import hashlib
+    import shutil
+    import tempfile
     from oracle.oracle import Reference
     if not Reference.available():
         return None
-    a = tb_csr.arrays()
     info = tb_csr.info()
-    m = Reference.from_arrays("CSR", info["ln_row"], info["ln_col"], a["values"], a["rowptr"], a["colidx"])
-    del a
-    r = Reference.Ax_repeat(m, np.zeros(info["ln_row"]), x, 1)
-    exact = bool(np.array_equal(r, y_gpu))
-    reps = 10
-    t = time.perf_counter()
-    Reference.Ax_repeat(m, r, x, reps)
-    dt = (time.perf_counter() - t) / reps
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(bytes_alg / dt / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "reference",
-            "sample": f"{reps} x MatrixCSR::Ax on the full block (reference sources, g++ -O3 -fopenmp), "
-                      f"{dt * 1e3:.1f} ms each",
-            "gpu_y_bit_exact_vs_reference": exact}
+    d = tempfile.mkdtemp(prefix="tbsla_cpu_")
+    try:
+        _dump_csr(d, tb_csr.arrays(), n_row=info["ln_row"], n_col=info["ln_col"])
+        x.tofile(os.path.join(d, "x.bin"))
+        res = _cpu_child("spmv", d)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    ax, sp, cores = res["Ax"], res["spmv_api"], res["cores"]
+    gbs = lambda t: round(bytes_alg / t / 1e9, 2)  # noqa: E731
+    return {"value": gbs(ax["median_s"]), "unit": "GB/s", "cores": cores["threads"], "kind": "reference",
+            "sample": f"MatrixCSR::Ax on the full block, 1 warm-up + {ax['runs']} timed runs (median; best "
+                      f"{ax['best_s'] * 1e3:.1f} ms, median {ax['median_s'] * 1e3:.1f} ms), reference sources "
+                      "g++ -O3 -fopenmp, OMP_PLACES=cores OMP_PROC_BIND=close",
+            "host": cores,
+            "Ax": {"best_gbs": gbs(ax["best_s"]), "median_gbs": gbs(ax["median_s"]),
+                   "best_ms": round(ax["best_s"] * 1e3, 3), "median_ms": round(ax["median_s"] * 1e3, 3)},
+            "spmv_api": {"note": "spmv() returns a zeroed n_row-length vector (MatrixCSR.cpp:123-127)",
+                         "best_ms": round(sp["best_s"] * 1e3, 3), "median_ms": round(sp["median_s"] * 1e3, 3),
+                         "median_gbs": gbs(sp["median_s"])},
+            "gpu_y_bit_exact_vs_reference": res["y_sha256"] == hashlib.sha256(y_gpu.tobytes()).hexdigest()}
 
 
 def copy_calibration():
@@ -120,11 +166,16 @@ def leg_formats(torch, tb, dev, steps):
         fm[fmt] = {"us": round(ms2 * 1e3, 1), "gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1),
                    "bytes": mi2["algorithmic_bytes"]}
         if fmt in ("SCOO", "COO"):
-            # the kernel runs on the compressed-row mirror: 12 B per entry, 8 B per row
-            # segment, x once, y read and written
+            # the kernel reads the compressed-row mirror, not the format's
+            # arrays: 12 B per entry, 8 B per row segment, x once, y read and
+            # written.  Those are the bytes it must move, so they are the
+            # primary figure; the format's own bytes are kept beside it.
             mb = mi2["entries"] * 12 + mi2["rows"] * 8 + 8 * mi2["x_need"] + 16 * (mi2["y_hi"] - mi2["y_lo"])
-            fm[fmt]["mirror_bytes"] = mb
-            fm[fmt]["mirror_gbs"] = round(mb / ms2 / 1e6, 1)
+            fm[fmt] = {"us": fm[fmt]["us"], "gbs": round(mb / ms2 / 1e6, 1), "bytes": mb,
+                       "bytes_model": "compressed-row mirror read by the kernel",
+                       "format_bytes": mi2["algorithmic_bytes"],
+                       "format_bytes_gbs": round(mi2["algorithmic_bytes"] / ms2 / 1e6, 1)}
+        fm[fmt]["pct_hbm_peak"] = round(fm[fmt]["gbs"] / HBM_PEAK_GBS * 100, 2)
         log(f"{fmt} {ms2 * 1e3:.1f} us {fm[fmt]['gbs']} GB/s")
         del m, xx, yy
         torch.cuda.empty_cache()
@@ -139,13 +190,23 @@ def leg_formats(torch, tb, dev, steps):
     return fm
 
 
+PSTAR_ITERATIONS = 48   # the reference iteration is in its limit 2-cycle from ~40 iterations on
+CPU_PR_SAMPLE = 5       # reference page_rank iterations timed on the host
+
+
 def leg_pagerank(torch, tb, dev):
+    """Every precision to the sum-normalised 1e-10 bound (median of three
+    timed runs each), then the FP64 fixed point p* on the host (the oracle's
+    restatement of the reference iteration, 48 iterations; its SHA-256 is
+    checked against the reference's own vector) and the reference CPU
+    page_rank timed on a bounded sample of iterations."""
+    import hashlib
+    import shutil
+    import tempfile
     log("generating the PageRank graph")
     g = tb.MatrixCSR().fill_cqmat_stochastic(N, N, 8, 0.5, 7, device=True)
     n = g.get_n_col()
-    pstar = torch.zeros(n, dtype=torch.float64, device=dev)
-    g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
-    pr = {}
+    pr, outs = {}, {}
     for prec in ["f64", "f32", "bf16", "pattern", "pattern_f32"]:
         o = torch.zeros(n, dtype=torch.float64, device=dev)
         g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")  # warm
@@ -153,18 +214,55 @@ def leg_pagerank(torch, tb, dev):
         reps = [g.page_rank_device(o, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm")
                 for _ in range(3)]
         rep = sorted(reps, key=lambda r: r["seconds"])[1]
-        err = float((o - pstar).abs().sum().item())
-        pr[prec] = {"seconds": round(rep["seconds"], 6), "runs": 3, "iterations": rep["iterations"],
-                    "anchors": rep["anchors"], "l1_vs_fp64_fixed_point": err}
-        log(f"pagerank {prec}: {rep['seconds'] * 1e3:.2f} ms, {rep['iterations']} it, "
-            f"{rep['anchors']} anchors, L1 {err:.2e}")
+        outs[prec] = o
+        pr[prec] = {"seconds": round(rep["seconds"], 6), "runs": 3, "run_seconds": [round(r["seconds"], 6) for r in reps],
+                    "iterations": rep["iterations"], "anchors": rep["anchors"]}
+        log(f"pagerank {prec}: {rep['seconds'] * 1e3:.2f} ms, {rep['iterations']} it, {rep['anchors']} anchors")
     base = pr["f64"]["seconds"]
     for prec in pr:
         pr[prec]["speedup_vs_fp64"] = round(base / pr[prec]["seconds"], 3)
     # the value-free FP32 scheme against the value-free FP64 path as well
     pr["pattern_f32"]["speedup_vs_fp64_pattern"] = round(pr["pattern"]["seconds"] / pr["pattern_f32"]["seconds"], 3)
-    return {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7), beta 0.85",
-            "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr}
+    # the GPU's own FP64 limit: the reference iteration on fixed-order tiles
+    lim = torch.zeros(n, dtype=torch.float64, device=dev)
+    g.page_rank_device(lim, epsilon=0.0, max_iteration=PSTAR_ITERATIONS, precision="f64", rule="reference",
+                       gather="tiles")
+    out = {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7), beta 0.85",
+           "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr}
+    log("FP64 fixed point on the host (oracle) and the reference CPU page_rank")
+    d = tempfile.mkdtemp(prefix="tbsla_pr_")
+    try:
+        _dump_csr(d, g.arrays(), n=n, beta=0.85, pstar_iterations=PSTAR_ITERATIONS, sample_iterations=CPU_PR_SAMPLE)
+        res = _cpu_child("pagerank", d)
+        pstar = np.fromfile(os.path.join(d, "pstar.bin"), dtype=np.float64)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    with open(os.path.join(ROOT, "tests", "golden", "digests.json")) as f:
+        fix = json.load(f)["pagerank_cfg3"]
+    ps = torch.from_numpy(pstar).to(dev)
+    out["pstar"] = {"source": f"oracle restatement of Matrix.cpp:13-61, {PSTAR_ITERATIONS} iterations, host",
+                    "equals_reference_vector": res["pstar"]["sha256"] == fix[f"it{PSTAR_ITERATIONS}"]["sha256"]
+                    and hashlib.sha256(pstar.tobytes()).hexdigest() == res["pstar"]["sha256"],
+                    "host_seconds": round(res["pstar"]["seconds"], 2),
+                    "gpu_fp64_limit_l1": float((lim - ps).abs().sum().item())}
+    for prec, o in outs.items():
+        pr[prec]["l1_vs_oracle_pstar"] = float((o - ps).abs().sum().item())
+        log(f"pagerank {prec}: L1 vs oracle p* {pr[prec]['l1_vs_oracle_pstar']:.2e}")
+    if "reference" in res:
+        it_f64 = pr["f64"]["iterations"]
+        spi = res["reference"]["seconds_per_iteration"]
+        out["cpu_baseline"] = {
+            "kind": "reference", "cores": res["cores"]["threads"],
+            "sample": f"Matrix::page_rank (reference sources, -O3 -fopenmp) for {res['reference']['iterations']} "
+                      "iterations on the same graph, OMP_PLACES=cores OMP_PROC_BIND=close",
+            "seconds_per_iteration": round(spi, 4),
+            "iterations_to_1e-10": it_f64,
+            "seconds_to_1e-10": round(spi * it_f64, 3),
+            "note": "the reference's own stopping quantity never falls below ~2e-9 at this size (2-cycle); "
+                    "the FP64 iteration count of the sum-normalised bound is used for both",
+            "gpu_fp64_speedup": round(spi * it_f64 / pr["f64"]["seconds"], 1)}
+    del outs, ps, lim
+    return out
 
 
 def leg_batch(torch, tb, dev):
@@ -273,8 +371,9 @@ def main():
     y = torch.zeros(info["ln_row"], dtype=torch.float64, device=dev)
     csr.prepare("f64")
     mi = csr.mirror_info("f64")
-    ms = time_kernel(torch, lambda: csr.Ax_device(x, y), args.steps, args.warmup)
+    ms, per_batch = time_kernel(torch, lambda: csr.Ax_device(x, y), args.steps, args.warmup, batches=5)
     gbs = mi["algorithmic_bytes"] / (ms * 1e-3) / 1e9
+    batch_gbs = sorted(mi["algorithmic_bytes"] / (b * 1e-3) / 1e9 for b in per_batch)
     log(f"CSR {ms * 1e3:.1f} us/launch {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
     # the C-ABI host-vector path (x and y cross PCIe inside the call)
     t = time.perf_counter()
@@ -292,7 +391,11 @@ def main():
            "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("csr_tile_kernel_cqmat2^24_block"),
                         "kernel": "csr_tile_kernel<double,double,double>",
-                        "algorithmic_bytes_per_launch": mi["algorithmic_bytes"]}}
+                        "algorithmic_bytes_per_launch": mi["algorithmic_bytes"],
+                        "batches": {"n": len(batch_gbs), "min_gbs": round(batch_gbs[0], 1),
+                                    "median_gbs": round(batch_gbs[len(batch_gbs) // 2], 1),
+                                    "max_gbs": round(batch_gbs[-1], 1),
+                                    "median_frac": round(batch_gbs[len(batch_gbs) // 2] / HBM_PEAK_GBS, 4)}}}
 
     if not args.no_cpu:
         log("timing the reference CPU path")
@@ -303,6 +406,9 @@ def main():
         out["formats"] = leg_formats(torch, tb, dev, args.steps)
     if not args.no_pagerank:
         out["pagerank"] = leg_pagerank(torch, tb, dev)
+        cpu_pr = out["pagerank"].pop("cpu_baseline", None)
+        if out.get("cpu_baseline") and cpu_pr:
+            out["cpu_baseline"]["pagerank"] = cpu_pr
         torch.cuda.empty_cache()
     if not args.no_batch:
         out["batch"] = leg_batch(torch, tb, dev)
@@ -321,7 +427,11 @@ def main():
         best = max(mixed, key=mixed.get)
         tg["pagerank_mixed_speedup_vs_fp64"] = mixed
         tg["pagerank_mixed_best"] = {"variant": best, "speedup_vs_fp64": mixed[best],
-                                     "l1_vs_fp64_fixed_point": var[best]["l1_vs_fp64_fixed_point"]}
+                                     "l1_vs_oracle_pstar": var[best]["l1_vs_oracle_pstar"]}
+        stored = {k: v for k, v in mixed.items() if k in ("f32", "bf16")}
+        bs = max(stored, key=stored.get)
+        tg["pagerank_stored_values_best"] = {"variant": bs, "speedup_vs_fp64": stored[bs],
+                                             "l1_vs_oracle_pstar": var[bs]["l1_vs_oracle_pstar"]}
     out["targets"] = tg
 
     print(json.dumps(out), flush=True)

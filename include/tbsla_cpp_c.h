@@ -16,8 +16,11 @@
  *
  * Differences: an exception thrown by the library (for instance a vector
  * too short for the matrix, which the reference would read past) is caught
- * at this boundary; _spmv then returns an empty vector and the message is
- * available from tbsla_last_error() (tbsla_hip.h).
+ * at this boundary, and its message is available from tbsla_last_error()
+ * (tbsla_hip.h).  _spmv clears that message on entry and returns an empty
+ * vector on failure; _fill_* leave the matrix as the failed call left it;
+ * _read and _write return false, and a truncated or corrupt file leaves the
+ * matrix unchanged (the reference reads past the end of such a file).
  */
 #ifndef TBSLA_CPP_C_H
 #define TBSLA_CPP_C_H

@@ -115,8 +115,9 @@ typedef struct {
   double anchor_tau; /* 0: default */
   const int *nodes;  /* personalised teleport set, or NULL */
   int n_nodes;
-  int gather;        /* CSR products: 0 auto (propagation-blocked for rule 1 and n >= 2^16),
-                        1 fixed-order entry tiles, 2 propagation-blocked */
+  int gather;        /* CSR products: 0 auto (propagation-blocked for n >= 2^16),
+                        1 fixed-order entry tiles (the reference's row sums),
+                        2 propagation-blocked */
 } tbsla_pagerank_params;
 
 typedef struct {
@@ -127,9 +128,10 @@ typedef struct {
   double seconds;
 } tbsla_pagerank_report;
 
-/* Sum-normalised PageRank vector (n_col doubles) into device memory. */
-int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, void *stream,
-                        tbsla_pagerank_report *report);
+/* Sum-normalised PageRank vector (n_col doubles) into device memory;
+ * TBSLA_EINVAL, before any launch, when n_out < n_col. */
+int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, size_t n_out,
+                        void *stream, tbsla_pagerank_report *report);
 
 /* PageRank with explicit device inputs (CSR, FP64): values_dev (n_values =
  * the stored entry count, CSR entry order) replaces the stored edge values,
@@ -139,7 +141,7 @@ int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double 
  * counterpart: the forward pass of the differentiable PageRank. */
 int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, const double *values_dev,
                            size_t n_values, const double *teleport_dev, size_t n_teleport, double *out_dev,
-                           void *stream, tbsla_pagerank_report *report);
+                           size_t n_out, void *stream, tbsla_pagerank_report *report);
 
 typedef struct {
   double beta;

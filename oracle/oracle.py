@@ -367,6 +367,8 @@ class _Reference(_Backend):
         L.ref_serialize.restype = C.c_void_p
         L.ref_serialize.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t)]
         L.ref_blob_free.argtypes = [C.c_void_p]
+        L.ref_deserialize.restype = C.c_void_p
+        L.ref_deserialize.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
         L.ref_get_nnz.restype = C.c_long
         L.ref_get_nnz.argtypes = [C.c_void_p]
         L.ref_get_n_row.restype = C.c_int
@@ -400,6 +402,24 @@ class _Reference(_Backend):
         fmt = FORMATS[fmt] if isinstance(fmt, str) else fmt
         return Mat(self, self.lib.ref_convert(coo.h, fmt), fmt)
 
+    def serialize(self, m: Mat) -> bytes:
+        """The bytes the reference's write() produces for m."""
+        return self._serialize(m.h)
+
+    def _serialize(self, h) -> bytes:
+        p = C.c_void_p()
+        n = C.c_size_t()
+        blob = self.lib.ref_serialize(h, C.byref(p), C.byref(n))
+        try:
+            return C.string_at(p, n.value)
+        finally:
+            self.lib.ref_blob_free(blob)
+
+    def deserialize(self, fmt, data: bytes) -> Mat:
+        """A matrix read by the reference's read() from `data`."""
+        fmt = FORMATS[fmt] if isinstance(fmt, str) else fmt
+        return Mat(self, self.lib.ref_deserialize(fmt, data, len(data)), fmt)
+
     def _free(self, h):
         self.lib.ref_free(h)
 
@@ -422,14 +442,7 @@ class _Reference(_Backend):
                 "nnz": self.lib.ref_get_nnz(h)}
 
     def _arrays(self, h, fmt):
-        p = C.c_void_p()
-        n = C.c_size_t()
-        blob = self.lib.ref_serialize(h, C.byref(p), C.byref(n))
-        try:
-            raw = C.string_at(p, n.value)
-        finally:
-            self.lib.ref_blob_free(blob)
-        info, arrs, _ = decode_binary(fmt, raw)
+        info, arrs, _ = decode_binary(fmt, self._serialize(h))
         return info, arrs
 
     def _spmv(self, h, x, vi, out):

@@ -99,6 +99,15 @@ void *ref_serialize(void *m, const char **data, size_t *size) {
 }
 void ref_blob_free(void *b) { delete static_cast<Blob *>(b); }
 
+// Read one matrix with the reference's read() (MatrixCSR.cpp:206-232 and
+// siblings) from `size` bytes at `data`; returns the new matrix.
+void *ref_deserialize(int fmt, const char *data, size_t size) {
+  Matrix *m = make(fmt);
+  std::istringstream is(std::string(data, size), std::ios::binary);
+  m->read(is);
+  return m;
+}
+
 long ref_get_nnz(void *m) { return static_cast<Matrix *>(m)->get_nnz(); }
 int ref_get_n_row(void *m) { return static_cast<Matrix *>(m)->get_n_row(); }
 int ref_get_n_col(void *m) { return static_cast<Matrix *>(m)->get_n_col(); }

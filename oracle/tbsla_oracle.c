@@ -698,41 +698,75 @@ void orc_a_axpx(const orc_mat *m, const double *v, int vect_incr, double *out) {
   free(r);
 }
 
-/* Matrix.cpp:13-61 (square matrices; n_row == n_col) */
-int orc_page_rank(const orc_mat *m, double epsilon, double beta, int max_iteration,
-                  double *out, double *last_error) {
-  const int n = m->n_col;
-  double *b = out;
-  double *bt = (double *)malloc(((size_t)n ? (size_t)n : 1) * sizeof(double));
-  for (int i = 0; i < n; i++) b[i] = 1.0;
-  int converge = 0, it = 0;
-  double max, error = 0;
+/* Matrix.cpp:13-114.  Per iteration (reference order):
+ *   b_t = b; b = A b_t; tele = (1-beta)/n * sum(b_t)        (personalised:
+ *   (1-beta)/|nodes| * sum(b_t), added to listed nodes only); b = beta*b + tele;
+ *   max = max(b); b /= max; err = sum|b - b_t|; stop when err < epsilon;
+ *   at the end b /= sum(b).
+ * The arithmetic of every element and the order of every sum that feeds the
+ * iterate (sum(b_t), sum(b)) or the stopping test (err) is the reference's:
+ * those sums run serially in ascending i.  Elementwise steps and the max
+ * (exact whatever the order) run in parallel, and sum(b_t) of the next
+ * iteration is accumulated in the same serial loop as err, so the iterate is
+ * bit-identical to the reference's at a fraction of its serial passes. */
+static int orc_pr_loop(const orc_mat *m, double epsilon, double beta, int max_iteration,
+                       const char *member, double tele_div, double b0, double *out,
+                       double *last_error) {
+  const long n = m->n_col;
+  double *buf[2];
+  buf[0] = out;
+  buf[1] = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+  double *b = buf[0];
+  for (long i = 0; i < n; i++) b[i] = b0;
+  double sum_bt = 0.0;                       /* sum(b_t) of the next iteration */
+  for (long i = 0; i < n; i++) sum_bt += b[i];
+  int converge = 0, it = 0, cur = 0;
+  double error = 0;
   while (!converge && it < max_iteration) {
-    double tele = 0.0;
-    memcpy(bt, b, (size_t)n * sizeof(double));
-    orc_spmv(m, bt, 0, b);
-    for (int i = 0; i < n; i++) tele += bt[i];
-    tele *= (1 - beta) / n;
-    b[0] = beta * b[0] + tele;
-    max = b[0];
-    for (int i = 1; i < n; i++) {
-      b[i] = beta * b[i] + tele;
-      if (max < b[i]) max = b[i];
+    double *bt = buf[cur], *nb = buf[cur ^ 1];
+    orc_spmv(m, bt, 0, nb);
+    const double tele = sum_bt * ((1 - beta) / tele_div);
+    double max = -INFINITY;
+    if (member) {
+      #pragma omp parallel for schedule(static) reduction(max:max)
+      for (long i = 0; i < n; i++) {
+        nb[i] = beta * nb[i];
+        if (member[i]) nb[i] += tele;
+        if (max < nb[i]) max = nb[i];
+      }
+    } else {
+      #pragma omp parallel for schedule(static) reduction(max:max)
+      for (long i = 0; i < n; i++) {
+        nb[i] = beta * nb[i] + tele;
+        if (max < nb[i]) max = nb[i];
+      }
     }
+    #pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) nb[i] = nb[i] / max;
     error = 0.0;
-    for (int i = 0; i < n; i++) {
-      b[i] = b[i] / max;
-      error += fabs(b[i] - bt[i]);
+    sum_bt = 0.0;
+    for (long i = 0; i < n; i++) {
+      error += fabs(nb[i] - bt[i]);
+      sum_bt += nb[i];
     }
     if (error < epsilon) converge = 1;
     it++;
+    cur ^= 1;
   }
-  double sum = b[0];
-  for (int i = 1; i < n; i++) sum += b[i];
-  for (int i = 0; i < n; i++) b[i] = b[i] / sum;
+  b = buf[cur];
+  /* final sum normalisation: the same serial sum as sum(b_t) above */
+  const double sum = sum_bt;
+  #pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; i++) out[i] = b[i] / sum;
   if (last_error) *last_error = error;
-  free(bt);
+  free(buf[1]);
   return it;
+}
+
+/* Matrix.cpp:13-61 (square matrices; n_row == n_col) */
+int orc_page_rank(const orc_mat *m, double epsilon, double beta, int max_iteration,
+                  double *out, double *last_error) {
+  return orc_pr_loop(m, epsilon, beta, max_iteration, NULL, (double)m->n_col, 1.0, out, last_error);
 }
 
 /* Matrix.cpp:63-114 */
@@ -740,38 +774,11 @@ int orc_personalized_page_rank(const orc_mat *m, double epsilon, double beta,
                                int max_iteration, const int *nodes, int n_nodes,
                                double *out, double *last_error) {
   const int n = m->n_col;
-  double *b = out;
-  double *bt = (double *)malloc(((size_t)n ? (size_t)n : 1) * sizeof(double));
   char *member = (char *)calloc((size_t)n ? (size_t)n : 1, 1);
   for (int k = 0; k < n_nodes; k++) if (nodes[k] >= 0 && nodes[k] < n) member[nodes[k]] = 1;
-  for (int i = 0; i < n; i++) b[i] = 0.25;
-  int converge = 0, it = 0;
-  double max, error = 0;
-  while (!converge && it < max_iteration) {
-    double tele = 0.0;
-    memcpy(bt, b, (size_t)n * sizeof(double));
-    orc_spmv(m, bt, 0, b);
-    for (int i = 0; i < n; i++) tele += bt[i];
-    tele *= (1 - beta) / (double)(size_t)n_nodes;
-    for (int i = 0; i < n; i++) {
-      b[i] = beta * b[i];
-      if (member[i]) b[i] += tele;
-    }
-    max = b[0];
-    for (int i = 1; i < n; i++) if (max < b[i]) max = b[i];
-    error = 0.0;
-    for (int i = 0; i < n; i++) {
-      b[i] = b[i] / max;
-      error += fabs(b[i] - bt[i]);
-    }
-    if (error < epsilon) converge = 1;
-    it++;
-  }
-  double sum = b[0];
-  for (int i = 1; i < n; i++) sum += b[i];
-  for (int i = 0; i < n; i++) b[i] = b[i] / sum;
-  if (last_error) *last_error = error;
-  free(bt); free(member);
+  const int it = orc_pr_loop(m, epsilon, beta, max_iteration, member, (double)(size_t)n_nodes, 0.25, out,
+                             last_error);
+  free(member);
   return it;
 }
 

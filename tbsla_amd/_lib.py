@@ -92,8 +92,8 @@ SIGNATURES = {
     "tbsla_hip_mirror_info": (_i, [_vp, _i, C.POINTER(_l)]),
     "tbsla_hip_Ax": (_i, [_vp, _i, _vp, _sz, _i, _vp, _sz, _i, _i, _vp]),
     "tbsla_hip_fill": (_i, [_vp, _i, _sz, _d, _vp]),
-    "tbsla_hip_page_rank": (_i, [_vp, C.POINTER(PageRankParams), _dp, _vp, C.POINTER(PageRankReport)]),
-    "tbsla_hip_page_rank_ex": (_i, [_vp, C.POINTER(PageRankParams), _vp, _sz, _vp, _sz, _vp, _vp,
+    "tbsla_hip_page_rank": (_i, [_vp, C.POINTER(PageRankParams), _dp, _sz, _vp, C.POINTER(PageRankReport)]),
+    "tbsla_hip_page_rank_ex": (_i, [_vp, C.POINTER(PageRankParams), _vp, _sz, _vp, _sz, _vp, _sz, _vp,
                                     C.POINTER(PageRankReport)]),
     "tbsla_hip_page_rank_adjoint": (_i, [_vp, C.POINTER(AdjointParams), _vp, _sz, _vp, _sz, _vp, _vp, _sz, _vp,
                                          _vp, _vp, C.POINTER(AdjointReport)]),

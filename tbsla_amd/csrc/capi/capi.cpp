@@ -130,8 +130,7 @@ bool C_CPP_Vector_read(C_CPP_Vector_t *v, char *filename, int seek) {
   std::ifstream is(std::string(filename), std::ifstream::binary);
   if (!is.is_open()) return false;
   is.seekg(seek, is.beg);
-  host::read_array(is, *vec(v));
-  return true;
+  return guarded([&] { host::read_array(is, *vec(v)); }) == TBSLA_OK;
 }
 // Vector.cpp:90-103 (appends)
 bool C_CPP_Vector_write(C_CPP_Vector_t *v, char *filename) {
@@ -139,7 +138,7 @@ bool C_CPP_Vector_write(C_CPP_Vector_t *v, char *filename) {
   std::ofstream os(std::string(filename), std::ofstream::binary | std::ofstream::app);
   if (!os.is_open()) return false;
   host::write_array(os, *vec(v));
-  return true;
+  return static_cast<bool>(os);
 }
 // Vector.cpp:105-114
 bool C_CPP_Vector_add(C_CPP_Vector_t *v1, C_CPP_Vector_t *v2) {
@@ -185,27 +184,28 @@ bool C_CPP_Vector_gather(C_CPP_Vector_t *v1, C_CPP_Vector_t *v2) {
   extern "C" void C_CPP_Matrix##F##_fill_cdiag(C_CPP_Matrix##F##_t *m, int n_row, int n_col, int cdiag, int pr,  \
                                                int pc, int NR, int NC) {                                       \
     if (m == NULL) return;                                                                                     \
-    static_cast<cpp::Matrix##F *>(m->obj)->fill_cdiag(n_row, n_col, cdiag, pr, pc, NR, NC);                    \
+    guarded([&] { static_cast<cpp::Matrix##F *>(m->obj)->fill_cdiag(n_row, n_col, cdiag, pr, pc, NR, NC); });  \
   }                                                                                                            \
   extern "C" void C_CPP_Matrix##F##_fill_cqmat(C_CPP_Matrix##F##_t *m, int n_row, int n_col, int c, double q,    \
                                                unsigned int seed_mult, int pr, int pc, int NR, int NC) {       \
     if (m == NULL) return;                                                                                     \
-    static_cast<cpp::Matrix##F *>(m->obj)->fill_cqmat(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC);          \
+    guarded([&] { static_cast<cpp::Matrix##F *>(m->obj)->fill_cqmat(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC); }); \
   }                                                                                                            \
   extern "C" bool C_CPP_Matrix##F##_read(C_CPP_Matrix##F##_t *m, char *filename, int seek) {                   \
     if (m == NULL) return false;                                                                               \
     std::ifstream is(std::string(filename), std::ifstream::binary);                                            \
     if (!is.is_open()) return false;                                                                           \
     is.seekg(seek, is.beg);                                                                                    \
-    static_cast<cpp::Matrix##F *>(m->obj)->read(is);                                                           \
-    return true;                                                                                               \
+    return guarded([&] { static_cast<cpp::Matrix##F *>(m->obj)->read(is); }) == TBSLA_OK;                      \
   }                                                                                                            \
   extern "C" bool C_CPP_Matrix##F##_write(C_CPP_Matrix##F##_t *m, char *filename) {                            \
     if (m == NULL) return false;                                                                               \
     std::ofstream os(std::string(filename), std::ofstream::binary | std::ofstream::app);                       \
     if (!os.is_open()) return false;                                                                           \
-    static_cast<cpp::Matrix##F *>(m->obj)->write(os);                                                          \
-    return true;                                                                                               \
+    return guarded([&] {                                                                                       \
+             static_cast<cpp::Matrix##F *>(m->obj)->write(os);                                                 \
+             if (!os) throw std::runtime_error(std::string("tbsla: write to ") + filename + " failed");        \
+           }) == TBSLA_OK;                                                                                     \
   }                                                                                                            \
   extern "C" void C_CPP_Matrix##F##_print(C_CPP_Matrix##F##_t *m) {                                            \
     if (m == NULL) return;                                                                                     \
@@ -215,6 +215,7 @@ bool C_CPP_Vector_gather(C_CPP_Vector_t *v1, C_CPP_Vector_t *v2) {
   }                                                                                                            \
   extern "C" C_CPP_Vector_t *C_CPP_Matrix##F##_spmv(C_CPP_Matrix##F##_t *m, C_CPP_Vector_t *v) {               \
     C_CPP_Vector_t *r = C_CPP_Vector_create();                                                                 \
+    g_last_error.clear();                                                                                      \
     if (m == NULL || v == NULL) return r;                                                                      \
     guarded([&] {                                                                                              \
       std::vector<double> y = static_cast<cpp::Matrix##F *>(m->obj)->spmv(*vec(v), 0);                         \
@@ -273,7 +274,7 @@ Arrays arrays_of(const cpp::Matrix &m) {
 
 extern "C" {
 
-int tbsla_abi_version(void) { return 1; }
+int tbsla_abi_version(void) { return 2; }
 const char *tbsla_last_error(void) { return g_last_error.c_str(); }
 
 tbsla_matrix *tbsla_matrix_create(int format) {
@@ -543,9 +544,9 @@ int tbsla_hip_fill(void *y, int y_dtype, size_t n, double value, void *stream) {
   });
 }
 
-int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, void *stream,
-                        tbsla_pagerank_report *report) {
-  return tbsla_hip_page_rank_ex(m, p, nullptr, 0, nullptr, 0, out_dev, stream, report);
+int tbsla_hip_page_rank(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, size_t n_out,
+                        void *stream, tbsla_pagerank_report *report) {
+  return tbsla_hip_page_rank_ex(m, p, nullptr, 0, nullptr, 0, out_dev, n_out, stream, report);
 }
 
 namespace {
@@ -591,9 +592,12 @@ int tbsla_hip_page_rank_adjoint(tbsla_matrix *m, const tbsla_adjoint_params *p, 
 
 int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, const double *values_dev,
                            size_t n_values, const double *teleport_dev, size_t n_teleport, double *out_dev,
-                           void *stream, tbsla_pagerank_report *report) {
+                           size_t n_out, void *stream, tbsla_pagerank_report *report) {
   if (!m || !p || !out_dev) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
   return guarded([&] {
+    if (n_out < static_cast<size_t>(m->m->get_n_col()))
+      throw std::invalid_argument("tbsla: out has " + std::to_string(n_out) + " entries, PageRank writes " +
+                                  std::to_string(m->m->get_n_col()));
     if (values_dev && n_values < csr_entries(*m->m))
       throw std::invalid_argument("tbsla: values shorter than the stored entries");
     if (teleport_dev && n_teleport < static_cast<size_t>(m->m->get_n_col()))

@@ -859,11 +859,11 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   if (p.precision == PrPrecision::F32) dlo = &mirror(m, Dtype::F32);
   if (p.precision == PrPrecision::BF16) dlo = &mirror(m, Dtype::BF16);
   const bool fused = csr && !d64.noop && d64.rows == n;
-  // propagation-blocked products: explicit choice, or by default for the
-  // sum-normalised rule on large graphs (the reference rule keeps the
-  // fixed-order tile products)
-  const bool use_pb = fused && !p.values &&
-                      (p.gather == 2 || (p.gather == 0 && p.rule == PrRule::SUMNORM && n >= (1L << 16)));
+  // propagation-blocked products: explicit choice, or by default on graphs
+  // of 2^16 nodes and more, whatever the stopping rule (their row sums are
+  // rounded in a run-dependent order; gather = 1 keeps the fixed-order tile
+  // products, whose sums are the reference's)
+  const bool use_pb = fused && !p.values && (p.gather == 2 || (p.gather == 0 && n >= (1L << 16)));
   const PbMatrix *pbm = nullptr;
   if (use_pb) {
     const bool mixed_pat = mixed && m.count_stochastic();
@@ -908,7 +908,16 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     std::vector<double> cs(static_cast<std::size_t>(n), 0.0);
     const auto &ci = csrm.get_colidx();
     const auto &va = csrm.get_values();
-    for (std::size_t k = 0; k < va.size(); k++) cs[static_cast<std::size_t>(ci[k])] = va[k];
+    const auto &rp = csrm.get_rowptr();
+    // the stored entries the products read: [0, rowptr[rows] - rowptr[0])
+    const std::size_t used = rp.size() < 2 ? 0 : static_cast<std::size_t>(rp.back() - rp.front());
+    if (used > ci.size() || used > va.size())
+      throw std::invalid_argument("tbsla: CSR colidx / values shorter than rowptr says");
+    for (std::size_t k = 0; k < used; k++) {
+      const int c = ci[k];
+      if (c < 0 || c >= n) throw std::invalid_argument("tbsla: column index outside [0, n_col)");
+      cs[static_cast<std::size_t>(c)] = va[k];
+    }
     w.colscale.upload(cs.data(), cs.size() * sizeof(double), s);
     if (pat32) {
       std::vector<float> c32(cs.begin(), cs.end());

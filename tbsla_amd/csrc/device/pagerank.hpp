@@ -57,8 +57,8 @@ struct PageRankParams {
   //             (1-beta) * sum(b_t) * v[i], i.e. M = beta*A + (1-beta) v 1^T
   const double *values = nullptr;
   const double *teleport = nullptr;
-  // CSR product structure: 0 = propagation-blocked for the SUMNORM rule on
-  // n >= 2^16 (else tiles), 1 = fixed-order entry tiles, 2 = propagation-blocked
+  // CSR product structure: 0 = propagation-blocked on n >= 2^16 (else
+  // tiles), 1 = fixed-order entry tiles, 2 = propagation-blocked
   int gather = 0;
 };
 

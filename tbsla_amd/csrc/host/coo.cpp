@@ -146,10 +146,22 @@ std::ostream &MatrixCOO::write(std::ostream &os) {
 }
 
 std::istream &MatrixCOO::read(std::istream &is, std::size_t, std::size_t) {
-  host::read_header(is, n_row, n_col, ln_row, ln_col, f_row, f_col, nnz, pr, pc, NR, NC);
-  host::read_array(is, values);
-  host::read_array(is, row);
-  host::read_array(is, col);
+  // read into temporaries: a truncated or corrupt file throws and leaves
+  // the matrix as it was
+  int h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_pr, h_pc, h_NR, h_NC;
+  long h_nnz;
+  host::read_header(is, h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_nnz, h_pr, h_pc, h_NR, h_NC);
+  std::vector<double> r_values;
+  host::read_array(is, r_values);
+  std::vector<int> r_row;
+  host::read_array(is, r_row);
+  std::vector<int> r_col;
+  host::read_array(is, r_col);
+  n_row = h_nr; n_col = h_nc; ln_row = h_lnr; ln_col = h_lnc; f_row = h_fr; f_col = h_fc;
+  nnz = h_nnz; pr = h_pr; pc = h_pc; NR = h_NR; NC = h_NC;
+  values.swap(r_values);
+  row.swap(r_row);
+  col.swap(r_col);
   count_stochastic_ = false;
   invalidate_device();
   return is;
@@ -421,10 +433,22 @@ std::ostream &MatrixSCOO::write(std::ostream &os) {
 }
 
 std::istream &MatrixSCOO::read(std::istream &is, std::size_t, std::size_t) {
-  host::read_header(is, n_row, n_col, ln_row, ln_col, f_row, f_col, nnz, pr, pc, NR, NC);
-  host::read_array(is, values);
-  host::read_array(is, row);
-  host::read_array(is, col);
+  // read into temporaries: a truncated or corrupt file throws and leaves
+  // the matrix as it was
+  int h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_pr, h_pc, h_NR, h_NC;
+  long h_nnz;
+  host::read_header(is, h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_nnz, h_pr, h_pc, h_NR, h_NC);
+  std::vector<double> r_values;
+  host::read_array(is, r_values);
+  std::vector<int> r_row;
+  host::read_array(is, r_row);
+  std::vector<int> r_col;
+  host::read_array(is, r_col);
+  n_row = h_nr; n_col = h_nc; ln_row = h_lnr; ln_col = h_lnc; f_row = h_fr; f_col = h_fc;
+  nnz = h_nnz; pr = h_pr; pc = h_pc; NR = h_NR; NC = h_NC;
+  values.swap(r_values);
+  row.swap(r_row);
+  col.swap(r_col);
   count_stochastic_ = false;
   invalidate_device();
   return is;

@@ -132,10 +132,23 @@ std::ostream &MatrixELL::write(std::ostream &os) {
 }
 
 std::istream &MatrixELL::read(std::istream &is, std::size_t, std::size_t) {
-  host::read_header(is, n_row, n_col, ln_row, ln_col, f_row, f_col, nnz, pr, pc, NR, NC);
-  is.read(reinterpret_cast<char *>(&max_col), sizeof(max_col));
-  host::read_array(is, values);
-  host::read_array(is, columns);
+  // read into temporaries: a truncated or corrupt file throws and leaves
+  // the matrix as it was
+  int h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_pr, h_pc, h_NR, h_NC;
+  long h_nnz;
+  host::read_header(is, h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_nnz, h_pr, h_pc, h_NR, h_NC);
+  int h_max_col = 0;
+  is.read(reinterpret_cast<char *>(&h_max_col), sizeof(h_max_col));
+  if (!is) throw std::runtime_error("tbsla: truncated binary file (ELL max_col)");
+  std::vector<double> r_values;
+  host::read_array(is, r_values);
+  std::vector<int> r_columns;
+  host::read_array(is, r_columns);
+  n_row = h_nr; n_col = h_nc; ln_row = h_lnr; ln_col = h_lnc; f_row = h_fr; f_col = h_fc;
+  nnz = h_nnz; pr = h_pr; pc = h_pc; NR = h_NR; NC = h_NC;
+  max_col = h_max_col;
+  values.swap(r_values);
+  columns.swap(r_columns);
   count_stochastic_ = false;
   invalidate_device();
   return is;
@@ -340,8 +353,16 @@ std::ostream &MatrixDENSE::write(std::ostream &os) {
 }
 
 std::istream &MatrixDENSE::read(std::istream &is, std::size_t, std::size_t) {
-  host::read_header(is, n_row, n_col, ln_row, ln_col, f_row, f_col, nnz, pr, pc, NR, NC);
-  host::read_array(is, values);
+  // read into temporaries: a truncated or corrupt file throws and leaves
+  // the matrix as it was
+  int h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_pr, h_pc, h_NR, h_NC;
+  long h_nnz;
+  host::read_header(is, h_nr, h_nc, h_lnr, h_lnc, h_fr, h_fc, h_nnz, h_pr, h_pc, h_NR, h_NC);
+  std::vector<double> r_values;
+  host::read_array(is, r_values);
+  n_row = h_nr; n_col = h_nc; ln_row = h_lnr; ln_col = h_lnc; f_row = h_fr; f_col = h_fc;
+  nnz = h_nnz; pr = h_pr; pc = h_pc; NR = h_NR; NC = h_NC;
+  values.swap(r_values);
   count_stochastic_ = false;
   invalidate_device();
   return is;

@@ -158,6 +158,7 @@ void read_header(std::istream &is, int &n_row, int &n_col, int &ln_row, int &ln_
   is.read(reinterpret_cast<char *>(h1), sizeof(h1));
   is.read(reinterpret_cast<char *>(&nnz), sizeof(nnz));
   is.read(reinterpret_cast<char *>(h2), sizeof(h2));
+  if (!is) throw std::runtime_error("tbsla: truncated binary file (header)");
   n_row = h1[0]; n_col = h1[1]; ln_row = h1[2]; ln_col = h1[3]; f_row = h1[4]; f_col = h1[5];
   pr = h2[0]; pc = h2[1]; NR = h2[2]; NC = h2[3];
 }

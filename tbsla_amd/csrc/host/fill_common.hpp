@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <iomanip>
 #include <iostream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -96,12 +97,25 @@ void write_array(std::ostream &os, const std::vector<T> &v) {
   os.write(reinterpret_cast<const char *>(&n), sizeof(n));
   os.write(reinterpret_cast<const char *>(v.data()), static_cast<std::streamsize>(n * sizeof(T)));
 }
+// A size that the rest of a seekable stream cannot hold (a truncated or
+// corrupt file) throws std::runtime_error before anything is allocated.
 template <class T>
 void read_array(std::istream &is, std::vector<T> &v) {
   std::size_t n = 0;
   is.read(reinterpret_cast<char *>(&n), sizeof(n));
+  if (!is) throw std::runtime_error("tbsla: truncated binary file (array size missing)");
+  const std::streampos here = is.tellg();
+  if (here != std::streampos(-1)) {
+    is.seekg(0, std::ios::end);
+    const std::streamoff left = is.tellg() - here;
+    is.seekg(here);
+    if (left < 0 || n > static_cast<std::size_t>(left) / sizeof(T))
+      throw std::runtime_error("tbsla: binary file holds " + std::to_string(left) + " bytes after an array size of " +
+                               std::to_string(n) + " elements");
+  }
   v.resize(n);
   is.read(reinterpret_cast<char *>(v.data()), static_cast<std::streamsize>(n * sizeof(T)));
+  if (!is) throw std::runtime_error("tbsla: truncated binary file");
 }
 
 }  // namespace host

@@ -218,9 +218,9 @@ class Matrix:
         ``values`` (float64 device tensor, one per stored CSR entry) replaces
         the stored edge values; ``teleport`` (float64 device tensor, n_col)
         is the teleport distribution v of b = beta*A b + (1-beta)(1^T b) v.
-        ``gather``: "auto", "tiles" (fixed-order entry tiles) or "blocked"
-        (propagation-blocked products; auto picks them for the sumnorm rule
-        on n >= 2^16)."""
+        ``gather``: "auto", "tiles" (fixed-order entry tiles, the
+        reference's row sums) or "blocked" (propagation-blocked products;
+        auto picks them on n >= 2^16)."""
         p = _lib.PageRankParams()
         p.epsilon, p.beta, p.max_iteration = epsilon, beta, max_iteration
         p.precision = dtype_code(precision)
@@ -234,11 +234,13 @@ class Matrix:
             p.nodes = iptr(keep)
             p.n_nodes = keep.size
         _check_f64(out, "out")
+        if out.numel() < self.get_n_col():
+            raise ValueError(f"out has {out.numel()} entries, PageRank writes {self.get_n_col()}")
         vals_p, vals_n = _opt_f64(values, "values")
         tele_p, tele_n = _opt_f64(teleport, "teleport")
         rep = _lib.PageRankReport()
         check(lib.tbsla_hip_page_rank_ex(self._h, C.byref(p), vals_p, vals_n, tele_p, tele_n, _tensor_ptr(out),
-                                         _stream_handle(stream), C.byref(rep)))
+                                         out.numel(), _stream_handle(stream), C.byref(rep)))
         del keep
         return {"iterations": rep.iterations, "anchors": rep.anchors, "last_error": rep.last_error,
                 "bound": rep.bound, "seconds": rep.seconds}

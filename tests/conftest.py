@@ -56,6 +56,22 @@ def gpu():
     return torch.device("cuda:0")
 
 
+class single_thread_omp:
+    """Run the OpenMP regions started from this thread with one thread, so
+    the reference's COO/SCOO vector reductions (MatrixCOO.cpp:109-112,
+    MatrixSCOO.cpp:116-119) take their sequential order."""
+
+    def __enter__(self):
+        import ctypes
+        self.gomp = ctypes.CDLL("libgomp.so.1")
+        self.prev = self.gomp.omp_get_max_threads()
+        self.gomp.omp_set_num_threads(1)
+        return self
+
+    def __exit__(self, *exc):
+        self.gomp.omp_set_num_threads(self.prev)
+
+
 def grid_q():
     """q values of the reference test grid (tests/test_spmv_cqmat.cpp:155)."""
     out, q = [], 0.0

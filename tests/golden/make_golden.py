@@ -12,6 +12,13 @@ Writes, next to this script:
                            tests/test_spmv_cdiag.cpp) and, with ``--big``, for
                            the full-size workloads (one rank's block of the
                            n=2^24 matrix, the PageRank graphs, ...).
+* ``ref_binary/``       -- small matrix files written by the reference's own
+                           write() (MatrixCSR.cpp:179-204 and siblings), one
+                           per format and case, for file interchange tests.
+* with ``--cfg3``: the FP64 PageRank limit of
+  fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7), beta 0.85 -- the reference
+  iteration run into its 2-cycle, both cycle states as SHA-256 digests and
+  sampled entries (digests.json "pagerank_cfg3", golden_small.npz).
 
 Every number here comes from the reference itself (oracle/_ref), never from the
 restatement under test.  OMP_NUM_THREADS=1 makes the COO/SCOO vector reduction
@@ -190,23 +197,98 @@ def pagerank_big(vectors):
     return out
 
 
+BINARY_CASES = {
+    # name: (generator, args)  -- reference generators only (values 1.0 or 1/count)
+    "cqmat40c6q0.3s2_block1.0.2.2": ("fill_cqmat", (40, 40, 6, 0.3, 2, 1, 0, 2, 2)),
+    "stoch50c5q0.5s3": ("fill_cqmat_stochastic", (50, 50, 5, 0.5, 3)),
+    "cdiag30c4_block1.2.2.3": ("fill_cdiag", (30, 30, 4, 1, 2, 2, 3)),
+}
+
+
+def binary_files(vectors):
+    """Files written by the reference's write(), and its SpMV of each (x =
+    0, 1, 2, ... over the x the format reads), for the interchange tests."""
+    d = os.path.join(HERE, "ref_binary")
+    os.makedirs(d, exist_ok=True)
+    out = {}
+    for name, (gen, args) in BINARY_CASES.items():
+        for fmt in FMT_ORDER:
+            m = getattr(R.new(fmt), gen)(*args)
+            blob = R.serialize(m)
+            fn = f"{name}.{fmt}"
+            with open(os.path.join(d, fn), "wb") as f:
+                f.write(blob)
+            info, _ = m.arrays()
+            nx = info["n_col"] if fmt == "COO" else info["ln_col"]
+            x = np.arange(nx, dtype=np.float64)
+            vectors[f"binary/{fn}/spmv_iota"] = m.spmv(x)
+            out[fn] = {"sha256": hashlib.sha256(blob).hexdigest(), "bytes": len(blob), "nx": nx}
+    # two matrices appended to one file, as C_CPP_Matrix<F>_write does
+    a = R.new("CSR").fill_cqmat(40, 40, 6, 0.3, 2, 1, 0, 2, 2)
+    b = R.new("CSR").fill_cqmat_stochastic(50, 50, 5, 0.5, 3)
+    ba, bb = R.serialize(a), R.serialize(b)
+    with open(os.path.join(d, "appended2.CSR"), "wb") as f:
+        f.write(ba + bb)
+    out["appended2.CSR"] = {"sha256": hashlib.sha256(ba + bb).hexdigest(), "bytes": len(ba) + len(bb),
+                            "second_at": len(ba)}
+    return out
+
+
+CFG3_ITERATIONS = (48, 49)
+
+
+def pagerank_cfg3(vectors):
+    """The reference FP64 PageRank of fill_cqmat_stochastic(2^24, 2^24, 8,
+    0.5, 7), beta 0.85, run for 48 and 49 iterations (epsilon 0: no early
+    stop).  From about 40 iterations on, the max-normalised iterate cycles
+    between two states; their sum-normalised forms differ by ~1e-16 in L1,
+    so either is the FP64 fixed point p*."""
+    n = 1 << 24
+    t = time.time()
+    m = R.new("CSR").fill_cqmat_stochastic(n, n, 8, 0.5, 7)
+    print(f"  cfg3 graph {time.time() - t:.1f}s", flush=True)
+    idx = np.random.default_rng(24).choice(n, 8192, replace=False).astype(np.int64)
+    vectors["cfg3_sample_idx"] = idx
+    out = {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7)", "beta": 0.85, "epsilon": 0.0}
+    states = []
+    for k in CFG3_ITERATIONS:
+        pr, its = m.page_rank(0.0, 0.85, k)
+        assert its == k
+        states.append(pr)
+        out[f"it{k}"] = {"sha256": sha(pr), "sum": float(pr.sum())}
+        vectors[f"cfg3_it{k}_sample_val"] = pr[idx]
+        print(f"  cfg3 {k} iterations {time.time() - t:.1f}s", flush=True)
+    out["l1_between_cycle_states"] = float(np.abs(states[0] - states[1]).sum())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true")
+    ap.add_argument("--cfg3", action="store_true", help="only add the n=2^24 PageRank fixture")
     a = ap.parse_args()
     if not R.available():
         sys.exit("oracle/_ref/libtbsla_ref.so missing: run `make -C oracle ref` first")
-    if os.environ.get("OMP_NUM_THREADS") != "1":
+    if os.environ.get("OMP_NUM_THREADS") != "1" and not a.cfg3:
+        # (the cfg3 fixture is CSR only, whose row sums do not depend on the
+        # thread count)
         sys.exit("run with OMP_NUM_THREADS=1 (sequential COO/SCOO reduction order)")
     path = os.path.join(HERE, "digests.json")
     dig = json.load(open(path)) if os.path.exists(path) else {}
     vec = dict(np.load(os.path.join(HERE, "golden_small.npz"))) if os.path.exists(
         os.path.join(HERE, "golden_small.npz")) else {}
     t = time.time()
+    if a.cfg3:
+        dig["pagerank_cfg3"] = pagerank_cfg3(vec)
+        json.dump(dig, open(path, "w"), indent=1, sort_keys=True)
+        np.savez_compressed(os.path.join(HERE, "golden_small.npz"), **vec)
+        print(f"done {time.time() - t:.1f}s")
+        return
     dig["grids"] = small_grids()
     print(f"grids {time.time() - t:.1f}s", flush=True)
     dig["conversions"] = conversions_and_stochastic()
     dig["pagerank_small"] = pagerank_small(vec)
+    dig["binary"] = binary_files(vec)
     print(f"small {time.time() - t:.1f}s", flush=True)
     # 5x5 known answer (tests/csr.cpp:8-25, tests/coo.cpp:8-25)
     vec["ka_values"] = np.arange(1, 11, dtype=np.float64)

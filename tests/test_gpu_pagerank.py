@@ -30,6 +30,26 @@ def test_page_rank_small_matches_reference(tb, digests, golden, gpu):
                 assert np.abs(pr - golden[key]).sum() <= 1e-13, key
 
 
+def test_page_rank_small_blocked_matches_reference(tb, digests, golden, gpu):
+    """The reference-rule fixtures again with propagation-blocked products
+    (the default from 2^16 nodes on), for CSR: the iteration count within 1
+    and the vector within 1e-13 L1."""
+    import torch
+    d = digests["pagerank_small"]
+    for (n, c, q, s) in [(200, 8, 0.5, 7), (1000, 8, 0.5, 7), (2000, 12, 0.3, 5), (1500, 16, 0.1, 1)]:
+        m = tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s)
+        out = torch.zeros(n, dtype=torch.float64, device=gpu)
+        for (eps, beta, it) in [(1e-10, 0.85, 500), (0.85, 0.01, 10)]:
+            nodes = np.array([1, n // 3, n - 1], dtype=np.int32)
+            for kind, kw in (("pr", {}), ("ppr", {"nodes": nodes})):
+                rep = m.page_rank_device(out, epsilon=eps, beta=beta, max_iteration=it, rule="reference",
+                                         gather="blocked", **kw)
+                key = f"{kind}/{n}c{c}q{q}s{s}/CSR/eps{eps}b{beta}"
+                assert abs(rep["iterations"] - d[key]) <= 1, (key, rep["iterations"], d[key])
+                err = np.abs(out.cpu().numpy() - golden[key]).sum()
+                assert err <= 1e-13, (key, err)
+
+
 @pytest.fixture(scope="module")
 def graph_2_20(tb, oracle):
     n = 1 << 20
@@ -42,13 +62,21 @@ def graph_2_20(tb, oracle):
 
 
 def test_page_rank_2_20_reference_rule(graph_2_20, digests, gpu):
-    """n=2^20, c=8, q=0.5, seed 7: the reference stops after 44 iterations."""
+    """n=2^20, c=8, q=0.5, seed 7: the reference stops after 44 iterations.
+    page_rank() (propagation-blocked products at this size) and the
+    fixed-order tiles both stop within one iteration of it, within 1e-13 L1
+    of its vector."""
+    import torch
     p, pr_ref, its_ref, pstar = graph_2_20
     assert its_ref == digests["pagerank_big"]["pr/2^20c8q0.5s7/CSR/eps1e-10b0.85"]["iterations"]
     pr, its = p.page_rank(1e-10, 0.85, 10000)
     assert abs(its - its_ref) <= 1
-    assert np.abs(pr - pr_ref).sum() <= 1e-12
+    assert np.abs(pr - pr_ref).sum() <= 1e-13, np.abs(pr - pr_ref).sum()
     assert abs(pr.sum() - 1) < 1e-14
+    out = torch.zeros(p.get_n_col(), dtype=torch.float64, device=gpu)
+    rep = p.page_rank_device(out, epsilon=1e-10, max_iteration=10000, rule="reference", gather="tiles")
+    assert abs(rep["iterations"] - its_ref) <= 1
+    assert np.abs(out.cpu().numpy() - pr_ref).sum() <= 1e-13
 
 
 @pytest.mark.parametrize("gather", ["tiles", "blocked"])
@@ -222,20 +250,47 @@ def test_page_rank_zero_iterations(tb, gpu):
     assert its == 0 and np.allclose(pr, 1.0 / 100)
 
 
+def oracle_pstar_2_24(g, oracle, digests, golden):
+    """The FP64 fixed point p* of fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7),
+    beta 0.85, on the host: the oracle's 48-iteration reference iteration over
+    the product's arrays, whose digests are the reference's (checked here),
+    and p* itself is checked against the reference's own vector digest."""
+    import hashlib
+    big = digests["big"]["stoch2^24c8q0.5s7/CSR"]
+    a = g.arrays()
+    for k in ("values", "rowptr", "colidx"):
+        assert hashlib.sha256(a[k].tobytes()).hexdigest() == big[f"sha256_{k}"], k
+    n = g.get_n_col()
+    o = oracle.from_arrays("CSR", n, n, a["values"], a["rowptr"], a["colidx"])
+    del a
+    pstar, its = o.page_rank(0.0, 0.85, 48)
+    assert its == 48
+    assert hashlib.sha256(pstar.tobytes()).hexdigest() == digests["pagerank_cfg3"]["it48"]["sha256"]
+    assert np.array_equal(pstar[golden["cfg3_sample_idx"]], golden["cfg3_it48_sample_val"])
+    return pstar
+
+
 @pytest.mark.slow
-def test_full_size_pagerank_2_24(tb, gpu):
-    """fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) at full size: every
-    precision with propagation-blocked products, and FP64 with fixed-order
-    tiles, lands within 1e-10 L1 of the FP64 fixed point (100 reference-rule
-    iterations, tile products) and sums to 1."""
+def test_full_size_pagerank_2_24(tb, oracle, digests, golden, gpu):
+    """fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7) at full size against the
+    FP64 fixed point p* computed on the host by the oracle (and equal to the
+    reference's own limit vector): the GPU's FP64 limit (48 reference-rule
+    iterations, tile products) is within 1e-13 L1 of p*, and every precision
+    with propagation-blocked products, and FP64 with tiles, stopped by the
+    sum-normalised bound at 1e-10, lands within 1e-10 L1 of p* and sums to 1."""
     import torch
     n = 1 << 24
-    g = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 8, 0.5, 7)
-    pstar = torch.zeros(n, dtype=torch.float64, device=gpu)
-    g.page_rank_device(pstar, epsilon=0.0, max_iteration=100, precision="f64", rule="reference")
+    print("generating the 2^24 graph on the GPU", flush=True)
+    g = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 8, 0.5, 7, device=True)
+    print("oracle p* on the host (48 iterations)", flush=True)
+    pstar = torch.from_numpy(oracle_pstar_2_24(g, oracle, digests, golden)).to(gpu)
     out = torch.zeros_like(pstar)
+    g.page_rank_device(out, epsilon=0.0, max_iteration=48, precision="f64", rule="reference", gather="tiles")
+    lim = float((out - pstar).abs().sum())
+    assert lim <= 1e-13, lim
     for prec, gather in [("f64", "tiles"), ("f64", "blocked"), ("pattern", "blocked"), ("f32", "blocked"),
                          ("bf16", "blocked"), ("pattern_f32", "blocked")]:
+        print(f"2^24 {prec} {gather}", flush=True)
         rep = g.page_rank_device(out, epsilon=1e-10, max_iteration=300, precision=prec, rule="sumnorm",
                                  gather=gather)
         err = float((out - pstar).abs().sum())

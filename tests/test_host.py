@@ -209,7 +209,7 @@ def test_library_exports_every_declared_symbol(tb):
     for n in sorted(names):
         assert hasattr(_lib.lib, n), n
     assert set(_lib.SIGNATURES) == names
-    assert _lib.lib.tbsla_abi_version() == 1
+    assert _lib.lib.tbsla_abi_version() == 2
 
 
 def test_c_abi_errors_without_device_calls(tb):
@@ -241,9 +241,17 @@ def test_c_abi_errors_without_device_calls(tb):
         prm = _lib.PageRankParams(epsilon=1e-10, beta=0.85, max_iteration=10, precision=0, rule=1, check_every=8,
                                   anchor_tau=0.0, nodes=None, n_nodes=0, gather=0)
         setattr(prm, field, bad)
-        rc = _lib.lib.tbsla_hip_page_rank_ex(g._h, ctypes.byref(prm), None, 0, None, 0, ctypes.c_void_p(8), None,
-                                             None)
+        rc = _lib.lib.tbsla_hip_page_rank_ex(g._h, ctypes.byref(prm), None, 0, None, 0, ctypes.c_void_p(8), 50,
+                                             None, None)
         assert rc == einval, (field, bad, rc)
+    # an output shorter than n_col is refused before any launch (the pointer
+    # is a dummy that no kernel may write through)
+    prm = _lib.PageRankParams(epsilon=1e-10, beta=0.85, max_iteration=10, precision=0, rule=1, check_every=8,
+                              anchor_tau=0.0, nodes=None, n_nodes=0, gather=0)
+    for fn, args in ((_lib.lib.tbsla_hip_page_rank_ex, (None, 0, None, 0, ctypes.c_void_p(8), 49, None, None)),
+                     (_lib.lib.tbsla_hip_page_rank, (ctypes.cast(ctypes.c_void_p(8), _lib._dp), 49, None, None))):
+        assert fn(g._h, ctypes.byref(prm), *args) == einval
+        assert b"PageRank writes 50" in _lib.lib.tbsla_last_error()
     v = _lib.lib.C_CPP_Vector_create()
     _lib.lib.C_CPP_Vector_fill(v, 5, 3)
     assert not _lib.lib.C_CPP_Vector_add(v, None)
@@ -279,3 +287,17 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=300)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_consumers_link_and_start():
+    """The C++ and C consumers built by __graft_entry__.build() load
+    libtbsla_hip.so and reject bad arguments without touching a GPU."""
+    import subprocess
+    cons = os.path.join(ROOT, "tests", "consumers")
+    for exe, args in (("tbsla_perf_gpu", ["--format", "XYZ"]), ("c_consumer", [])):
+        path = os.path.join(cons, exe)
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", cons], check=True)
+        r = subprocess.run([path] + args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1, (exe, r.returncode, r.stderr)
+        assert "unrecognized" in r.stderr or "usage" in r.stderr

@@ -273,3 +273,19 @@ def test_overlay_spmv_against_reference(oracle, reference):
 def test_cdiag10000_golden(oracle, golden):
     y = oracle.new("CSR").fill_cdiag(10000, 10000, 10).spmv(golden["cdiag10000_x"])
     assert np.array_equal(y, golden["cdiag10000_y"])
+
+
+@pytest.mark.slow
+def test_pagerank_2_24_limit_cycle_golden(oracle, digests, golden):
+    """fill_cqmat_stochastic(2^24, 2^24, 8, 0.5, 7), beta 0.85, no early stop:
+    after 48 and 49 iterations the restatement's vectors are the reference's
+    two limit-cycle states bit for bit (digest and sampled entries)."""
+    d = digests["pagerank_cfg3"]
+    n = 1 << 24
+    m = oracle.new("CSR").fill_cqmat_stochastic(n, n, 8, 0.5, 7)
+    idx = golden["cfg3_sample_idx"]
+    for k in (48, 49):
+        pr, its = m.page_rank(0.0, 0.85, k)
+        assert its == k
+        assert hashlib.sha256(pr.tobytes()).hexdigest() == d[f"it{k}"]["sha256"], k
+        assert np.array_equal(pr[idx], golden[f"cfg3_it{k}_sample_val"]), k
