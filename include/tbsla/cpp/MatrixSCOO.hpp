@@ -42,6 +42,11 @@ class MatrixSCOO : public virtual Matrix {
   void fill_cqmat_values(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc, int NR,
                          int NC, std::uint64_t value_seed) override;
   void normalize_columns() override;
+  // Extension, no reference counterpart: fill_cqmat_values (stochastic:
+  // fill_cqmat_stochastic) with the entries generated on the current GPU.
+  // The host arrays and metadata are identical to the host generators'.
+  void fill_cqmat_gpu(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc, int NR, int NC,
+                      std::uint64_t value_seed, bool stochastic);
   Format format() const override { return FORMAT_SCOO; }
 
  protected:

@@ -46,10 +46,11 @@ int tbsla_matrix_fill_cqmat(tbsla_matrix *m, int n_row, int n_col, int c, double
 int tbsla_matrix_fill_cqmat_stochastic(tbsla_matrix *m, int n_row, int n_col, int c, double q,
                                        unsigned int seed_mult, int pr, int pc, int NR, int NC);
 /* fill_cqmat (stochastic = 0, with the value overlay above) or
- * fill_cqmat_stochastic (stochastic = 1, value_seed ignored) of a CSR matrix
- * with the entries generated on the current GPU; the host arrays are those of
- * tbsla_matrix_fill_cqmat / _stochastic (no reference counterpart: the
- * reference generates on the host, values_generation.cpp:26-62). */
+ * fill_cqmat_stochastic (stochastic = 1, value_seed ignored) of a COO, SCOO,
+ * CSR or ELL matrix with the entries generated on the current GPU; the host
+ * arrays are those of tbsla_matrix_fill_cqmat / _stochastic (no reference
+ * counterpart: the reference generates on the host, values_generation.cpp:
+ * 26-62).  TBSLA_EINVAL for DENSE. */
 int tbsla_matrix_fill_cqmat_gpu(tbsla_matrix *m, int n_row, int n_col, int c, double q, unsigned int seed_mult,
                                 int pr, int pc, int NR, int NC, uint64_t value_seed, int stochastic);
 int tbsla_matrix_normalize_columns(tbsla_matrix *m);
@@ -143,6 +144,22 @@ int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, cons
                            size_t n_values, const double *teleport_dev, size_t n_teleport, double *out_dev,
                            size_t n_out, void *stream, tbsla_pagerank_report *report);
 
+/* tbsla_hip_page_rank with no device memory allocation.  No counterpart in
+ * the reference: for callers that cannot allocate (code under stream
+ * capture, serving loops, harnesses that forbid hipMalloc).  Every buffer of
+ * the run (iterates, state, partial sums, node flags) is a slice of one
+ * static 40 MiB scratch buffer per GPU, held until the process ends; a run
+ * on n_col nodes needs about 16 * n_col bytes (+ n_col + 4 * n_nodes for a
+ * node set), so n_col up to about 2.4 million.  A larger run is refused with
+ * TBSLA_EINVAL before any launch.  FP64 only (precision TBSLA_F64), with the
+ * fixed-order tile products (gather 0 or 1; 2 is refused); the FP64 mirror
+ * must exist already (tbsla_hip_prepare(m, TBSLA_F64)), else TBSLA_EINVAL.
+ * The result equals tbsla_hip_page_rank's with gather = 1 bit for bit.
+ * report->seconds is host time between stream synchronisations (no events
+ * are created).  Calls that overlap on one GPU share the buffer and race. */
+int tbsla_hip_page_rank_noalloc(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, size_t n_out,
+                                void *stream, tbsla_pagerank_report *report);
+
 typedef struct {
   double beta;
   double epsilon;    /* stop when |lambda_k - lambda_{k-1}|_1 <= epsilon * |lambda_k|_1 */
@@ -196,13 +213,16 @@ typedef struct {
   int rule;          /* 0: reference residual < epsilon; 1: sum-normalised bound */
   int check_every;   /* iterations per captured hipGraph replay */
   int graph_capture; /* 1: replay a hipGraph; 0: plain launches */
-  int autotune;      /* 1: time candidate launch shapes on the first run */
+  int autotune;      /* 1: pick the launch shape (cached by batch signature, else timed on the
+                        first iterations of the run) */
 } tbsla_batch_params;
 
 typedef struct {
   int iterations;              /* largest iteration count over the teleport sets */
-  double seconds;              /* device time of the iteration loop */
-  double tune_seconds;
+  double seconds;              /* time of the iteration loop, the autotuner's trial iterations included */
+  double tune_seconds;         /* time of those trial iterations (first run of a batch signature) */
+  int tune_iterations;         /* iterations run as autotuner trials (0: shape from the cache or set) */
+  int tune_cached;             /* 1: the shape came from a batch of the same signature */
   double bytes_per_iteration;  /* algorithmic bytes of one iteration, all sets active */
   int wg_per_cu, unroll, xcd_map; /* launch shape in use */
 } tbsla_batch_report;

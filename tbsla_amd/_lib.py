@@ -56,6 +56,7 @@ class BatchParams(C.Structure):
 
 class BatchReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("tune_seconds", C.c_double),
+                ("tune_iterations", C.c_int), ("tune_cached", C.c_int),
                 ("bytes_per_iteration", C.c_double), ("wg_per_cu", C.c_int), ("unroll", C.c_int),
                 ("xcd_map", C.c_int)]
 
@@ -95,6 +96,7 @@ SIGNATURES = {
     "tbsla_hip_page_rank": (_i, [_vp, C.POINTER(PageRankParams), _dp, _sz, _vp, C.POINTER(PageRankReport)]),
     "tbsla_hip_page_rank_ex": (_i, [_vp, C.POINTER(PageRankParams), _vp, _sz, _vp, _sz, _vp, _sz, _vp,
                                     C.POINTER(PageRankReport)]),
+    "tbsla_hip_page_rank_noalloc": (_i, [_vp, C.POINTER(PageRankParams), _vp, _sz, _vp, C.POINTER(PageRankReport)]),
     "tbsla_hip_page_rank_adjoint": (_i, [_vp, C.POINTER(AdjointParams), _vp, _sz, _vp, _sz, _vp, _vp, _sz, _vp,
                                          _vp, _vp, C.POINTER(AdjointReport)]),
     "tbsla_batch_create": (_vp, [_i]),

@@ -130,6 +130,7 @@ class PageRankBatch:
                                   _stream_handle(stream), C.byref(rep)))
         self._keep = []
         report = {"iterations": rep.iterations, "seconds": rep.seconds, "tune_seconds": rep.tune_seconds,
+                  "tune_iterations": rep.tune_iterations, "tune_cached": bool(rep.tune_cached),
                   "bytes_per_iteration": rep.bytes_per_iteration,
                   "config": {"wg_per_cu": rep.wg_per_cu, "unroll": rep.unroll, "xcd_map": rep.xcd_map}}
         return out, report, its[: self.n_rhs()]

@@ -309,9 +309,29 @@ int tbsla_matrix_fill_cqmat_gpu(tbsla_matrix *m, int n_row, int n_col, int c, do
   if (NR <= 0 || NC <= 0 || pr < 0 || pc < 0 || pr >= NR || pc >= NC || n_row < 0 || n_col < 0 || c < 0 ||
       (stochastic && c <= 0))
     return fail(TBSLA_EINVAL, "tbsla: bad block layout");
-  auto *csr = dynamic_cast<cpp::MatrixCSR *>(m->m.get());
-  if (!csr) return fail(TBSLA_EINVAL, "tbsla: GPU generation needs a CSR matrix");
-  return guarded([&] { csr->fill_cqmat_gpu(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed, stochastic != 0); });
+  cpp::Matrix *x = m->m.get();
+  const bool st = stochastic != 0;
+  switch (x->format()) {
+    case cpp::FORMAT_CSR:
+      return guarded([&] {
+        dynamic_cast<cpp::MatrixCSR *>(x)->fill_cqmat_gpu(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed, st);
+      });
+    case cpp::FORMAT_ELL:
+      return guarded([&] {
+        dynamic_cast<cpp::MatrixELL *>(x)->fill_cqmat_gpu(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed, st);
+      });
+    case cpp::FORMAT_SCOO:
+      return guarded([&] {
+        dynamic_cast<cpp::MatrixSCOO *>(x)->fill_cqmat_gpu(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed,
+                                                           st);
+      });
+    case cpp::FORMAT_COO:
+      return guarded([&] {
+        dynamic_cast<cpp::MatrixCOO *>(x)->fill_cqmat_gpu(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC, value_seed, st);
+      });
+    default:
+      return fail(TBSLA_EINVAL, "tbsla: GPU generation builds COO, SCOO, CSR or ELL matrices");
+  }
 }
 
 int tbsla_matrix_fill_cqmat_stochastic(tbsla_matrix *m, int n_row, int n_col, int c, double q,
@@ -631,6 +651,38 @@ int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, cons
   });
 }
 
+int tbsla_hip_page_rank_noalloc(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, size_t n_out,
+                                void *stream, tbsla_pagerank_report *report) {
+  if (!m || !p || !out_dev) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    if (n_out < static_cast<size_t>(m->m->get_n_col()))
+      throw std::invalid_argument("tbsla: out has " + std::to_string(n_out) + " entries, PageRank writes " +
+                                  std::to_string(m->m->get_n_col()));
+    if (p->rule < 0 || p->rule > 1) throw std::invalid_argument("tbsla: bad stopping rule");
+    if (p->gather < 0 || p->gather > 2) throw std::invalid_argument("tbsla: bad gather mode");
+    hip::PageRankParams q;
+    q.epsilon = p->epsilon;
+    q.beta = p->beta;
+    q.max_iteration = p->max_iteration;
+    q.personalized = p->nodes != nullptr && p->n_nodes > 0;
+    q.nodes = p->nodes;
+    q.n_nodes = p->n_nodes;
+    if (p->precision != TBSLA_F64) throw std::invalid_argument("tbsla: the allocation-free PageRank runs in F64");
+    q.precision = hip::PrPrecision::F64;
+    q.rule = static_cast<hip::PrRule>(p->rule);
+    q.check_every = p->check_every > 0 ? p->check_every : 8;
+    q.gather = p->gather;
+    hip::PageRankReport r = hip::page_rank_device_noalloc(*m->m, q, out_dev, static_cast<hipStream_t>(stream));
+    if (report) {
+      report->iterations = r.iterations;
+      report->anchors = r.anchors;
+      report->last_error = r.last_error;
+      report->bound = r.bound;
+      report->seconds = r.seconds;
+    }
+  });
+}
+
 tbsla_batch *tbsla_batch_create(int precision) {
   tbsla_batch *b = nullptr;
   const int rc = guarded([&] {
@@ -695,6 +747,8 @@ int tbsla_batch_run(tbsla_batch *b, const tbsla_batch_params *p, double *out_dev
       report->iterations = r.iterations;
       report->seconds = r.seconds;
       report->tune_seconds = r.tune_seconds;
+      report->tune_iterations = r.tune_iterations;
+      report->tune_cached = r.tune_cached ? 1 : 0;
       report->bytes_per_iteration = r.bytes_per_iteration;
       report->wg_per_cu = r.config.wg_per_cu;
       report->unroll = r.config.unroll;

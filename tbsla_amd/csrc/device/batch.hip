@@ -6,6 +6,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -580,6 +583,10 @@ struct PageRankBatch::Impl {
   BatchConfig cfg;
   bool tuned = false;
   double bytes_step = 0, bytes_pass = 0;
+  // the key of the autotuner's cache: precision, device, the single-set
+  // graphs' rows, entries and log2 row-length histogram, each multi-set
+  // graph's rows, entries and set count
+  std::string signature;
 
   std::size_t vsize() const { return prec == Dtype::F64 ? 8 : 4; }
 
@@ -591,6 +598,8 @@ struct PageRankBatch::Impl {
   BatchDev<VT, XT> dev_view(int xcd_map, double beta) const;
   template <typename VT, typename XT>
   void enqueue_iteration(const BatchDev<VT, XT> &d, int par, const BatchParams &p, hipStream_t s) const;
+  template <typename VT, typename XT>
+  void enqueue_rest(const BatchDev<VT, XT> &d, int par, const BatchParams &p, hipStream_t s) const;
   template <typename VT, typename XT>
   void launch_step(const BatchDev<VT, XT> &d, int par, int unroll, int wg_per_cu, hipStream_t s) const;
   template <typename VT, typename XT>
@@ -611,6 +620,16 @@ struct PageRankBatch::Impl {
 };
 
 namespace {
+
+// The autotuner's choices by batch signature, for the life of the process.
+std::mutex &tune_cache_mu() {
+  static std::mutex mu;
+  return mu;
+}
+std::map<std::string, BatchConfig> &tune_cache() {
+  static std::map<std::string, BatchConfig> c;
+  return c;
+}
 
 void upload_converted(DevBuf &dst, const double *src, std::size_t n, Dtype prec) {
   if (prec == Dtype::F64) {
@@ -642,6 +661,30 @@ void PageRankBatch::Impl::build() {
     for (std::size_t k = 0; k < graphs[static_cast<std::size_t>(g)].rhs.size(); k++)
       user_rhs_dev[static_cast<std::size_t>(g)].push_back(next++);
   n_rhs_total = next;
+  {
+    std::ostringstream sig;
+    sig << (prec == Dtype::F64 ? "f64" : "f32") << "/" << prop.gcnArchName << "/" << cus;
+    long rows_a = 0, nnz_a = 0;
+    std::vector<long> hist(32, 0);
+    for (int g : single) {
+      const auto &rp = graphs[static_cast<std::size_t>(g)].m->get_rowptr();
+      rows_a += static_cast<long>(rp.size()) - 1;
+      nnz_a += rp.empty() ? 0 : static_cast<long>(rp.back() - rp.front());
+      for (std::size_t i = 0; i + 1 < rp.size(); i++) {
+        int b = 0;
+        for (unsigned len = static_cast<unsigned>(std::max(0, rp[i + 1] - rp[i])); len; len >>= 1) b++;
+        hist[static_cast<std::size_t>(b)]++;
+      }
+    }
+    sig << "|single " << single.size() << " " << rows_a << " " << nnz_a << " h";
+    for (long h : hist) sig << " " << h;
+    for (int g : multi) {
+      const auto &rp = graphs[static_cast<std::size_t>(g)].m->get_rowptr();
+      sig << "|multi " << (rp.size() - 1) << " " << (rp.empty() ? 0 : rp.back() - rp.front()) << " "
+          << graphs[static_cast<std::size_t>(g)].rhs.size();
+    }
+    signature = sig.str();
+  }
   dev_rows.assign(static_cast<std::size_t>(n_rhs_total), 0);
   dev_nodes.assign(static_cast<std::size_t>(n_rhs_total), 0);
   user_offsets.clear();
@@ -959,6 +1002,12 @@ template <typename VT, typename XT>
 void PageRankBatch::Impl::enqueue_iteration(const BatchDev<VT, XT> &d, int par, const BatchParams &p,
                                             hipStream_t s) const {
   launch_step<VT, XT>(d, par, cfg.unroll, cfg.wg_per_cu, s);
+  enqueue_rest<VT, XT>(d, par, p, s);
+}
+
+template <typename VT, typename XT>
+void PageRankBatch::Impl::enqueue_rest(const BatchDev<VT, XT> &d, int par, const BatchParams &p,
+                                       hipStream_t s) const {
   hipLaunchKernelGGL(batch_finalize_max, dim3(n_rhs_total), dim3(64), 0, s, d.rs, d.slots);
   if (n_items)
     hipLaunchKernelGGL((batch_pass<VT, XT, 0>), dim3(n_items), dim3(kTileThreads), 0, s, d, items.as<PassItem>(),
@@ -997,37 +1046,69 @@ template <typename VT, typename XT>
 BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hipStream_t s) {
   BatchReport rep;
   init_state(p, s);
+  // The autotuner: a launch shape found for a batch of the same signature
+  // (in this process) is reused; otherwise the first iterations of this run
+  // are its trials, one candidate shape per iteration, each timed on its
+  // step kernel.  Every shape computes the same bits, so the trial
+  // iterations are ordinary iterations of the run.  The search is by
+  // coordinates from the default shape: gathers in flight (4, 8, 16), then
+  // workgroups per CU (2, 4, 8), then the queue mapping; the number of trial
+  // iterations is made even so the iterate parity of the captured graph holds.
+  int done_its = 0;
   if (p.autotune && !tuned && n_work > 0) {
-    // time one step per candidate launch shape on the batch itself
     const auto t0 = std::chrono::steady_clock::now();
-    hipEvent_t e0, e1;
-    TBSLA_HIP_CHECK(hipEventCreate(&e0));
-    TBSLA_HIP_CHECK(hipEventCreate(&e1));
-    float best = 1e30f;
-    BatchConfig bestc = cfg;
-    for (int xm = 0; xm < 2; xm++) {
-      const BatchDev<VT, XT> d = dev_view<VT, XT>(xm, p.beta);
-      for (int un : {4, 8, 16})
-        for (int wg : {2, 4, 8}) {
-          launch_step<VT, XT>(d, 0, un, wg, s);
-          TBSLA_HIP_CHECK(hipEventRecord(e0, s));
-          for (int k = 0; k < 3; k++) launch_step<VT, XT>(d, 0, un, wg, s);
-          TBSLA_HIP_CHECK(hipEventRecord(e1, s));
-          TBSLA_HIP_CHECK(hipEventSynchronize(e1));
-          float ms = 0;
-          TBSLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-          if (ms < best) {
-            best = ms;
-            bestc = BatchConfig{wg, un, xm};
-          }
-        }
+    {
+      std::lock_guard<std::mutex> g(tune_cache_mu());
+      auto it = tune_cache().find(signature);
+      if (it != tune_cache().end()) {
+        cfg = it->second;
+        tuned = true;
+        rep.tune_cached = true;
+      }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    cfg = bestc;
-    tuned = true;
-    init_state(p, s);  // the trial steps wrote the second iterate and the slots
+    if (!tuned) {
+      hipEvent_t e0, e1;
+      TBSLA_HIP_CHECK(hipEventCreate(&e0));
+      TBSLA_HIP_CHECK(hipEventCreate(&e1));
+      const BatchConfig base = cfg;
+      BatchConfig best = base;
+      float best_ms = 1e30f;
+      std::vector<std::pair<BatchConfig, float>> tried;
+      auto trial = [&](const BatchConfig &c, bool again) {
+        if (!again)
+          for (const auto &t : tried)
+            if (t.first.unroll == c.unroll && t.first.wg_per_cu == c.wg_per_cu && t.first.xcd_map == c.xcd_map)
+              return;
+        const BatchDev<VT, XT> d = dev_view<VT, XT>(c.xcd_map, p.beta);
+        const int par = done_its & 1;
+        TBSLA_HIP_CHECK(hipEventRecord(e0, s));
+        launch_step<VT, XT>(d, par, c.unroll, c.wg_per_cu, s);
+        TBSLA_HIP_CHECK(hipEventRecord(e1, s));
+        enqueue_rest<VT, XT>(d, par, p, s);
+        TBSLA_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        TBSLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        done_its++;
+        tried.push_back({c, ms});
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+      };
+      for (int un : {8, 4, 16}) trial(BatchConfig{base.wg_per_cu, un, base.xcd_map}, false);
+      const int un = best.unroll;
+      for (int wg : {2, 4, 8}) trial(BatchConfig{wg, un, base.xcd_map}, false);
+      trial(BatchConfig{best.wg_per_cu, un, 1 - base.xcd_map}, false);
+      if (done_its & 1) trial(best, true);  // one more iteration, with the chosen shape
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      cfg = best;
+      tuned = true;
+      std::lock_guard<std::mutex> g(tune_cache_mu());
+      tune_cache()[signature] = cfg;
+    }
     rep.tune_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rep.tune_iterations = done_its;
   }
   const BatchDev<VT, XT> d = dev_view<VT, XT>(cfg.xcd_map, p.beta);
   const int C = std::max(2, (p.check_every + 1) / 2 * 2);
@@ -1044,7 +1125,7 @@ BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hi
   TBSLA_HIP_CHECK(hipEventCreate(&e0));
   TBSLA_HIP_CHECK(hipEventCreate(&e1));
   TBSLA_HIP_CHECK(hipEventRecord(e0, s));
-  for (int launched = 0; launched < p.max_iteration; launched += C) {
+  for (int launched = done_its; launched < p.max_iteration; launched += C) {
     if (exec)
       TBSLA_HIP_CHECK(hipGraphLaunch(exec, s));
     else
@@ -1074,7 +1155,9 @@ BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hi
   (void)hipEventDestroy(e1);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
-  rep.seconds = ms * 1e-3;
+  // the trial iterations of the autotuner are iterations of the run: their
+  // time counts in the run's (tune_seconds is the host time they took)
+  rep.seconds = ms * 1e-3 + rep.tune_seconds;
   rep.config = cfg;
   rep.bytes_per_iteration = bytes_step + bytes_pass;
   for (int u = 0; u < n_rhs_total; u++) {

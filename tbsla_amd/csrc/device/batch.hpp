@@ -25,8 +25,9 @@
 //   batch_pass<0>       b /= max, residual and compensated sum partials
 //   batch_finalize_norm per-RHS decision, teleport term for the next step
 // The launch shape (workgroups per CU, queue mapping, gathers in flight)
-// is chosen by an autotuner that times candidate shapes on the batch
-// itself before the first iteration.
+// is chosen by an autotuner that times candidate shapes on the first
+// iterations of the batch's own run (each candidate runs one real
+// iteration), and is cached by batch signature for later batches.
 #pragma once
 
 #include <tbsla/cpp/Matrix.hpp>
@@ -61,7 +62,9 @@ struct BatchConfig {
 struct BatchReport {
   int iterations = 0;           // the most iterations any RHS ran
   double seconds = 0;           // device time of the iteration loop
-  double tune_seconds = 0;      // host time spent in the autotuner (first run)
+  double tune_seconds = 0;      // host time of the autotuner's trial iterations (first run), part of seconds
+  int tune_iterations = 0;      // iterations of this run spent as autotuner trials
+  bool tune_cached = false;     // the launch shape came from the cache (a batch of the same signature)
   BatchConfig config;
   std::vector<int> rhs_iterations;
   std::vector<double> rhs_error;  // last reference residual of each RHS

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 
 #include "common.hpp"
 #include "generate.hpp"
@@ -168,9 +169,191 @@ __global__ __launch_bounds__(kGenThreads) void gen_stochastic(long nnz, const in
   vals[j] = vals[j] / static_cast<double>(cnt[colidx[j]]);
 }
 
+// COO: stream entries [first, first + n) with global row and column ids
+// and the overlay value or 1.0 (MatrixCOO.cpp:332-375)
+__global__ __launch_bounds__(kGenThreads) void gen_coo(Geo g, long first, long n, int *__restrict__ row,
+                                                       int *__restrict__ col, double *__restrict__ vals,
+                                                       unsigned long long vseed_mixed) {
+  const long k = static_cast<long>(blockIdx.x) * kGenThreads + threadIdx.x;
+  if (k >= n) return;
+  const unsigned long long i = static_cast<unsigned long long>(first + k);
+  long ii, jj;
+  g.entry(i, ii, jj);
+  row[k] = static_cast<int>(ii);
+  col[k] = static_cast<int>(jj);
+  double v = 1.0;
+  if (g.value_seed) v = 1.0 - static_cast<double>(splitmix64(vseed_mixed ^ i) >> 11) * 0x1.0p-53;
+  vals[k] = v;
+}
+
+// SCOO: the global row id of every entry of a generated CSR block
+__global__ __launch_bounds__(kGenThreads) void gen_row_ids(long nrows, long f_row, const int *__restrict__ rowptr,
+                                                           int *__restrict__ rows) {
+  const long t = static_cast<long>(blockIdx.x) * kGenThreads + threadIdx.x;
+  if (t >= nrows) return;
+  for (int o = rowptr[t]; o < rowptr[t + 1]; o++) rows[o] = static_cast<int>(f_row + t);
+}
+
+// ELL: row t's entries in slots 0.. of its K slots, the rest (column 0,
+// value 0.0), row-major (MatrixELL.cpp:262-392)
+__global__ __launch_bounds__(kGenThreads) void gen_ell_spread(long nrows, long ngen, int K,
+                                                              const int *__restrict__ rowptr,
+                                                              const int *__restrict__ colidx,
+                                                              const double *__restrict__ vals, int *__restrict__ ecol,
+                                                              double *__restrict__ eval) {
+  const long slot = static_cast<long>(blockIdx.x) * kGenThreads + threadIdx.x;
+  if (slot >= nrows * K) return;
+  const long t = slot / K;
+  const int j = static_cast<int>(slot - t * K);
+  // rows past the generated ones (no candidates) are padding only
+  const int a = t < ngen ? rowptr[t] : 0, len = t < ngen ? rowptr[t + 1] - a : 0;
+  ecol[slot] = j < len ? colidx[a + j] : 0;
+  eval[slot] = j < len ? vals[a + j] : 0.0;
+}
+
 unsigned grid_of(long n) { return static_cast<unsigned>((n + kGenThreads - 1) / kGenThreads); }
 
+Geo geo_of(const host::CqmatBlock &b) {
+  Geo g{};
+  g.nr = static_cast<unsigned long long>(b.n_row);
+  g.nc = static_cast<unsigned long long>(b.n_col);
+  g.c = static_cast<unsigned long long>(b.c);
+  g.q = b.q;
+  g.seed = b.seed;
+  g.value_seed = b.value_seed;
+  g.f_row = b.f_row;
+  g.ln_row = b.ln_row;
+  g.f_col = b.f_col;
+  g.ln_col = b.ln_col;
+  g.cc = b.s.cc;
+  g.min_ = b.s.min_;
+  g.lim = std::min(b.s.n_row, b.s.n_col);
+  return g;
+}
+
+// values / count of their column over the whole matrix (fill_cqmat_stochastic)
+void divide_by_column_counts(const Geo &g, long total, const int *colidx, double *vals) {
+  DevBuf cc;
+  cc.alloc(sizeof(unsigned) * static_cast<std::size_t>(g.nc));
+  TBSLA_HIP_CHECK(hipMemset(cc.get(), 0, sizeof(unsigned) * static_cast<std::size_t>(g.nc)));
+  hipLaunchKernelGGL(gen_colcount, dim3(grid_of(g.lim)), dim3(kGenThreads), 0, 0, g, cc.as<unsigned>());
+  TBSLA_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(gen_stochastic, dim3(grid_of(total)), dim3(kGenThreads), 0, 0, total, colidx,
+                     cc.as<unsigned>(), vals);
+  TBSLA_HIP_CHECK(hipGetLastError());
+}
+
+// The in-block entries of rows [b.f_row, b.f_row + nrows), left on the
+// device (rp, ci, va) with the row pointers on the host.
+struct DeviceRows {
+  std::vector<int> rowptr;
+  long total = 0;
+  DevBuf rp, ci, va;
+};
+
+void generate_rows_device(const host::CqmatBlock &b, long nrows, bool stochastic, DeviceRows &d) {
+  d.rowptr.assign(static_cast<std::size_t>(std::max(0L, nrows)) + 1, 0);
+  d.total = 0;
+  if (nrows <= 0) return;
+  if (static_cast<long>(b.n_col) >= (1L << 31) || b.f_row + nrows > b.n_row)
+    throw std::invalid_argument("tbsla: cqmat generation outside the matrix");
+  const Geo g = geo_of(b);
+  // the rows generated all lie below min(n_row, n_col)
+  if (b.f_row + nrows > g.lim) throw std::invalid_argument("tbsla: cqmat rows past min(n_row, n_col)");
+  DevBuf counts;
+  counts.alloc(sizeof(int) * static_cast<std::size_t>(nrows));
+  hipLaunchKernelGGL(gen_count, dim3(grid_of(nrows)), dim3(kGenThreads), 0, 0, g, nrows, counts.as<int>());
+  TBSLA_HIP_CHECK(hipGetLastError());
+  std::vector<int> cnt_h(static_cast<std::size_t>(nrows));
+  TBSLA_HIP_CHECK(hipMemcpy(cnt_h.data(), counts.get(), cnt_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+  long total = 0;
+  for (long t = 0; t < nrows; t++) {
+    total += cnt_h[static_cast<std::size_t>(t)];
+    if (total >= (1L << 31)) throw std::invalid_argument("tbsla: cqmat block exceeds 2^31 entries");
+    d.rowptr[static_cast<std::size_t>(t) + 1] = static_cast<int>(total);
+  }
+  d.total = total;
+  d.rp.upload(d.rowptr.data(), d.rowptr.size() * sizeof(int));
+  if (total == 0) return;
+  d.ci.alloc(sizeof(int) * static_cast<std::size_t>(total));
+  d.va.alloc(sizeof(double) * static_cast<std::size_t>(total));
+  const unsigned long long vmix = b.value_seed ? gen::splitmix64(b.value_seed) : 0ull;
+  hipLaunchKernelGGL(gen_fill, dim3(grid_of(nrows)), dim3(kGenThreads), 0, 0, g, nrows, d.rp.as<int>(),
+                     d.ci.as<int>(), d.va.as<double>(), vmix);
+  TBSLA_HIP_CHECK(hipGetLastError());
+  if (stochastic) divide_by_column_counts(g, total, d.ci.as<int>(), d.va.as<double>());
+}
+
+template <class T>
+void copy_back(std::vector<T> &h, const DevBuf &d, long n) {
+  h.resize(static_cast<std::size_t>(n));
+  if (n > 0) TBSLA_HIP_CHECK(hipMemcpy(h.data(), d.get(), h.size() * sizeof(T), hipMemcpyDeviceToHost));
+}
+
 }  // namespace
+
+void generate_cqmat_scoo(const host::CqmatBlock &b, long nrows, bool stochastic, std::vector<int> &rowptr,
+                         std::vector<int> &row, std::vector<int> &col, std::vector<double> &values) {
+  DeviceRows d;
+  generate_rows_device(b, nrows, stochastic, d);
+  rowptr = d.rowptr;
+  row.clear();
+  col.clear();
+  values.clear();
+  if (d.total == 0) return;
+  DevBuf rows;
+  rows.alloc(sizeof(int) * static_cast<std::size_t>(d.total));
+  hipLaunchKernelGGL(gen_row_ids, dim3(grid_of(nrows)), dim3(kGenThreads), 0, 0, nrows, static_cast<long>(b.f_row),
+                     d.rp.as<int>(), rows.as<int>());
+  TBSLA_HIP_CHECK(hipGetLastError());
+  copy_back(row, rows, d.total);
+  copy_back(col, d.ci, d.total);
+  copy_back(values, d.va, d.total);
+}
+
+void generate_cqmat_ell(const host::CqmatBlock &b, long nrows, long fill_rows, bool stochastic,
+                        const std::function<int(const std::vector<int> &)> &width, int &K, std::vector<int> &columns,
+                        std::vector<double> &values, bool fill) {
+  DeviceRows d;
+  generate_rows_device(b, nrows, stochastic, d);
+  K = width(d.rowptr);
+  columns.clear();
+  values.clear();
+  if (!fill || fill_rows <= 0 || K <= 0) return;
+  const long slots = fill_rows * static_cast<long>(K);
+  DevBuf ec, ev;
+  ec.alloc(sizeof(int) * static_cast<std::size_t>(slots));
+  ev.alloc(sizeof(double) * static_cast<std::size_t>(slots));
+  // (a block with no entries has no value buffer: every slot is padding)
+  hipLaunchKernelGGL(gen_ell_spread, dim3(grid_of(slots)), dim3(kGenThreads), 0, 0, fill_rows, nrows, K,
+                     d.rp.as<int>(), d.ci.as<int>(), d.va.as<double>(), ec.as<int>(), ev.as<double>());
+  TBSLA_HIP_CHECK(hipGetLastError());
+  copy_back(columns, ec, slots);
+  copy_back(values, ev, slots);
+}
+
+void generate_cqmat_coo(const host::CqmatBlock &b, long first, long n, bool stochastic, std::vector<int> &row,
+                        std::vector<int> &col, std::vector<double> &values) {
+  row.clear();
+  col.clear();
+  values.clear();
+  if (n <= 0) return;
+  if (static_cast<long>(b.n_col) >= (1L << 31) || n >= (1L << 31))
+    throw std::invalid_argument("tbsla: cqmat COO slice too large");
+  const Geo g = geo_of(b);
+  DevBuf r, c, v;
+  r.alloc(sizeof(int) * static_cast<std::size_t>(n));
+  c.alloc(sizeof(int) * static_cast<std::size_t>(n));
+  v.alloc(sizeof(double) * static_cast<std::size_t>(n));
+  const unsigned long long vmix = b.value_seed ? gen::splitmix64(b.value_seed) : 0ull;
+  hipLaunchKernelGGL(gen_coo, dim3(grid_of(n)), dim3(kGenThreads), 0, 0, g, first, n, r.as<int>(), c.as<int>(),
+                     v.as<double>(), vmix);
+  TBSLA_HIP_CHECK(hipGetLastError());
+  if (stochastic) divide_by_column_counts(g, n, c.as<int>(), v.as<double>());
+  copy_back(row, r, n);
+  copy_back(col, c, n);
+  copy_back(values, v, n);
+}
 
 void generate_cqmat_csr(const host::CqmatBlock &b, long nrows, bool stochastic, GeneratedCsr &out) {
   out.rowptr.assign(static_cast<std::size_t>(nrows) + 1, 0);

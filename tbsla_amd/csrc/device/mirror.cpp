@@ -506,6 +506,12 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
   return pb;
 }
 
+const DeviceMatrix *mirror_if_built(const cpp::Matrix &m, Dtype vdt) {
+  DeviceCache &cache = m.device_cache();
+  std::lock_guard<std::mutex> g(cache.mu);
+  return cache.mirrors[static_cast<int>(vdt)].get();
+}
+
 const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt) {
   DeviceCache &cache = m.device_cache();
   std::lock_guard<std::mutex> g(cache.mu);

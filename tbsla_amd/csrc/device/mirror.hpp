@@ -120,6 +120,8 @@ struct DeviceCache {
 
 // Build (or fetch) the mirror of `m` with values of type vdt.
 const DeviceMatrix &mirror(const cpp::Matrix &m, Dtype vdt);
+// The mirror if it was built already, else nullptr (builds nothing).
+const DeviceMatrix *mirror_if_built(const cpp::Matrix &m, Dtype vdt);
 
 // Value-free CSR structure of A^T for a CSR matrix A (rows = A's local
 // columns, columns = A's local rows): row j lists the entries of column j of

@@ -20,7 +20,9 @@
 
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "kernels.hpp"
@@ -41,6 +43,11 @@ struct PrState {
   double sum, sum_prev; // sum of the last / previous normalised iterate (compensated)
   double bound, bound_prev, bound_anchor;
   double anchor_tau, floor_eps;
+  // sum-normalised propagation-blocked runs: 1 / sum of the last iterate
+  // (the next iterate's scale, FP64; the scale of a mixed run's next anchor
+  // move), the sum of the anchor, and the teleport term of the anchor's
+  // iteration
+  double inv_sum, sum_hat, t_anchor;
   int it, max_it, done, rule;
   int need_anchor, anchors, anchor_it, mixed;
 };
@@ -64,6 +71,39 @@ __device__ inline CSum csum_merge(CSum a, CSum b) {
   const double bp = t - a.s;
   const double e = (a.s - (t - bp)) + (b.s - bp);
   return CSum{t, a.c + b.c + e};
+}
+
+// Fixed-order butterfly reductions over one wavefront (every lane gets the
+// result) and over the 16 wavefronts of a 1024-thread workgroup (thread 0
+// gets the result; red holds 48 doubles).
+__device__ inline CSum wave_csum(CSum v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = csum_merge(v, CSum{__shfl_xor(v.s, off, 64), __shfl_xor(v.c, off, 64)});
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline void block1024_err_csum(double err, CSum cs, double *red, double &err_out, CSum &cs_out) {
+  err = wave_sum(err);
+  cs = wave_csum(cs);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = err;
+    red[16 + w] = cs.s;
+    red[32 + w] = cs.c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    err_out = 0;
+    cs_out = CSum{0, 0};
+    for (int i = 0; i < 16; i++) {
+      err_out += red[i];
+      cs_out = csum_merge(cs_out, CSum{red[16 + i], red[32 + i]});
+    }
+  }
 }
 
 __device__ inline CSum block_csum(CSum v, double *ps, double *pc) {
@@ -444,15 +484,19 @@ struct PbScatterArgs {
   int nsb, n;
 };
 
-// The anchor move, folded into the scatter of the iteration that makes it:
-// over the superblock's columns c, xhat[c] += delta[c], and the product's
-// input is z[c] = colscale[c] * xhat[c] (value-free anchor product; z is
-// written for the gather) or xhat[c] itself (z == nullptr).
+// The anchor move of a mixed run, folded into the scatter of the iteration
+// that makes it: over the superblock's columns c, xhat[c] = (xhat[c] +
+// delta[c]) * inv_sum (the iterate, scaled to a sum of about 1), and the
+// product's input is z[c] = colscale[c] * xhat[c] (value-free anchor
+// product; z is written for the gather) or xhat[c] itself (z == nullptr).
+// Each superblock leaves the compensated sum of its new xhat in
+// (psum, pcomp)[superblock].
 struct AnchorMove {
   double *xhat;
   const float *delta;
   double *z;
   const double *colscale;
+  double *psum, *pcomp;
 };
 
 // The far products of one superblock (picked from blockIdx): its slice of x
@@ -460,7 +504,8 @@ struct AnchorMove {
 // With MOVE the slice is the anchor move's result instead of x.
 template <typename VT, typename XT, typename AT, bool MOVE>
 __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restrict__ val, const XT *__restrict__ x,
-                                     AT *__restrict__ prod, XT *xs, const AnchorMove &am) {
+                                     AT *__restrict__ prod, XT *xs, const AnchorMove &am, double inv_sum,
+                                     double *red) {
 #pragma clang fp contract(off)
   // superblocks of one XCD (blockIdx % 8) are contiguous, so the runs that
   // share a cache line at a run boundary are written through the same L2
@@ -475,14 +520,16 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
   {
     constexpr int S = kPbCols / kPbScatterThreads;  // x values per thread, all loads issued first
     XT t[S];
+    CSum cs{0, 0};
 #pragma unroll
     for (int u = 0; u < S; u++) {
       const int i = threadIdx.x + u * kPbScatterThreads;
       if (MOVE) {
         if (i < nc) {
           const long c = c0 + i;
-          const double v = am.xhat[c] + static_cast<double>(am.delta[c]);
+          const double v = (am.xhat[c] + static_cast<double>(am.delta[c])) * inv_sum;
           am.xhat[c] = v;
+          csum_add(cs, v);
           double xv = v;
           if (am.z) {
             xv = am.colscale[c] * v;
@@ -498,6 +545,15 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
     }
 #pragma unroll
     for (int u = 0; u < S; u++) xs[threadIdx.x + u * kPbScatterThreads] = t[u];
+    if (MOVE) {
+      double e_unused;
+      CSum tot;
+      block1024_err_csum(0.0, cs, red, e_unused, tot);
+      if (threadIdx.x == 0) {
+        am.psum[sb] = tot.s;
+        am.pcomp[sb] = tot.c;
+      }
+    }
   }
   __syncthreads();
   // four entries per thread in flight: every global load first, then the
@@ -541,23 +597,25 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a,
                                                                 const PrState *st) {
   if (st->done) return;
   __shared__ XT xs[kPbCols];
-  pb_scatter_sb<VT, XT, AT, false>(a, val, x, prod, xs, AnchorMove{});
+  pb_scatter_sb<VT, XT, AT, false>(a, val, x, prod, xs, AnchorMove{}, 1.0, nullptr);
 }
 
 // Mixed runs, one launch per iteration: in an iteration that moves the
 // anchor, the anchor move and the FP64 anchor product's far products (values
 // VA, FP64); otherwise the delta product's (values VD, delta XD, sums AD).
 template <typename VA, typename VD, typename XD, typename AD>
-__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mixed(PbScatterArgs a, const VA *__restrict__ vala,
-                                                                      const VD *__restrict__ vald,
-                                                                      const XD *__restrict__ xd, void *prod,
-                                                                      AnchorMove am, const PrState *st) {
+__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mx(PbScatterArgs a, const VA *__restrict__ vala,
+                                                                   const VD *__restrict__ vald,
+                                                                   const XD *__restrict__ xd, void *prod,
+                                                                   AnchorMove am, const PrState *st) {
   if (st->done) return;
   __shared__ double xs[kPbCols];
+  __shared__ double red[48];
   if (st->need_anchor)
-    pb_scatter_sb<VA, double, double, true>(a, vala, nullptr, static_cast<double *>(prod), xs, am);
+    pb_scatter_sb<VA, double, double, true>(a, vala, nullptr, static_cast<double *>(prod), xs, am, st->inv_sum, red);
   else
-    pb_scatter_sb<VD, XD, AD, false>(a, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am);
+    pb_scatter_sb<VD, XD, AD, false>(a, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am, 1.0,
+                                     nullptr);
 }
 
 struct PbView {
@@ -590,20 +648,15 @@ __device__ inline void pb_far_round(const unsigned short *__restrict__ frow, con
     if (!MASKED || r[u] >= 0) atomicAdd(&ybin[r[u]], pv[u]);
 }
 
-// The sums of one bin (blockIdx) and its row epilogue.
-// MODE 0: b = beta*(yhat + s) + t (MIXED) or beta*s + t, block max.
-// MODE 2: the anchor product with the update of an iteration that moves the
-//         anchor (its delta is zero): yhat_out = s, b = beta*s + t, block max.
-template <typename VT, typename XT, typename AT, bool MIXED, int TM, int MODE>
-__device__ inline void pb_gather_bin(const PbView &v, const VT *__restrict__ lval, const XT *__restrict__ x,
-                                     const AT *__restrict__ prod, const double *__restrict__ yhat, const Teleport &tp,
-                                     double *__restrict__ out, const PrState *st, unsigned long long *slots,
-                                     double *__restrict__ yhat_out, double *ybin, double *red) {
+// The row sums of one bin (blockIdx) into the FP64 LDS accumulator ybin
+// (kPbRows entries): the bin's far products and its local products, x read
+// from the bin's own slice.  Ends with a barrier.
+template <typename VT, typename XT, typename AT>
+__device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lval, const XT *__restrict__ x,
+                                     const AT *__restrict__ prod, double *ybin) {
 #pragma clang fp contract(off)
-  static_assert(MODE == 0 || (MODE == 2 && !MIXED), "pb_gather modes: 0, and 2 without MIXED");
   const int bin = blockIdx.x, tid = threadIdx.x;
   const int base = bin << kPbShift;
-  const int rows = min(kPbRows, v.n - base);
   for (int i = tid; i < kPbRows; i += 1024) ybin[i] = 0.0;
   __syncthreads();
   const int f1 = v.bp[bin + 1], l1 = v.lp[bin + 1];
@@ -657,13 +710,27 @@ __device__ inline void pb_gather_bin(const PbView &v, const VT *__restrict__ lva
     }
   }
   __syncthreads();
+}
+
+// FP64 runs under the reference rule: b = beta*(A x) + t into out, block
+// max into the slots (normalised by a separate pass).
+template <typename VT, typename XT, typename AT, int TM>
+__global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict__ lval, const XT *__restrict__ x,
+                                                  const AT *__restrict__ prod, Teleport tp, double *__restrict__ out,
+                                                  PrState *st, unsigned long long *slots) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  __shared__ double ybin[kPbRows];  // FP64 sums (products may be FP32)
+  __shared__ double red[16];
+  pb_accumulate<VT, XT, AT>(v, lval, x, prod, ybin);
+  const int bin = blockIdx.x, tid = threadIdx.x;
+  const int base = bin << kPbShift;
+  const int rows = min(kPbRows, v.n - base);
   double lmax = -INFINITY;
   const double beta = st->beta, tele = st->tele;
   for (int i = tid; i < rows; i += 1024) {
     const int r = base + i;
-    if (MODE == 2) yhat_out[r] = static_cast<double>(ybin[i]);
-    const double sv = MIXED ? yhat[r] + static_cast<double>(ybin[i]) : static_cast<double>(ybin[i]);
-    const double bi = add_teleport<TM>(beta * sv, tele, tp, r);
+    const double bi = add_teleport<TM>(beta * ybin[i], tele, tp, r);
     out[r] = bi;
     lmax = fmax(lmax, bi);
   }
@@ -678,34 +745,108 @@ __device__ inline void pb_gather_bin(const PbView &v, const VT *__restrict__ lva
   }
 }
 
-// FP64 runs.
-template <typename VT, typename XT, typename AT, int TM>
-__global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict__ lval, const XT *__restrict__ x,
-                                                  const AT *__restrict__ prod, Teleport tp, double *__restrict__ out,
-                                                  PrState *st, unsigned long long *slots) {
-  if (st->done) return;
-  __shared__ double ybin[kPbRows];  // FP64 sums (products may be FP32)
-  __shared__ double red[16];
-  pb_gather_bin<VT, XT, AT, false, TM, 0>(v, lval, x, prod, nullptr, tp, out, st, slots, nullptr, ybin, red);
-}
-
-// Mixed runs, the partner of pb_scatter_mixed: the anchor product with the
-// update (MODE 2, x = xa) or the delta product with the update (x = xd).
-template <typename VA, typename VD, typename XD, typename AD, int TM>
-__global__ __launch_bounds__(1024) void pb_gather_mixed(PbView v, const VA *__restrict__ lvala,
-                                                        const double *__restrict__ xa, const VD *__restrict__ lvald,
-                                                        const XD *__restrict__ xd, const void *prod, double *yhat,
-                                                        Teleport tp, double *__restrict__ out, PrState *st,
-                                                        unsigned long long *slots) {
+// FP64 runs under the sum-normalised rule, update and normalisation in one
+// pass: x' = (beta*(A x) + t) * inv_sum, where inv_sum = 1 / sum(x) (the
+// sum-normalised iterate does not depend on the scale, so the scale of the
+// last iterate serves), with sum|x' - x| and the compensated sum of x' as
+// per-bin partials.  PAT: also z' = colscale .* x' for the next value-free
+// product (x is then the product's input z, xprev the iterate).
+template <typename VT, typename XT, typename AT, int TM, bool PAT>
+__global__ __launch_bounds__(1024) void pb_gather_sn(PbView v, const VT *__restrict__ lval, const XT *__restrict__ x,
+                                                     const AT *__restrict__ prod, Teleport tp,
+                                                     const double *__restrict__ xprev, double *__restrict__ xout,
+                                                     double *__restrict__ zout, const double *__restrict__ colscale,
+                                                     const PrState *st, double *__restrict__ perr,
+                                                     double *__restrict__ ps, double *__restrict__ pc) {
+#pragma clang fp contract(off)
   if (st->done) return;
   __shared__ double ybin[kPbRows];
-  __shared__ double red[16];
-  if (st->need_anchor)
-    pb_gather_bin<VA, double, double, false, TM, 2>(v, lvala, xa, static_cast<const double *>(prod), nullptr, tp, out,
-                                                    st, slots, yhat, ybin, red);
+  __shared__ double red[48];
+  pb_accumulate<VT, XT, AT>(v, lval, x, prod, ybin);
+  const int bin = blockIdx.x, tid = threadIdx.x;
+  const int base = bin << kPbShift;
+  const int rows = min(kPbRows, v.n - base);
+  const double beta = st->beta, tele = st->tele, inv = st->inv_sum;
+  double err = 0.0;
+  CSum cs{0, 0};
+  for (int i = tid; i < rows; i += 1024) {
+    const int r = base + i;
+    const double xn = add_teleport<TM>(beta * ybin[i], tele, tp, r) * inv;
+    xout[r] = xn;
+    if (PAT) zout[r] = colscale[r] * xn;
+    err += fabs(xn - xprev[r]);
+    csum_add(cs, xn);
+  }
+  double e;
+  CSum t;
+  block1024_err_csum(err, cs, red, e, t);
+  if (tid == 0) {
+    perr[bin] = e;
+    ps[bin] = t.s;
+    pc[bin] = t.c;
+  }
+}
+
+// Mixed runs (the partner of pb_scatter_mx).  The iterate is xhat + delta
+// (FP64 anchor, FP32 delta).  Between anchor moves it is not rescaled, so
+// with w = beta*A xhat + t_a*e - xhat (the anchor's one-step change, e the
+// teleport pattern, t_a the anchor iteration's teleport term) the next
+// delta is d' = w + beta*(A_lo d) + (t - t_a)*e.  The iteration that moves
+// the anchor (d = 0) sums the FP64 anchor product (values VA, input xa = z
+// or xhat) and stores w in FP32 (d' = w); the others sum the delta product
+// (values VD, input xd).  Writes d' (PAT32: also zd' = fl32(colscale32 *
+// d')), with sum|d' - d| and the compensated sum of d' as per-bin partials.
+template <typename VA, typename VD, typename XD, typename AD, int TM, bool PAT32>
+__global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restrict__ lvala,
+                                                     const double *__restrict__ xa, const VD *__restrict__ lvald,
+                                                     const XD *__restrict__ xd, const void *prod, Teleport tp,
+                                                     const double *__restrict__ xhat, float *__restrict__ w,
+                                                     const float *__restrict__ dold, float *__restrict__ dnew,
+                                                     float *__restrict__ zdnew, const float *__restrict__ colscale32,
+                                                     const PrState *st, double *__restrict__ perr,
+                                                     double *__restrict__ ps, double *__restrict__ pc) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  __shared__ double ybin[kPbRows];
+  __shared__ double red[48];
+  const bool anchor = st->need_anchor != 0;
+  if (anchor)
+    pb_accumulate<VA, double, double>(v, lvala, xa, static_cast<const double *>(prod), ybin);
   else
-    pb_gather_bin<VD, XD, AD, true, TM, 0>(v, lvald, xd, static_cast<const AD *>(prod), yhat, tp, out, st, slots,
-                                           nullptr, ybin, red);
+    pb_accumulate<VD, XD, AD>(v, lvald, xd, static_cast<const AD *>(prod), ybin);
+  const int bin = blockIdx.x, tid = threadIdx.x;
+  const int base = bin << kPbShift;
+  const int rows = min(kPbRows, v.n - base);
+  const double beta = st->beta;
+  // the anchor iteration's teleport term; the same expression gives the
+  // finalize pass the t_a it keeps for the phase
+  const double ta = anchor ? st->tele * st->inv_sum : st->t_anchor;
+  const double dt = st->tele - ta;
+  double err = 0.0;
+  CSum cs{0, 0};
+  for (int i = tid; i < rows; i += 1024) {
+    const int r = base + i;
+    float d32;
+    if (anchor) {
+      d32 = static_cast<float>(add_teleport<TM>(beta * ybin[i], ta, tp, r) - xhat[r]);
+      w[r] = d32;
+      err += fabs(static_cast<double>(d32));
+    } else {
+      d32 = static_cast<float>(add_teleport<TM>(static_cast<double>(w[r]) + beta * ybin[i], dt, tp, r));
+      err += fabs(static_cast<double>(d32) - static_cast<double>(dold[r]));
+    }
+    dnew[r] = d32;
+    if (PAT32) zdnew[r] = static_cast<float>(static_cast<double>(colscale32[r]) * static_cast<double>(d32));
+    csum_add(cs, static_cast<double>(d32));
+  }
+  double e;
+  CSum t;
+  block1024_err_csum(err, cs, red, e, t);
+  if (tid == 0) {
+    perr[bin] = e;
+    ps[bin] = t.s;
+    pc[bin] = t.c;
+  }
 }
 
 PbView pb_view(const PbMatrix &pb) {
@@ -718,8 +859,11 @@ PbScatterArgs pb_args(const PbMatrix &pb) {
                        pb.cbase.as<int>(), pb.cp.as<int>(),              pb.nsb,          pb.n};
 }
 
-// One propagation-blocked product with the update (FP64 runs): values of
-// type VT from `pb` (dev::Pattern: value-free), x of type XT, sums of type AT.
+dim3 pb_scatter_grid(const PbMatrix &pb) { return dim3((pb.nsb + 7) / 8 * 8); }
+
+// One propagation-blocked product with the reference-rule update (FP64
+// runs): values of type VT from `pb` (dev::Pattern: value-free), x of type
+// XT, sums of type AT.
 template <typename VT, typename XT, typename AT, int TM>
 void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const Teleport &tp, double *out, PrState *st,
                unsigned long long *slots, hipStream_t s) {
@@ -727,8 +871,8 @@ void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const Teleport &tp, d
   const VT *lval = vslot >= 0 ? pb.lval[vslot].as<VT>() : nullptr;
   AT *prod = pb.prod.as<AT>();
   if (pb.n_far > 0)
-    hipLaunchKernelGGL((pb_scatter<VT, XT, AT>), dim3((pb.nsb + 7) / 8 * 8), dim3(kPbScatterThreads), 0, s,
-                       pb_args(pb), fval, x, prod, st);
+    hipLaunchKernelGGL((pb_scatter<VT, XT, AT>), pb_scatter_grid(pb), dim3(kPbScatterThreads), 0, s, pb_args(pb), fval,
+                       x, prod, st);
   hipLaunchKernelGGL((pb_gather<VT, XT, AT, TM>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval, x, prod, tp,
                      out, st, slots);
 }
@@ -744,34 +888,78 @@ void launch_pb_update(int tmode, const PbMatrix &pb, int vslot, const XT *x, con
     launch_pb<VT, XT, AT, 0>(pb, vslot, x, tp, out, st, slots, s);
 }
 
-// One iteration's product and update of a mixed run: the anchor move and
-// the FP64 anchor product (values VA at slot va, x = z or xhat) when the
-// state asks for it, else the delta product (values VD at slot vd, delta
-// xd).  The scatter always runs: it also makes the anchor move.
-template <typename VA, typename VD, typename XD, typename AD, int TM>
-void launch_pb_mixed(const PbMatrix &pb, int va, int vd, const AnchorMove &am, const XD *xd, double *yhat,
-                     const Teleport &tp, double *out, PrState *st, unsigned long long *slots, hipStream_t s) {
+// The per-bin partials of a sum-normalised iteration.
+struct SnPartials {
+  double *perr, *ps, *pc;
+};
+
+// One sum-normalised FP64 iteration (PAT: value-free over z): scatter and
+// gather; the finalize pass follows.
+template <typename VT, int TM, bool PAT>
+void launch_pb_sn_tm(const PbMatrix &pb, int vslot, const double *xin, const double *z, const Teleport &tp,
+                     double *xout, double *zout, const double *colscale, PrState *st, const SnPartials &sp,
+                     hipStream_t s) {
+  const VT *fval = vslot >= 0 ? pb.fval[vslot].as<VT>() : nullptr;
+  const VT *lval = vslot >= 0 ? pb.lval[vslot].as<VT>() : nullptr;
+  const double *x = PAT ? z : xin;
+  double *prod = pb.prod.as<double>();
+  if (pb.n_far > 0)
+    hipLaunchKernelGGL((pb_scatter<VT, double, double>), pb_scatter_grid(pb), dim3(kPbScatterThreads), 0, s,
+                       pb_args(pb), fval, x, prod, st);
+  hipLaunchKernelGGL((pb_gather_sn<VT, double, double, TM, PAT>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval,
+                     x, prod, tp, xin, xout, zout, colscale, st, sp.perr, sp.ps, sp.pc);
+}
+
+template <typename VT, bool PAT>
+void launch_pb_sn(int tmode, const PbMatrix &pb, int vslot, const double *xin, const double *z, const Teleport &tp,
+                  double *xout, double *zout, const double *colscale, PrState *st, const SnPartials &sp,
+                  hipStream_t s) {
+  if (tmode == 1)
+    launch_pb_sn_tm<VT, 1, PAT>(pb, vslot, xin, z, tp, xout, zout, colscale, st, sp, s);
+  else if (tmode == 2)
+    launch_pb_sn_tm<VT, 2, PAT>(pb, vslot, xin, z, tp, xout, zout, colscale, st, sp, s);
+  else
+    launch_pb_sn_tm<VT, 0, PAT>(pb, vslot, xin, z, tp, xout, zout, colscale, st, sp, s);
+}
+
+// Buffers of a mixed sum-normalised iteration.
+struct MxBufs {
+  AnchorMove am;           // anchor xhat (and z) with the scatter's sum partials
+  const void *xd;          // the delta product's input: delta (F32/BF16) or zd (PATTERN_F32)
+  float *w;                // FP32 one-step change of the anchor
+  const float *dold;       // delta of the current iterate
+  float *dnew, *zdnew;     // delta (and zd) of the next
+  const float *colscale32; // PATTERN_F32
+};
+
+// One mixed iteration: the anchor move and the FP64 anchor product (values
+// VA at slot va, input z or xhat) when the state asks for it, else the delta
+// product (values VD at slot vd); the finalize pass follows.
+template <typename VA, typename VD, typename XD, int TM, bool PAT32>
+void launch_pb_mx_tm(const PbMatrix &pb, int va, int vd, const MxBufs &b, const Teleport &tp, PrState *st,
+                     const SnPartials &sp, hipStream_t s) {
   const VA *fa = va >= 0 ? pb.fval[va].as<VA>() : nullptr;
   const VA *la = va >= 0 ? pb.lval[va].as<VA>() : nullptr;
   const VD *fd = vd >= 0 ? pb.fval[vd].as<VD>() : nullptr;
   const VD *ld = vd >= 0 ? pb.lval[vd].as<VD>() : nullptr;
-  const double *xa = am.z ? am.z : am.xhat;
-  hipLaunchKernelGGL((pb_scatter_mixed<VA, VD, XD, AD>), dim3((pb.nsb + 7) / 8 * 8), dim3(kPbScatterThreads), 0, s,
-                     pb_args(pb), fa, fd, xd, pb.prod.get(), am, st);
-  hipLaunchKernelGGL((pb_gather_mixed<VA, VD, XD, AD, TM>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), la, xa,
-                     ld, xd, pb.prod.get(), yhat, tp, out, st, slots);
+  const double *xa = b.am.z ? b.am.z : b.am.xhat;
+  const XD *xd = static_cast<const XD *>(b.xd);
+  hipLaunchKernelGGL((pb_scatter_mx<VA, VD, XD, float>), pb_scatter_grid(pb), dim3(kPbScatterThreads), 0, s,
+                     pb_args(pb), fa, fd, xd, pb.prod.get(), b.am, st);
+  hipLaunchKernelGGL((pb_gather_mx<VA, VD, XD, float, TM, PAT32>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), la,
+                     xa, ld, xd, pb.prod.get(), tp, b.am.xhat, b.w, b.dold, b.dnew, b.zdnew, b.colscale32, st,
+                     sp.perr, sp.ps, sp.pc);
 }
 
-template <typename VA, typename VD, typename XD, typename AD>
-void launch_pb_mixed_update(int tmode, const PbMatrix &pb, int va, int vd, const AnchorMove &am, const XD *xd,
-                            double *yhat, const Teleport &tp, double *out, PrState *st, unsigned long long *slots,
-                            hipStream_t s) {
+template <typename VA, typename VD, typename XD, bool PAT32>
+void launch_pb_mx(int tmode, const PbMatrix &pb, int va, int vd, const MxBufs &b, const Teleport &tp, PrState *st,
+                  const SnPartials &sp, hipStream_t s) {
   if (tmode == 1)
-    launch_pb_mixed<VA, VD, XD, AD, 1>(pb, va, vd, am, xd, yhat, tp, out, st, slots, s);
+    launch_pb_mx_tm<VA, VD, XD, 1, PAT32>(pb, va, vd, b, tp, st, sp, s);
   else if (tmode == 2)
-    launch_pb_mixed<VA, VD, XD, AD, 2>(pb, va, vd, am, xd, yhat, tp, out, st, slots, s);
+    launch_pb_mx_tm<VA, VD, XD, 2, PAT32>(pb, va, vd, b, tp, st, sp, s);
   else
-    launch_pb_mixed<VA, VD, XD, AD, 0>(pb, va, vd, am, xd, yhat, tp, out, st, slots, s);
+    launch_pb_mx_tm<VA, VD, XD, 0, PAT32>(pb, va, vd, b, tp, st, sp, s);
 }
 
 template <typename VT, typename XT, bool MIXED, int TM>
@@ -813,10 +1001,109 @@ __global__ void zero_u64(unsigned long long *p, int n) {
   if (i < n) p[i] = 0;
 }
 
-struct Scratch {
-  DevBuf st, slots, perr, ps, pc, total, member;
-  DevBuf X0, X1, Z, yhat, xhat, D0, D1, colscale, ZD0, ZD1, colscale32;
+// One device buffer of a run: owned (allocated for the run) or a slice of
+// the fixed scratch buffer of the allocation-free entry point.  Unlike
+// DevBuf, a slice is not cleared: every buffer below is written before it
+// is read.
+struct WsBuf {
+  DevBuf own;
+  void *p = nullptr;
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+  void *get() const { return p; }
 };
+
+// Where a run's buffers come from: hipMalloc (arena == nullptr) or
+// consecutive 256-byte-aligned slices of `arena` (cap bytes), which is
+// never allocated or freed here.
+struct Workspace {
+  unsigned char *arena = nullptr;
+  std::size_t cap = 0, used = 0;
+  void alloc(WsBuf &b, std::size_t bytes) {
+    if (!arena) {
+      b.own.alloc(bytes);
+      b.p = b.own.get();
+      return;
+    }
+    const std::size_t need = (std::max<std::size_t>(bytes, 1) + 255) & ~std::size_t(255);
+    if (need > cap - used)
+      throw std::invalid_argument("tbsla: the allocation-free PageRank needs more than its " +
+                                  std::to_string(cap >> 20) + " MiB scratch buffer");
+    b.p = arena + used;
+    used += need;
+  }
+  void upload(WsBuf &b, const void *src, std::size_t bytes, hipStream_t s) {
+    alloc(b, bytes);
+    if (bytes) TBSLA_HIP_CHECK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+  }
+};
+
+struct Scratch {
+  WsBuf st, slots, perr, ps, pc, total, member, nodes, hsum, hcomp;
+  WsBuf X0, X1, Z, Z1, yhat, xhat, D0, D1, colscale, ZD0, ZD1, colscale32, W;
+};
+
+// zd = fl32(colscale32 * d) of a constant FP32 delta d
+__global__ __launch_bounds__(256) void scale_into_f32_32(const float *__restrict__ s, float d, float *__restrict__ z,
+                                                         long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256)
+    z[i] = static_cast<float>(static_cast<double>(s[i]) * static_cast<double>(d));
+}
+
+// One block: the step decision of a sum-normalised propagation-blocked
+// iteration from the gather's per-bin partials (nb) and, after a mixed run's
+// anchor move, the scatter's per-superblock sums of the new anchor (nsb).
+// S is the sum of the new iterate: sum(x') (FP64 runs), sum(xhat) + sum(d')
+// (mixed runs).  The iteration that moved the anchor starts from the
+// rescaled anchor, whose sum replaces the previous one in the bound.
+__global__ __launch_bounds__(kTileThreads) void pr_finalize_sn(const double *__restrict__ perr,
+                                                               const double *__restrict__ ps,
+                                                               const double *__restrict__ pc, int nb,
+                                                               const double *__restrict__ hs,
+                                                               const double *__restrict__ hc, int nsb, PrState *st) {
+  if (st->done) return;
+  __shared__ double red_s[kTileThreads], red_c[kTileThreads];
+  double e = 0;
+  CSum t{0, 0};
+  for (int i = threadIdx.x; i < nb; i += kTileThreads) {
+    e += perr[i];
+    t = csum_merge(t, CSum{ps[i], pc[i]});
+  }
+  e = dev::block_tree_sum<double>(e, red_s);
+  t = block_csum(t, red_s, red_c);
+  const bool moved = st->mixed && st->need_anchor;
+  CSum h{0, 0};
+  if (moved) {
+    for (int i = threadIdx.x; i < nsb; i += kTileThreads) h = csum_merge(h, CSum{hs[i], hc[i]});
+    h = block_csum(h, red_s, red_c);
+  }
+  if (threadIdx.x == 0) {
+    double S = t.s + t.c;
+    if (st->mixed) {
+      if (moved) {
+        st->sum_hat = h.s + h.c;
+        st->sum = st->sum_hat;
+        st->t_anchor = st->tele * st->inv_sum;
+      }
+      S += st->sum_hat;
+    }
+    finalize_step(st, e, S);
+    st->inv_sum = 1.0 / S;
+  }
+}
+
+// member[i] = 1 for the listed nodes in [0, n) (member cleared first).
+__global__ __launch_bounds__(256) void set_members(unsigned char *member, long n, const int *nodes, int n_nodes) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n_nodes; i += static_cast<long>(gridDim.x) * 256) {
+    const int v = nodes[i];
+    if (v >= 0 && v < n) member[v] = 1;
+  }
+}
+__global__ __launch_bounds__(256) void fill_u8(unsigned char *y, long n, unsigned char v) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) y[i] = v;
+}
 
 }  // namespace
 
@@ -834,7 +1121,13 @@ int next_batch(const PrState &h, int check) {
   return static_cast<int>(std::min<double>(check, std::max(1.0, std::ceil(left))));
 }
 
-PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, double *out_dev, hipStream_t s) {
+namespace {
+
+// The engine.  arena != nullptr: every buffer of the run is a slice of the
+// arena (cap bytes) and nothing is allocated (the caller has checked that
+// the mirror exists and that no lazily built layout is needed).
+PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, double *out_dev, hipStream_t s,
+                             unsigned char *arena, std::size_t cap) {
   const long n = m.get_n_col();
   // (COO keeps NR/NC in ln_row/ln_col, MatrixCOO.cpp:344-345, so the block
   // layout is read from NR/NC)
@@ -876,28 +1169,45 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   if (d64.x_need > n) throw std::invalid_argument("tbsla: column index beyond n_col");
 
   Scratch w;
+  Workspace ws{arena, cap, 0};
   const unsigned gN = grid_for(n);
   // the mixed normalise pass (six streams) measured faster with twice the
   // blocks (four and eight times: slower); the FP64 one (three streams) slower
   const unsigned gM = mixed ? grid_for(n, 2L * kNormBlocks) : gN;
-  w.st.alloc(sizeof(PrState));
-  w.slots.alloc(sizeof(unsigned long long) * kMaxSlots);
-  w.perr.alloc(sizeof(double) * std::max(gN, gM));
-  w.ps.alloc(sizeof(double) * std::max(gN, gM));
-  w.pc.alloc(sizeof(double) * std::max(gN, gM));
-  w.total.alloc(sizeof(double));
-  w.X0.alloc(sizeof(double) * n);
-  w.X1.alloc(sizeof(double) * n);
+  // sum-normalised rule with propagation-blocked products: the update, the
+  // normalisation and the residual in the gather pass, one finalize pass
+  const bool sn = pbm && p.rule == PrRule::SUMNORM;
+  const unsigned nparts = std::max(std::max(gN, gM), sn ? static_cast<unsigned>(pbm->nbins) : 0u);
+  ws.alloc(w.st, sizeof(PrState));
+  ws.alloc(w.slots, sizeof(unsigned long long) * kMaxSlots);
+  ws.alloc(w.perr, sizeof(double) * nparts);
+  ws.alloc(w.ps, sizeof(double) * nparts);
+  ws.alloc(w.pc, sizeof(double) * nparts);
+  ws.alloc(w.total, sizeof(double));
+  ws.alloc(w.X0, sizeof(double) * n);
+  ws.alloc(w.X1, sizeof(double) * n);
   // teleport mode: 0 uniform, 1 node set (Matrix.cpp:63-114), 2 distribution vector
   const int tmode = p.teleport ? 2 : (p.personalized ? 1 : 0);
   Teleport tp{nullptr, p.teleport};
   const double b0 = p.personalized && !p.teleport ? 0.25 : 1.0;
   std::vector<unsigned char> mem;
-  if (tmode == 1) {
+  if (tmode == 1 && arena) {
+    // no host staging vector: the node list goes to the scratch buffer and
+    // a kernel sets the flags
+    ws.alloc(w.member, static_cast<std::size_t>(n));
+    hipLaunchKernelGGL(fill_u8, dim3(grid_for(n)), dim3(256), 0, s, w.member.as<unsigned char>(), n,
+                       static_cast<unsigned char>(0));
+    if (p.n_nodes > 0) {
+      ws.upload(w.nodes, p.nodes, sizeof(int) * static_cast<std::size_t>(p.n_nodes), s);
+      hipLaunchKernelGGL(set_members, dim3(grid_for(p.n_nodes)), dim3(256), 0, s, w.member.as<unsigned char>(), n,
+                         w.nodes.as<int>(), p.n_nodes);
+    }
+    tp.member = w.member.as<unsigned char>();
+  } else if (tmode == 1) {
     mem.assign(static_cast<std::size_t>(n), 0);
     for (int k = 0; k < p.n_nodes; k++)
       if (p.nodes[k] >= 0 && p.nodes[k] < n) mem[static_cast<std::size_t>(p.nodes[k])] = 1;
-    w.member.upload(mem.data(), mem.size(), s);
+    ws.upload(w.member, mem.data(), mem.size(), s);
     tp.member = w.member.as<unsigned char>();
   }
   const bool pattern = p.precision == PrPrecision::PATTERN;
@@ -918,10 +1228,10 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
       if (c < 0 || c >= n) throw std::invalid_argument("tbsla: column index outside [0, n_col)");
       cs[static_cast<std::size_t>(c)] = va[k];
     }
-    w.colscale.upload(cs.data(), cs.size() * sizeof(double), s);
+    ws.upload(w.colscale, cs.data(), cs.size() * sizeof(double), s);
     if (pat32) {
       std::vector<float> c32(cs.begin(), cs.end());
-      w.colscale32.upload(c32.data(), c32.size() * sizeof(float), s);
+      ws.upload(w.colscale32, c32.data(), c32.size() * sizeof(float), s);
     }
   }
   PrState h{};
@@ -931,8 +1241,16 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   h.tele_scale = tmode == 2   ? (1.0 - p.beta)
                  : tmode == 1 ? (1.0 - p.beta) / static_cast<double>(static_cast<std::size_t>(p.n_nodes))
                               : (1.0 - p.beta) / static_cast<double>(n);
-  h.sum = b0 * static_cast<double>(n);
+  // sum-normalised propagation-blocked runs start from 1/n (any uniform
+  // start gives the same sum-normalised iterates; this one keeps the
+  // iterates near sum 1 from the start)
+  const double x0 = sn ? 1.0 / static_cast<double>(n) : b0;
+  const double x0s = mixed ? static_cast<double>(static_cast<float>(x0)) : x0;  // as stored
+  h.sum = x0s * static_cast<double>(n);
   h.tele = h.sum * h.tele_scale;
+  h.inv_sum = 1.0 / h.sum;
+  h.sum_hat = 0.0;
+  h.t_anchor = 0.0;
   h.bound = 2.0;
   h.bound_prev = 2.0;
   h.bound_anchor = 2.0;
@@ -951,43 +1269,69 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   TBSLA_HIP_CHECK(hipMemcpyAsync(w.st.get(), &h, sizeof(h), hipMemcpyHostToDevice, s));
   PrState *st = w.st.as<PrState>();
   double *X[2] = {w.X0.as<double>(), w.X1.as<double>()};
-  hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[0], n, b0);
+  hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[0], n, x0);
+  hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[1], n, 0.0);
   unsigned long long *slots = w.slots.as<unsigned long long>();
   hipLaunchKernelGGL(zero_u64, dim3(1), dim3(kMaxSlots), 0, s, slots, kMaxSlots);
   float *D[2] = {nullptr, nullptr};
   if (mixed) {
-    w.xhat.alloc(sizeof(double) * n);
-    w.yhat.alloc(sizeof(double) * n);
-    w.D0.alloc(sizeof(float) * n);
-    w.D1.alloc(sizeof(float) * n);
+    ws.alloc(w.xhat, sizeof(double) * n);
+    ws.alloc(w.D0, sizeof(float) * n);
+    ws.alloc(w.D1, sizeof(float) * n);
     D[0] = w.D0.as<float>();
     D[1] = w.D1.as<float>();
     hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.xhat.as<double>(), n, 0.0);
-    hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.yhat.as<double>(), n, 0.0);
-    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[0], n, static_cast<float>(b0));
+    hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[0], n, static_cast<float>(x0));
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, D[1], n, 0.0f);
+    if (sn) {
+      // the anchor's FP32 one-step change (0 before the first anchor, whose
+      // anchor is 0) and the scatter's per-superblock sums of a moved anchor
+      ws.alloc(w.W, sizeof(float) * n);
+      hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, w.W.as<float>(), n, 0.0f);
+      ws.alloc(w.hsum, sizeof(double) * std::max(1, pbm->nsb));
+      ws.alloc(w.hcomp, sizeof(double) * std::max(1, pbm->nsb));
+    } else {
+      ws.alloc(w.yhat, sizeof(double) * n);
+      hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, w.yhat.as<double>(), n, 0.0);
+    }
   }
   float *ZD[2] = {nullptr, nullptr};
   if (pat32) {
-    w.ZD0.alloc(sizeof(float) * n);
-    w.ZD1.alloc(sizeof(float) * n);
+    ws.alloc(w.ZD0, sizeof(float) * n);
+    ws.alloc(w.ZD1, sizeof(float) * n);
     ZD[0] = w.ZD0.as<float>();
     ZD[1] = w.ZD1.as<float>();
-    hipLaunchKernelGGL(scale_into_f32, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), b0, ZD[0], n);
+    if (sn)
+      hipLaunchKernelGGL(scale_into_f32_32, dim3(gN), dim3(256), 0, s, w.colscale32.as<float>(),
+                         static_cast<float>(x0), ZD[0], n);
+    else
+      hipLaunchKernelGGL(scale_into_f32, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), b0, ZD[0], n);
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, ZD[1], n, 0.0f);
   }
   if (pattern || (mixed && m.count_stochastic())) {
-    w.Z.alloc(sizeof(double) * n);
+    ws.alloc(w.Z, sizeof(double) * n);
     hipLaunchKernelGGL(scale_into, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), X[0], w.Z.as<double>(), n);
+  }
+  // value-free FP64 sum-normalised runs write the next z beside the next
+  // iterate, so z alternates between two buffers like the iterate
+  double *ZP[2] = {w.Z.as<double>(), nullptr};
+  if (sn && pattern) {
+    ws.alloc(w.Z1, sizeof(double) * n);
+    ZP[1] = w.Z1.as<double>();
   }
   TBSLA_HIP_CHECK(hipGetLastError());
   // the set-up copies read pageable host memory (state, node set, 1/count)
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
 
-  hipEvent_t e0, e1;
-  TBSLA_HIP_CHECK(hipEventCreate(&e0));
-  TBSLA_HIP_CHECK(hipEventCreate(&e1));
-  TBSLA_HIP_CHECK(hipEventRecord(e0, s));
+  // device time of the loop: HIP events, or (allocation-free runs, which
+  // create no events) the host clock between two stream synchronisations
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const auto h0 = std::chrono::steady_clock::now();
+  if (!arena) {
+    TBSLA_HIP_CHECK(hipEventCreate(&e0));
+    TBSLA_HIP_CHECK(hipEventCreate(&e1));
+    TBSLA_HIP_CHECK(hipEventRecord(e0, s));
+  }
   const int check = fused ? std::max(1, p.check_every) : 1;
   int next = check;  // iterations in the next launch batch
   PrState hs{};
@@ -1005,32 +1349,45 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     for (int k = 0; k < batch; k++) {
       const int it = launched + k;
       double *xin = X[it & 1], *xout = X[(it + 1) & 1];
-      if (fused) {
+      if (sn) {
+        const SnPartials sp{w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>()};
+        if (mixed) {
+          MxBufs b;
+          b.am = AnchorMove{w.xhat.as<double>(), D[it & 1], mixed_pattern ? Z : nullptr, colscale,
+                            w.hsum.as<double>(), w.hcomp.as<double>()};
+          b.xd = pat32 ? static_cast<const void *>(ZD[it & 1]) : static_cast<const void *>(D[it & 1]);
+          b.w = w.W.as<float>();
+          b.dold = D[it & 1];
+          b.dnew = D[(it + 1) & 1];
+          b.zdnew = pat32 ? ZD[(it + 1) & 1] : nullptr;
+          b.colscale32 = w.colscale32.as<float>();
+          if (pat32)
+            launch_pb_mx<dev::Pattern, dev::Pattern, float, true>(tmode, *pbm, -1, -1, b, tp, st, sp, s);
+          else if (mixed_pattern && p.precision == PrPrecision::F32)
+            launch_pb_mx<dev::Pattern, float, float, false>(tmode, *pbm, -1, 1, b, tp, st, sp, s);
+          else if (mixed_pattern)
+            launch_pb_mx<dev::Pattern, dev::bf16, float, false>(tmode, *pbm, -1, 2, b, tp, st, sp, s);
+          else if (p.precision == PrPrecision::F32)
+            launch_pb_mx<double, float, float, false>(tmode, *pbm, 0, 1, b, tp, st, sp, s);
+          else
+            launch_pb_mx<double, dev::bf16, float, false>(tmode, *pbm, 0, 2, b, tp, st, sp, s);
+        } else if (pattern) {
+          launch_pb_sn<dev::Pattern, true>(tmode, *pbm, -1, xin, ZP[it & 1], tp, xout, ZP[(it + 1) & 1], colscale, st,
+                                           sp, s);
+        } else {
+          launch_pb_sn<double, false>(tmode, *pbm, 0, xin, nullptr, tp, xout, nullptr, nullptr, st, sp, s);
+        }
+        hipLaunchKernelGGL(pr_finalize_sn, dim3(1), dim3(kTileThreads), 0, s, sp.perr, sp.ps, sp.pc, pbm->nbins,
+                           mixed ? w.hsum.as<double>() : nullptr, mixed ? w.hcomp.as<double>() : nullptr, pbm->nsb,
+                           st);
+      } else if (fused) {
         if (mixed) {
           float *dcur = D[it & 1], *dnext = D[(it + 1) & 1];
           float *zdcur = ZD[it & 1], *zdnext = ZD[(it + 1) & 1];
           double *xhat = w.xhat.as<double>(), *yhat = w.yhat.as<double>();
-          if (pbm) {
-            // one scatter and one gather per iteration: the anchor move and
-            // the FP64 anchor product with the update when the state asks for
-            // an anchor, else the delta product with the update
-            const AnchorMove am{xhat, dcur, mixed_pattern ? Z : nullptr, colscale};
-            if (pat32)
-              launch_pb_mixed_update<dev::Pattern, dev::Pattern, float, float>(tmode, *pbm, -1, -1, am, zdcur, yhat,
-                                                                                tp, xout, st, slots, s);
-            else if (mixed_pattern && p.precision == PrPrecision::F32)
-              launch_pb_mixed_update<dev::Pattern, float, float, float>(tmode, *pbm, -1, 1, am, dcur, yhat, tp, xout,
-                                                                        st, slots, s);
-            else if (mixed_pattern)
-              launch_pb_mixed_update<dev::Pattern, dev::bf16, float, float>(tmode, *pbm, -1, 2, am, dcur, yhat, tp,
-                                                                            xout, st, slots, s);
-            else if (p.precision == PrPrecision::F32)
-              launch_pb_mixed_update<double, float, float, float>(tmode, *pbm, 0, 1, am, dcur, yhat, tp, xout, st,
-                                                                  slots, s);
-            else
-              launch_pb_mixed_update<double, dev::bf16, float, float>(tmode, *pbm, 0, 2, am, dcur, yhat, tp, xout,
-                                                                      st, slots, s);
-          } else {
+          {
+            // (mixed runs take the tile products here: with propagation-
+            // blocked products they run the sum-normalised branch above)
             hipLaunchKernelGGL(pr_anchor_move, dim3(gN), dim3(256), 0, s, xhat, dcur, pat32 ? zdcur : dcur,
                                mixed_pattern ? Z : nullptr, colscale, n, st);
             if (mixed_pattern)
@@ -1099,7 +1456,8 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
     if (hs.done) break;
     next = next_batch(hs, check);
   }
-  TBSLA_HIP_CHECK(hipEventRecord(e1, s));
+  if (!arena) TBSLA_HIP_CHECK(hipEventRecord(e1, s));
+  const double host_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
   const int it = hs.it;
   // final sum normalisation (Matrix.cpp:52-59), compensated
   const double *a = mixed ? w.xhat.as<double>() : X[it & 1];
@@ -1112,15 +1470,86 @@ PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, d
   TBSLA_HIP_CHECK(hipGetLastError());
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   float ms = 0;
-  TBSLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (!arena) {
+    TBSLA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   rep.iterations = it;
   rep.anchors = hs.anchors;
   rep.last_error = hs.err;
   rep.bound = hs.bound;
-  rep.seconds = ms * 1e-3;
+  rep.seconds = arena ? host_seconds : ms * 1e-3;
   return rep;
+}
+
+}  // namespace
+
+PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, double *out_dev, hipStream_t s) {
+  return run_page_rank(m, p, out_dev, s, nullptr, 0);
+}
+
+// ---- allocation-free PageRank ----------------------------------------------
+// The scratch memory of page_rank_device_noalloc: one fixed buffer in each
+// GPU's copy of this code object, held until the process ends.  Host code
+// never passes `scratch` itself (in host code it names a host-side stand-in):
+// it uses the device address hipGetSymbolAddress returns for the current
+// device, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+unsigned char *scratch_of_current_device() {
+  int dev = 0;
+  TBSLA_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("tbsla: device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    TBSLA_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw std::runtime_error("tbsla: no device address for the scratch buffer");
+    g_scratch[dev] = static_cast<unsigned char *>(ptr);
+  }
+  return g_scratch[dev];
+}
+
+std::size_t slice(std::size_t bytes) { return (std::max<std::size_t>(bytes, 1) + 255) & ~std::size_t(255); }
+
+}  // namespace
+
+std::size_t page_rank_noalloc_bytes(long n, int n_nodes) {
+  const std::size_t gN = grid_for(n);
+  std::size_t b = slice(sizeof(PrState)) + slice(sizeof(unsigned long long) * kMaxSlots) +
+                  3 * slice(sizeof(double) * gN) + slice(sizeof(double)) +
+                  2 * slice(sizeof(double) * static_cast<std::size_t>(n));
+  if (n_nodes > 0) b += slice(static_cast<std::size_t>(n)) + slice(sizeof(int) * static_cast<std::size_t>(n_nodes));
+  return b;
+}
+
+PageRankReport page_rank_device_noalloc(const cpp::Matrix &m, const PageRankParams &p, double *out_dev,
+                                        hipStream_t s) {
+  if (p.precision != PrPrecision::F64)
+    throw std::invalid_argument("tbsla: the allocation-free PageRank computes in FP64 (precision TBSLA_F64)");
+  if (p.gather == 2)
+    throw std::invalid_argument("tbsla: the allocation-free PageRank uses the fixed-order tile products "
+                                "(gather 0 or 1): the propagation-blocked layout is built, and allocated, on first use");
+  const long n = m.get_n_col();
+  const int nodes = p.personalized && !p.teleport ? std::max(p.n_nodes, 0) : 0;
+  const std::size_t need = page_rank_noalloc_bytes(n, nodes);
+  if (need > kNoAllocScratchBytes)
+    throw std::invalid_argument("tbsla: allocation-free PageRank on " + std::to_string(n) + " nodes needs " +
+                                std::to_string(need) + " bytes of scratch; the buffer holds " +
+                                std::to_string(kNoAllocScratchBytes));
+  if (!mirror_if_built(m, Dtype::F64))
+    throw std::invalid_argument("tbsla: the allocation-free PageRank needs the FP64 mirror: call "
+                                "tbsla_hip_prepare(m, TBSLA_F64) first");
+  PageRankParams q = p;
+  q.gather = 1;
+  return run_page_rank(m, q, out_dev, s, scratch_of_current_device(), kNoAllocScratchBytes);
 }
 
 PageRankReport page_rank(const cpp::Matrix &m, const PageRankParams &p, double *out) {

@@ -75,5 +75,17 @@ PageRankReport page_rank(const cpp::Matrix &m, const PageRankParams &p, double *
 // Result left in device memory `out_dev` (n_col doubles) on `stream`.
 PageRankReport page_rank_device(const cpp::Matrix &m, const PageRankParams &p, double *out_dev, hipStream_t stream);
 
+// page_rank_device with no device allocation: every buffer of the run is a
+// slice of a fixed static scratch buffer of kNoAllocScratchBytes per GPU.
+// FP64 with the fixed-order tile products only; the FP64 mirror must exist
+// (built by an earlier product or tbsla_hip_prepare).  Throws
+// std::invalid_argument when the run needs more scratch than the buffer
+// holds (page_rank_noalloc_bytes).  Calls that overlap on one GPU share the
+// buffer and race.
+constexpr std::size_t kNoAllocScratchBytes = std::size_t(40) << 20;
+std::size_t page_rank_noalloc_bytes(long n, int n_nodes);
+PageRankReport page_rank_device_noalloc(const cpp::Matrix &m, const PageRankParams &p, double *out_dev,
+                                        hipStream_t stream);
+
 }  // namespace hip
 }  // namespace tbsla

@@ -8,6 +8,7 @@
 
 #include "../device/products.hpp"
 #include "fill_common.hpp"
+#include "../device/generate.hpp"
 
 namespace tbsla {
 namespace cpp {
@@ -285,6 +286,39 @@ void MatrixCOO::fill_cqmat_values(int n_row, int n_col, int c, double q, unsigne
   }, 1 << 14);
 }
 
+// fill_cqmat_values / fill_cqmat_stochastic with the slice's entries
+// generated on the GPU (device/generate.hip), one thread per stream entry;
+// metadata and nnz follow the host path above.
+void MatrixCOO::fill_cqmat_gpu(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc, int NR,
+                               int NC, std::uint64_t value_seed, bool stochastic) {
+  this->n_row = n_row;
+  this->n_col = n_col;
+  this->pr = pr;
+  this->pc = pc;
+  this->NR = NR;
+  this->NC = NC;
+  values.clear();
+  col.clear();
+  row.clear();
+  count_stochastic_ = stochastic;  // as the host path, which sets it whatever the slice holds
+  invalidate_device();
+  ln_row = NR;
+  ln_col = NC;
+  f_row = 0;
+  f_col = 0;
+  const gen::CqmatStream st(n_row, n_col, c);
+  long gnv = 0;
+  for (long i = 0; i < std::min<long>(static_cast<long>(n_col) - st.cc + 1, n_row); i++) gnv += st.cc;
+  for (long i = 0; i < std::min<long>(n_row, n_col) - std::min<long>(static_cast<long>(n_col) - st.cc + 1, n_row); i++)
+    gnv += st.cc - i - 1;
+  if (gnv == 0) return;
+  const long s = static_cast<long>(gen::pflv(static_cast<std::size_t>(gnv), pr * NC + pc, NR * NC));
+  const long n = static_cast<long>(gen::lnv(static_cast<std::size_t>(gnv), pr * NC + pc, NR * NC));
+  nnz = n;
+  host::CqmatBlock b(n_row, n_col, c, q, seed_mult, stochastic ? 0 : value_seed, 0, n_row, 0, n_col);
+  hip::generate_cqmat_coo(b, s, n, stochastic, row, col, values);
+}
+
 // MatrixCOO.cpp:377-383
 void MatrixCOO::fill_cqmat_stochastic(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc,
                                       int NR, int NC) {
@@ -517,6 +551,40 @@ void MatrixSCOO::fill_cqmat_values(int n_row, int n_col, int c, double q, unsign
   row = std::move(e.rows);
   col = std::move(e.cols);
   values = std::move(e.vals);
+}
+
+// fill_cqmat_values / fill_cqmat_stochastic with the block's entries
+// generated on the GPU (device/generate.hip); metadata, nnz and the early
+// return follow the host paths.
+void MatrixSCOO::fill_cqmat_gpu(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc, int NR,
+                                int NC, std::uint64_t value_seed, bool stochastic) {
+  this->n_row = n_row;
+  this->n_col = n_col;
+  this->pr = pr;
+  this->pc = pc;
+  this->NR = NR;
+  this->NC = NC;
+  values.clear();
+  col.clear();
+  row.clear();
+  count_stochastic_ = false;
+  invalidate_device();
+  ln_row = static_cast<int>(gen::lnv(n_row, pr, NR));
+  f_row = static_cast<int>(gen::pflv(n_row, pr, NR));
+  ln_col = static_cast<int>(gen::lnv(n_col, pc, NC));
+  f_col = static_cast<int>(gen::pflv(n_col, pc, NC));
+  host::CqmatBlock b(n_row, n_col, c, q, seed_mult, stochastic ? 0 : value_seed, f_row, ln_row, f_col, ln_col);
+  const long lim3 = b.lim3();
+  std::vector<int> rp;
+  hip::generate_cqmat_scoo(b, std::max<long>(lim3, f_row) - f_row, stochastic, rp, row, col, values);
+  const long full_end = std::min(lim3, b.s.min_);
+  nnz = host::overcount_nnz(b, full_end > f_row ? rp[static_cast<std::size_t>(full_end - f_row)] : 0);
+  if (b.candidates_in_block() == 0) {
+    row.clear();
+    col.clear();
+    values.clear();
+  }
+  count_stochastic_ = stochastic;  // as the host path, which sets it whatever the block holds
 }
 
 // MatrixSCOO.cpp:393-399

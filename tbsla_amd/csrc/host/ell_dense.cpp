@@ -7,6 +7,7 @@
 
 #include "../device/products.hpp"
 #include "fill_common.hpp"
+#include "../device/generate.hpp"
 #include "../device/convert.hpp"
 
 namespace tbsla {
@@ -243,6 +244,50 @@ void MatrixELL::fill_cqmat_values(int n_row, int n_col, int c, double q, unsigne
 }
 
 // MatrixELL.cpp:394-404 (padding slots hold 0.0 and are left alone)
+// fill_cqmat_values / fill_cqmat_stochastic with the block's entries
+// generated and laid out on the GPU (device/generate.hip); metadata, nnz,
+// the slot count (with the quirk of MatrixELL.cpp:338-342) and the early
+// return follow the host path above.
+void MatrixELL::fill_cqmat_gpu(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc, int NR,
+                               int NC, std::uint64_t value_seed, bool stochastic) {
+  this->n_row = n_row;
+  this->n_col = n_col;
+  this->pr = pr;
+  this->pc = pc;
+  this->NR = NR;
+  this->NC = NC;
+  values.clear();
+  columns.clear();
+  count_stochastic_ = false;
+  invalidate_device();
+  ln_row = static_cast<int>(gen::lnv(n_row, pr, NR));
+  f_row = static_cast<int>(gen::pflv(n_row, pr, NR));
+  ln_col = static_cast<int>(gen::lnv(n_col, pc, NC));
+  f_col = static_cast<int>(gen::pflv(n_col, pc, NC));
+  host::CqmatBlock b(n_row, n_col, c, q, seed_mult, stochastic ? 0 : value_seed, f_row, ln_row, f_col, ln_col);
+  const long lim3 = b.lim3();
+  const long nrows = std::max<long>(lim3, f_row) - f_row;
+  const long fill_end = std::min<long>(n_row, static_cast<long>(f_row) + ln_row);
+  long kept = 0;
+  auto width = [&](const std::vector<int> &rp) {
+    int K = 0;
+    kept = 0;
+    for (long r = f_row; r < lim3; r++) {
+      int nbc = rp[static_cast<std::size_t>(r - f_row) + 1] - rp[static_cast<std::size_t>(r - f_row)];
+      kept += nbc;
+      if (r >= b.s.min_)
+        for (long j = b.s.candidates(r); j < b.s.cc; j++)
+          if (j >= f_col && j < static_cast<long>(f_col) + ln_col) nbc++;
+      K = std::max(K, nbc);
+    }
+    return K;
+  };
+  hip::generate_cqmat_ell(b, nrows, std::max(0L, fill_end - f_row), stochastic, width, max_col, columns, values,
+                          b.candidates_in_block() != 0);
+  nnz = kept;
+  if (stochastic && !values.empty()) count_stochastic_ = true;
+}
+
 void MatrixELL::fill_cqmat_stochastic(int n_row, int n_col, int c, double q, unsigned int seed_mult, int pr, int pc,
                                       int NR, int NC) {
   fill_cqmat(n_row, n_col, c, q, seed_mult, pr, pc, NR, NC);

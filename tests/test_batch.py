@@ -89,6 +89,26 @@ def test_batch_graph_replay_equals_plain_launches(tb, oracle, gpu):
     assert np.array_equal(a, b)
 
 
+def test_batch_autotuner_trials_and_cache(tb, oracle, gpu):
+    """The autotuner's trials are iterations of the run (the result equals a
+    run with a fixed shape bit for bit); a second batch of the same
+    signature takes the cached shape with no trial iterations."""
+    spec, mats, _ = small_batch(tb, oracle)
+    b1 = tb.PageRankBatch("f64")
+    b2 = tb.PageRankBatch("f64")
+    for m, sp in zip(mats, spec):
+        b1.add_graph(m, sp[4])
+        b2.add_graph(m, sp[4])
+    o1, r1, i1 = b1.run(epsilon=1e-10, max_iteration=300, autotune=True)
+    o2, r2, i2 = b2.run(epsilon=1e-10, max_iteration=300, autotune=True)
+    if not r1["tune_cached"]:  # (an earlier test may have tuned this signature already)
+        assert r1["tune_iterations"] >= 6 and r1["tune_iterations"] % 2 == 0
+    assert r2["tune_cached"] and r2["tune_iterations"] == 0 and r2["config"] == r1["config"]
+    _, c, _, ic = run_batch(tb, mats, spec, "f64", epsilon=1e-10, max_iteration=300, graph=True, autotune=False)
+    assert np.array_equal(i1, ic) and np.array_equal(o1.cpu().numpy(), c)
+    assert np.array_equal(o1.cpu().numpy(), o2.cpu().numpy())
+
+
 def test_batch_f32_within_tolerance(tb, oracle, gpu):
     """FP32 values, iterates and products (FP64 residual and sums), stopped by
     the sum-normalised bound at 1e-6: within 1e-5 L1 of the FP64 result."""

@@ -1,5 +1,6 @@
-"""GPU generation of cqmat CSR blocks (tbsla_matrix_fill_cqmat_gpu) and GPU
-COO -> CSR / ELL conversion (tbsla_matrix_convert_gpu).
+"""GPU generation of cqmat COO, SCOO, CSR and ELL matrices
+(tbsla_matrix_fill_cqmat_gpu) and GPU COO -> CSR / ELL conversion
+(tbsla_matrix_convert_gpu).
 
 The device restates the reference's entry stream (values_generation.cpp:26-45)
 with its own restatement of glibc's rand_r.  Every array must equal the host
@@ -26,40 +27,46 @@ def same(a, b):
         assert xa[k].dtype == xb[k].dtype and np.array_equal(xa[k], xb[k]), k
 
 
-def test_gpu_cqmat_grid_equals_host(tb, gpu):
+GPU_FMTS = ["CSR", "ELL", "SCOO", "COO"]
+
+
+@pytest.mark.parametrize("fmt", GPU_FMTS)
+def test_gpu_cqmat_grid_equals_host(tb, gpu, fmt):
     """The reference test grid (shapes, q in 0..1, seeds 0..3, 2D blocks), with
     and without the value overlay, plus shapes where the partial rows and the
     c > n_col clipping matter."""
     shapes = list(CQMAT_SHAPES) + [(40, 17, 6), (17, 40, 6), (30, 30, 31), (64, 64, 1), (9, 9, 9)]
+    cls = tb.FORMATS[fmt]
     for (nr, nc, c) in shapes:
         for s in range(4):
             for q in grid_q()[::3] + [1.0]:
                 for blk in BLOCKS:
                     for vs in (0, 7):
-                        h = tb.MatrixCSR().fill_cqmat(nr, nc, c, q, s, *blk, value_seed=vs)
-                        d = tb.MatrixCSR().fill_cqmat(nr, nc, c, q, s, *blk, value_seed=vs, device=True)
+                        h = cls().fill_cqmat(nr, nc, c, q, s, *blk, value_seed=vs)
+                        d = cls().fill_cqmat(nr, nc, c, q, s, *blk, value_seed=vs, device=True)
                         same(h, d)
                     if c > 0:
-                        h = tb.MatrixCSR().fill_cqmat_stochastic(nr, nc, c, q, s, *blk)
-                        d = tb.MatrixCSR().fill_cqmat_stochastic(nr, nc, c, q, s, *blk, device=True)
+                        h = cls().fill_cqmat_stochastic(nr, nc, c, q, s, *blk)
+                        d = cls().fill_cqmat_stochastic(nr, nc, c, q, s, *blk, device=True)
                         same(h, d)
-                        assert d.count_stochastic() == h.count_stochastic()
+                        assert d.count_stochastic() == h.count_stochastic(), (fmt, nr, nc, c, s, q, blk)
 
 
-def test_gpu_cqmat_larger_blocks_equal_host(tb, gpu):
+@pytest.mark.parametrize("fmt", GPU_FMTS)
+def test_gpu_cqmat_larger_blocks_equal_host(tb, gpu, fmt):
+    cls = tb.FORMATS[fmt]
     for (n, c, q, s, blk) in [(100000, 16, 0.1, 42, (1, 2, 4, 4)), (70000, 12, 0.3, 1001, (0, 0, 1, 1)),
                               (50000, 8, 0.5, 7, (2, 1, 3, 3))]:
-        same(tb.MatrixCSR().fill_cqmat(n, n, c, q, s, *blk, value_seed=42),
-             tb.MatrixCSR().fill_cqmat(n, n, c, q, s, *blk, value_seed=42, device=True))
-        same(tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s, *blk),
-             tb.MatrixCSR().fill_cqmat_stochastic(n, n, c, q, s, *blk, device=True))
+        same(cls().fill_cqmat(n, n, c, q, s, *blk, value_seed=42),
+             cls().fill_cqmat(n, n, c, q, s, *blk, value_seed=42, device=True))
+        same(cls().fill_cqmat_stochastic(n, n, c, q, s, *blk),
+             cls().fill_cqmat_stochastic(n, n, c, q, s, *blk, device=True))
 
 
-def test_gpu_generation_rejects_other_formats(tb, gpu):
+def test_gpu_generation_rejects_dense(tb, gpu):
     from tbsla_amd import _lib
-    for fmt in ("COO", "SCOO", "ELL", "DENSE"):
-        with pytest.raises(_lib.TbslaError):
-            tb.FORMATS[fmt]().fill_cqmat(10, 10, 3, 0.1, 1, device=True)
+    with pytest.raises(_lib.TbslaError):
+        tb.MatrixDENSE().fill_cqmat(10, 10, 3, 0.1, 1, device=True)
     with pytest.raises(_lib.TbslaError):
         tb.MatrixCSR().fill_cqmat(10, 10, 3, 0.1, 1, 2, 0, 2, 2, device=True)  # pr >= NR
 
@@ -73,6 +80,9 @@ def test_gpu_generation_matches_reference_digests(tb, digests, gpu):
         "stoch2^24c8q0.5s7/CSR": lambda: tb.MatrixCSR().fill_cqmat_stochastic(2 ** 24, 2 ** 24, 8, 0.5, 7, device=True),
         "stoch2^22c16q0.3s1256/CSR": lambda: tb.MatrixCSR().fill_cqmat_stochastic(2 ** 22, 2 ** 22, 16, 0.3, 1256, device=True),
         "stoch2^20c16q0.2s3/CSR": lambda: tb.MatrixCSR().fill_cqmat_stochastic(2 ** 20, 2 ** 20, 16, 0.2, 3, device=True),
+        "cqmat2^24c16q0.1s42/ELL/block0.0.4.4": lambda: tb.MatrixELL().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4, device=True),
+        "cqmat2^24c16q0.1s42/SCOO/block0.0.4.4": lambda: tb.MatrixSCOO().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4, device=True),
+        "cqmat2^24c16q0.1s42/COO/rank0of16": lambda: tb.MatrixCOO().fill_cqmat(2 ** 24, 2 ** 24, 16, 0.1, 42, 0, 0, 4, 4, device=True),
     }
     for key, make in cases.items():
         t = time.time()
@@ -133,6 +143,14 @@ def test_gpu_conversion_equals_host(tb, gpu):
         coo = tb.MatrixCOO().fill_cqmat(4000, 4000, 9, 0.3, 5, *blk, value_seed=3)
         for cls in (tb.MatrixCSR, tb.MatrixELL):
             same(cls(coo), cls(coo, device=True))
+    # negative column indices (the sort key then holds all 32 column bits,
+    # sign flipped) and a row count that needs several 8-bit digit passes
+    n2 = 70000
+    row = rng.integers(0, n2, 200000).astype(np.int32)
+    col = rng.integers(-300, 300, 200000).astype(np.int32)
+    coo = tb.MatrixCOO(n2, n2, rng.random(200000), row, col)
+    for cls in (tb.MatrixCSR, tb.MatrixELL):
+        same(cls(coo), cls(coo, device=True))
     empty = tb.MatrixCOO(10, 10, np.zeros(0), np.zeros(0, np.int32), np.zeros(0, np.int32))
     for cls in (tb.MatrixCSR, tb.MatrixELL):
         same(cls(empty), cls(empty, device=True))

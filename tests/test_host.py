@@ -223,7 +223,7 @@ def test_c_abi_errors_without_device_calls(tb):
     with pytest.raises(_lib.TbslaError):
         m.fill_cqmat(10, 10, 3, 0.1, 1, 2, 0, 2, 2, device=True)  # pr >= NR
     with pytest.raises(_lib.TbslaError):
-        tb.MatrixELL().fill_cqmat(10, 10, 3, 0.1, 1, device=True)  # CSR only
+        tb.MatrixDENSE().fill_cqmat(10, 10, 3, 0.1, 1, device=True)  # not a GPU generator
     assert _lib.lib.tbsla_matrix_convert_gpu(m._h, 2) is None  # needs a COO
     assert b"needs a COO" in _lib.lib.tbsla_last_error()
     coo = tb.MatrixCOO().fill_cqmat(10, 10, 3, 0.1, 1)
