@@ -194,7 +194,7 @@ PSTAR_ITERATIONS = 48   # the reference iteration is in its limit 2-cycle from ~
 CPU_PR_SAMPLE = 5       # reference page_rank iterations timed on the host
 
 
-def leg_pagerank(torch, tb, dev):
+def leg_pagerank(torch, tb, dev, cpu=True):
     """Every precision to the sum-normalised 1e-10 bound (median of three
     timed runs each), then the FP64 fixed point p* on the host (the oracle's
     restatement of the reference iteration, 48 iterations; its SHA-256 is
@@ -229,6 +229,10 @@ def leg_pagerank(torch, tb, dev):
                        gather="tiles")
     out = {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7), beta 0.85",
            "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr}
+    if not cpu:  # (--no-cpu: no host legs; the GPU's own FP64 limit is the reference point)
+        for prec, o in outs.items():
+            pr[prec]["l1_vs_gpu_fp64_limit"] = float((o - lim).abs().sum().item())
+        return out
     log("FP64 fixed point on the host (oracle) and the reference CPU page_rank")
     d = tempfile.mkdtemp(prefix="tbsla_pr_")
     try:
@@ -417,7 +421,7 @@ def main():
     if not args.no_formats:
         out["formats"] = leg_formats(torch, tb, dev, args.steps)
     if not args.no_pagerank:
-        out["pagerank"] = leg_pagerank(torch, tb, dev)
+        out["pagerank"] = leg_pagerank(torch, tb, dev, cpu=not args.no_cpu)
         cpu_pr = out["pagerank"].pop("cpu_baseline", None)
         if out.get("cpu_baseline") and cpu_pr:
             out["cpu_baseline"]["pagerank"] = cpu_pr
@@ -438,12 +442,11 @@ def main():
         mixed = {k: v["speedup_vs_fp64"] for k, v in var.items() if k in ("f32", "bf16", "pattern_f32")}
         best = max(mixed, key=mixed.get)
         tg["pagerank_mixed_speedup_vs_fp64"] = mixed
-        tg["pagerank_mixed_best"] = {"variant": best, "speedup_vs_fp64": mixed[best],
-                                     "l1_vs_oracle_pstar": var[best]["l1_vs_oracle_pstar"]}
+        l1key = "l1_vs_oracle_pstar" if "l1_vs_oracle_pstar" in var[best] else "l1_vs_gpu_fp64_limit"
+        tg["pagerank_mixed_best"] = {"variant": best, "speedup_vs_fp64": mixed[best], l1key: var[best][l1key]}
         stored = {k: v for k, v in mixed.items() if k in ("f32", "bf16")}
         bs = max(stored, key=stored.get)
-        tg["pagerank_stored_values_best"] = {"variant": bs, "speedup_vs_fp64": stored[bs],
-                                             "l1_vs_oracle_pstar": var[bs]["l1_vs_oracle_pstar"]}
+        tg["pagerank_stored_values_best"] = {"variant": bs, "speedup_vs_fp64": stored[bs], l1key: var[bs][l1key]}
     out["targets"] = tg
 
     print(json.dumps(out), flush=True)

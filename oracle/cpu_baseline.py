@@ -84,21 +84,10 @@ def spmv_leg(d: str) -> dict:
     m = R.from_arrays("CSR", meta["n_row"], meta["n_col"], v, rp, ci)
     del v, rp, ci
     reps = 10
-    r = np.zeros(meta["n_row"])
-    R.Ax_repeat(m, r, x, 1)
-    ax = []
-    for _ in range(reps):
-        t = time.perf_counter()
-        R.Ax_repeat(m, r, x, 1)
-        ax.append(time.perf_counter() - t)
-    y = R.Ax_repeat(m, np.zeros(meta["n_row"]), x, 1)
-    sp = []
-    m.spmv(x)
-    for _ in range(reps):
-        t = time.perf_counter()
-        m.spmv(x)
-        sp.append(time.perf_counter() - t)
-    return {"Ax": _stats(ax), "spmv_api": _stats(sp), "y_sha256": hashlib.sha256(y.tobytes()).hexdigest()}
+    t, y = R.Ax_times(m, meta["n_row"], x, reps + 1)  # the first call warms up
+    ts = R.spmv_times(m, x, reps + 1)
+    return {"Ax": _stats(list(t[1:])), "spmv_api": _stats(list(ts[1:])),
+            "y_sha256": hashlib.sha256(y.tobytes()).hexdigest()}
 
 
 def pagerank_leg(d: str) -> dict:

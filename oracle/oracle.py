@@ -379,6 +379,8 @@ class _Reference(_Backend):
         L.ref_Ax.argtypes = [C.c_void_p, _dp, C.c_size_t, _dp, C.c_size_t, C.c_int]
         L.ref_Ax_repeat.argtypes = [C.c_void_p, _dp, C.c_size_t, _dp, C.c_size_t, C.c_int]
         L.ref_a_axpx.argtypes = [C.c_void_p, _dp, C.c_size_t, _dp]
+        L.ref_Ax_times.argtypes = [C.c_void_p, _dp, C.c_size_t, _dp, C.c_size_t, C.c_int, _dp]
+        L.ref_spmv_times.argtypes = [C.c_void_p, _dp, C.c_size_t, C.c_int, _dp]
         L.ref_page_rank.restype = C.c_int
         L.ref_page_rank.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, _dp]
         L.ref_read_mm.restype = C.c_int
@@ -467,6 +469,21 @@ class _Reference(_Backend):
         if not self.lib.ref_read_mm(m.h, path.encode()):
             raise ValueError("MatrixFormatReadException")
         return m
+
+    def Ax_times(self, m: Mat, nr, x, reps):
+        """(seconds of each of `reps` Ax calls on preallocated vectors, last result)."""
+        r = np.zeros(nr)
+        t = np.zeros(reps)
+        x = _f64(x)
+        self.lib.ref_Ax_times(m.h, r, r.size, x, x.size, reps, t)
+        return t, r
+
+    def spmv_times(self, m: Mat, x, reps):
+        """Seconds of each of `reps` spmv() calls (each allocates its result)."""
+        t = np.zeros(reps)
+        x = _f64(x)
+        self.lib.ref_spmv_times(m.h, x, x.size, reps, t)
+        return t
 
     def Ax_repeat(self, m: Mat, r, x, reps):
         r = _f64(r).copy()

@@ -13,6 +13,7 @@
 #include <tbsla/cpp/MatrixELL.hpp>
 #include <tbsla/cpp/MatrixSCOO.hpp>
 
+#include <chrono>
 #include <cstring>
 #include <sstream>
 #include <string>
@@ -129,6 +130,27 @@ void ref_Ax_repeat(void *m, double *r, size_t nr, const double *x, size_t nx, in
   std::vector<double> v(x, x + nx), rr(nr, 0.0);
   for (int k = 0; k < reps; k++) static_cast<Matrix *>(m)->Ax(rr, v, 0);
   std::memcpy(r, rr.data(), nr * sizeof(double));
+}
+// Timings for the CPU baseline: `reps` calls of Ax on preallocated vectors
+// (the kernel) and of spmv() (the API: a new zeroed n_row result per call),
+// each timed alone; r receives the last Ax result.
+void ref_Ax_times(void *m, double *r, size_t nr, const double *x, size_t nx, int reps, double *seconds) {
+  std::vector<double> v(x, x + nx), rr(nr, 0.0);
+  for (int k = 0; k < reps; k++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    static_cast<Matrix *>(m)->Ax(rr, v, 0);
+    seconds[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  std::memcpy(r, rr.data(), nr * sizeof(double));
+}
+void ref_spmv_times(void *m, const double *x, size_t nx, int reps, double *seconds) {
+  std::vector<double> v(x, x + nx);
+  for (int k = 0; k < reps; k++) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> r = static_cast<Matrix *>(m)->spmv(v, 0);
+    seconds[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (r.empty()) seconds[k] = -1;
+  }
 }
 void ref_a_axpx(void *m, const double *x, size_t nx, double *out) {
   std::vector<double> v(x, x + nx);

@@ -96,7 +96,7 @@ SIGNATURES = {
     "tbsla_hip_page_rank": (_i, [_vp, C.POINTER(PageRankParams), _dp, _sz, _vp, C.POINTER(PageRankReport)]),
     "tbsla_hip_page_rank_ex": (_i, [_vp, C.POINTER(PageRankParams), _vp, _sz, _vp, _sz, _vp, _sz, _vp,
                                     C.POINTER(PageRankReport)]),
-    "tbsla_hip_page_rank_noalloc": (_i, [_vp, C.POINTER(PageRankParams), _vp, _sz, _vp, C.POINTER(PageRankReport)]),
+    "tbsla_hip_page_rank_noalloc": (_i, [_vp, C.POINTER(PageRankParams), _dp, _sz, _vp, C.POINTER(PageRankReport)]),
     "tbsla_hip_page_rank_adjoint": (_i, [_vp, C.POINTER(AdjointParams), _vp, _sz, _vp, _sz, _vp, _vp, _sz, _vp,
                                          _vp, _vp, C.POINTER(AdjointReport)]),
     "tbsla_batch_create": (_vp, [_i]),

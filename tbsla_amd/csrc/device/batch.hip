@@ -1121,11 +1121,17 @@ BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hi
     TBSLA_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
   std::vector<RhsState> hs(static_cast<std::size_t>(n_rhs_total));
+  // the state after the autotuner's trial iterations (the loop below may
+  // have nothing left to run)
+  TBSLA_HIP_CHECK(hipMemcpyAsync(hs.data(), rs.get(), hs.size() * sizeof(RhsState), hipMemcpyDeviceToHost, s));
+  TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+  bool all_done = true;
+  for (const auto &st : hs) all_done = all_done && st.done;
   hipEvent_t e0, e1;
   TBSLA_HIP_CHECK(hipEventCreate(&e0));
   TBSLA_HIP_CHECK(hipEventCreate(&e1));
   TBSLA_HIP_CHECK(hipEventRecord(e0, s));
-  for (int launched = done_its; launched < p.max_iteration; launched += C) {
+  for (int launched = done_its; !all_done && launched < p.max_iteration; launched += C) {
     if (exec)
       TBSLA_HIP_CHECK(hipGraphLaunch(exec, s));
     else

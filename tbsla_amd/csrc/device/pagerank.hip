@@ -44,10 +44,9 @@ struct PrState {
   double bound, bound_prev, bound_anchor;
   double anchor_tau, floor_eps;
   // sum-normalised propagation-blocked runs: 1 / sum of the last iterate
-  // (the next iterate's scale, FP64; the scale of a mixed run's next anchor
-  // move), the sum of the anchor, and the teleport term of the anchor's
-  // iteration
-  double inv_sum, sum_hat, t_anchor;
+  // (the next iterate's scale), and for mixed runs the sum of the anchor,
+  // the teleport term and the scale of the iteration that moved it
+  double inv_sum, sum_hat, t_anchor, c_anchor;
   int it, max_it, done, rule;
   int need_anchor, anchors, anchor_it, mixed;
 };
@@ -213,6 +212,9 @@ __global__ __launch_bounds__(kTileThreads) void pr_update_vec(double *__restrict
 
 // Decision logic of the last block of pr_normalize (one thread).
 __device__ inline void finalize_step(PrState *st, double err, double S) {
+  // (mixed runs) this iteration moved the anchor: its step is the FP64
+  // anchor product's, so its bound is free of the low-precision error
+  const bool moved = st->mixed && st->need_anchor;
   st->it += 1;
   st->err_prev = st->err;
   st->err = err;
@@ -224,14 +226,25 @@ __device__ inline void finalize_step(PrState *st, double err, double S) {
   bool converged = st->rule == 0 ? (err < st->eps) : (bound <= st->bound_target);
   int anchor = 0;
   if (st->mixed) {
-    const double floor = st->floor_eps * st->bound_anchor * 10.0;
-    if (converged && floor > st->bound_target) {
-      converged = false;
-      anchor = 1;
+    // A low-precision phase converges to a point off the FP64 fixed point by
+    // up to about floor_eps * (bound at its anchor) / (1 - rho)^2, rho the
+    // phase's mean contraction per iteration.  When that may exceed the
+    // target, convergence is only declared after an iteration that moved the
+    // anchor (an FP64 step) meets the bound.
+    if (converged && !moved) {
+      const int k = st->it - st->anchor_it > 0 ? st->it - st->anchor_it : 1;
+      double rho = 0.999;
+      if (st->bound_anchor > 0.0 && bound > 0.0) rho = fmin(0.999, pow(bound / st->bound_anchor, 1.0 / k));
+      const double floor = st->floor_eps * st->bound_anchor * 10.0 / ((1.0 - rho) * (1.0 - rho));
+      if (floor > st->bound_target) {
+        converged = false;
+        anchor = 1;
+      }
     }
     if (!converged) {
       if (bound <= st->anchor_tau * st->bound_anchor) anchor = 1;
-      if (st->it - st->anchor_it >= 3 && bound > 0.9 * st->bound_prev) anchor = 1;
+      // progress has stopped (the phase reached its low-precision floor)
+      if (st->it - st->anchor_it >= 3 && bound > 0.99 * st->bound_prev) anchor = 1;
     }
     if (anchor) {
       st->bound_anchor = bound;
@@ -485,9 +498,8 @@ struct PbScatterArgs {
 };
 
 // The anchor move of a mixed run, folded into the scatter of the iteration
-// that makes it: over the superblock's columns c, xhat[c] = (xhat[c] +
-// delta[c]) * inv_sum (the iterate, scaled to a sum of about 1), and the
-// product's input is z[c] = colscale[c] * xhat[c] (value-free anchor
+// that makes it: over the superblock's columns c, xhat[c] += delta[c], and
+// the product's input is z[c] = colscale[c] * xhat[c] (value-free anchor
 // product; z is written for the gather) or xhat[c] itself (z == nullptr).
 // Each superblock leaves the compensated sum of its new xhat in
 // (psum, pcomp)[superblock].
@@ -504,8 +516,7 @@ struct AnchorMove {
 // With MOVE the slice is the anchor move's result instead of x.
 template <typename VT, typename XT, typename AT, bool MOVE>
 __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restrict__ val, const XT *__restrict__ x,
-                                     AT *__restrict__ prod, XT *xs, const AnchorMove &am, double inv_sum,
-                                     double *red) {
+                                     AT *__restrict__ prod, XT *xs, const AnchorMove &am, double *red) {
 #pragma clang fp contract(off)
   // superblocks of one XCD (blockIdx % 8) are contiguous, so the runs that
   // share a cache line at a run boundary are written through the same L2
@@ -527,7 +538,7 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
       if (MOVE) {
         if (i < nc) {
           const long c = c0 + i;
-          const double v = (am.xhat[c] + static_cast<double>(am.delta[c])) * inv_sum;
+          const double v = am.xhat[c] + static_cast<double>(am.delta[c]);
           am.xhat[c] = v;
           csum_add(cs, v);
           double xv = v;
@@ -597,7 +608,7 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a,
                                                                 const PrState *st) {
   if (st->done) return;
   __shared__ XT xs[kPbCols];
-  pb_scatter_sb<VT, XT, AT, false>(a, val, x, prod, xs, AnchorMove{}, 1.0, nullptr);
+  pb_scatter_sb<VT, XT, AT, false>(a, val, x, prod, xs, AnchorMove{}, nullptr);
 }
 
 // Mixed runs, one launch per iteration: in an iteration that moves the
@@ -612,10 +623,9 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mx(PbScatterArgs
   __shared__ double xs[kPbCols];
   __shared__ double red[48];
   if (st->need_anchor)
-    pb_scatter_sb<VA, double, double, true>(a, vala, nullptr, static_cast<double *>(prod), xs, am, st->inv_sum, red);
+    pb_scatter_sb<VA, double, double, true>(a, vala, nullptr, static_cast<double *>(prod), xs, am, red);
   else
-    pb_scatter_sb<VD, XD, AD, false>(a, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am, 1.0,
-                                     nullptr);
+    pb_scatter_sb<VD, XD, AD, false>(a, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am, nullptr);
 }
 
 struct PbView {
@@ -788,14 +798,20 @@ __global__ __launch_bounds__(1024) void pb_gather_sn(PbView v, const VT *__restr
 }
 
 // Mixed runs (the partner of pb_scatter_mx).  The iterate is xhat + delta
-// (FP64 anchor, FP32 delta).  Between anchor moves it is not rescaled, so
-// with w = beta*A xhat + t_a*e - xhat (the anchor's one-step change, e the
-// teleport pattern, t_a the anchor iteration's teleport term) the next
-// delta is d' = w + beta*(A_lo d) + (t - t_a)*e.  The iteration that moves
-// the anchor (d = 0) sums the FP64 anchor product (values VA, input xa = z
-// or xhat) and stores w in FP32 (d' = w); the others sum the delta product
-// (values VD, input xd).  Writes d' (PAT32: also zd' = fl32(colscale32 *
-// d')), with sum|d' - d| and the compensated sum of d' as per-bin partials.
+// (FP64 anchor, FP32 delta) and is scaled by c = 1 / sum each iteration, as
+// the FP64 runs are: x' = c*(beta*A x + t*e), e the teleport pattern.  The
+// iteration that moves the anchor (xhat += delta, then delta = 0) sums the
+// FP64 anchor product (values VA, input xa = z or xhat) and stores the
+// anchor's scaled one-step change w = c_a*(beta*A xhat + t_a*e) - xhat in
+// FP32 (c_a, t_a: that iteration's scale and teleport term); the next delta
+// is d' = w.  The other iterations sum the delta product (values VD, input
+// xd): with r = c / c_a,
+//   d' = r*w + (r - 1)*xhat + c*(beta*(A_lo d) + (t - t_a)*e),
+// which is x' - xhat.  w and r - 1 stay of the order of the iterate's
+// change (the scale moves with the iterate's sum, which leaks at dangling
+// columns), so the FP32 delta carries no part of the iterate itself.
+// Writes d' (PAT32: also zd' = fl32(colscale32 * d')), with sum|d' - d| and
+// the compensated sum of d' as per-bin partials.
 template <typename VA, typename VD, typename XD, typename AD, int TM, bool PAT32>
 __global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restrict__ lvala,
                                                      const double *__restrict__ xa, const VD *__restrict__ lvald,
@@ -817,26 +833,28 @@ __global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restr
   const int bin = blockIdx.x, tid = threadIdx.x;
   const int base = bin << kPbShift;
   const int rows = min(kPbRows, v.n - base);
-  const double beta = st->beta;
-  // the anchor iteration's teleport term; the same expression gives the
-  // finalize pass the t_a it keeps for the phase
-  const double ta = anchor ? st->tele * st->inv_sum : st->t_anchor;
+  const double beta = st->beta, c = st->inv_sum;
+  // the anchor iteration's teleport term and scale (kept by the finalize
+  // pass of that iteration for the phase)
+  const double ta = anchor ? st->tele : st->t_anchor;
+  const double r = anchor ? 1.0 : c / st->c_anchor, r1 = r - 1.0;
   const double dt = st->tele - ta;
   double err = 0.0;
   CSum cs{0, 0};
   for (int i = tid; i < rows; i += 1024) {
-    const int r = base + i;
+    const int row = base + i;
     float d32;
     if (anchor) {
-      d32 = static_cast<float>(add_teleport<TM>(beta * ybin[i], ta, tp, r) - xhat[r]);
-      w[r] = d32;
+      d32 = static_cast<float>(c * add_teleport<TM>(beta * ybin[i], ta, tp, row) - xhat[row]);
+      w[row] = d32;
       err += fabs(static_cast<double>(d32));
     } else {
-      d32 = static_cast<float>(add_teleport<TM>(static_cast<double>(w[r]) + beta * ybin[i], dt, tp, r));
-      err += fabs(static_cast<double>(d32) - static_cast<double>(dold[r]));
+      d32 = static_cast<float>(r * static_cast<double>(w[row]) + r1 * xhat[row] +
+                               c * add_teleport<TM>(beta * ybin[i], dt, tp, row));
+      err += fabs(static_cast<double>(d32) - static_cast<double>(dold[row]));
     }
-    dnew[r] = d32;
-    if (PAT32) zdnew[r] = static_cast<float>(static_cast<double>(colscale32[r]) * static_cast<double>(d32));
+    dnew[row] = d32;
+    if (PAT32) zdnew[row] = static_cast<float>(static_cast<double>(colscale32[row]) * static_cast<double>(d32));
     csum_add(cs, static_cast<double>(d32));
   }
   double e;
@@ -1084,8 +1102,9 @@ __global__ __launch_bounds__(kTileThreads) void pr_finalize_sn(const double *__r
     if (st->mixed) {
       if (moved) {
         st->sum_hat = h.s + h.c;
-        st->sum = st->sum_hat;
-        st->t_anchor = st->tele * st->inv_sum;
+        st->sum = st->sum_hat;  // the iterate the step started from: the moved anchor
+        st->t_anchor = st->tele;
+        st->c_anchor = st->inv_sum;
       }
       S += st->sum_hat;
     }
@@ -1251,6 +1270,7 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
   h.inv_sum = 1.0 / h.sum;
   h.sum_hat = 0.0;
   h.t_anchor = 0.0;
+  h.c_anchor = 1.0;
   h.bound = 2.0;
   h.bound_prev = 2.0;
   h.bound_anchor = 2.0;

@@ -26,7 +26,8 @@ def params(tb, **kw):
 def run(fn, m, p, out):
     from tbsla_amd import _lib
     rep = _lib.PageRankReport()
-    rc = fn(m._h, ctypes.byref(p), ctypes.c_void_p(out.data_ptr()), out.numel(), None, ctypes.byref(rep))
+    dst = ctypes.cast(ctypes.c_void_p(out.data_ptr()), ctypes.POINTER(ctypes.c_double))
+    rc = fn(m._h, ctypes.byref(p), dst, out.numel(), None, ctypes.byref(rep))
     return rc, rep
 
 
