@@ -13,6 +13,7 @@ passed as raw files in a scratch directory; no GPU is touched here.
 
     python -m oracle.cpu_baseline spmv DIR       # DIR/{values,rowptr,colidx,x}.bin, DIR/meta.json
     python -m oracle.cpu_baseline pagerank DIR   # DIR/{values,rowptr,colidx}.bin, DIR/meta.json
+    python -m oracle.cpu_baseline pagerank-time DIR   # the timing only (no p*)
 
 Each prints one JSON line.
 """
@@ -32,9 +33,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 
 
-def omp_env(threads: int) -> dict:
+def omp_env(threads: int, bind: bool = True) -> dict:
+    """The reference's OpenMP setting (bind), or the same thread count with
+    the threads left to the OS scheduler (no places, no binding)."""
     env = dict(os.environ)
-    env.update({"OMP_NUM_THREADS": str(threads), "OMP_PLACES": "cores", "OMP_PROC_BIND": "close"})
+    env["OMP_NUM_THREADS"] = str(threads)
+    if bind:
+        env.update({"OMP_PLACES": "cores", "OMP_PROC_BIND": "close"})
+    else:
+        env.pop("OMP_PLACES", None)
+        env["OMP_PROC_BIND"] = "false"
     return env
 
 
@@ -90,7 +98,7 @@ def spmv_leg(d: str) -> dict:
             "y_sha256": hashlib.sha256(y.tobytes()).hexdigest()}
 
 
-def pagerank_leg(d: str) -> dict:
+def pagerank_leg(d: str, pstar: bool = True) -> dict:
     """The reference's Matrix::page_rank (Matrix.cpp:13-61) timed on a
     bounded number of iterations (its stopping test cannot fire with
     epsilon 0), and the FP64 fixed point p* from the oracle's restatement of
@@ -109,6 +117,8 @@ def pagerank_leg(d: str) -> dict:
         dt = time.perf_counter() - t
         del r
         out["reference"] = {"iterations": its, "seconds": dt, "seconds_per_iteration": dt / its}
+    if not pstar:
+        return out
     o = Oracle.from_arrays("CSR", n, n, v, rp, ci)
     del v, rp, ci
     t = time.perf_counter()
@@ -121,7 +131,7 @@ def pagerank_leg(d: str) -> dict:
 
 def main():
     task, d = sys.argv[1], sys.argv[2]
-    res = spmv_leg(d) if task == "spmv" else pagerank_leg(d)
+    res = spmv_leg(d) if task == "spmv" else pagerank_leg(d, pstar=task == "pagerank")
     print(json.dumps(res), flush=True)
 
 

@@ -500,11 +500,12 @@ struct PbScatterArgs {
 // The anchor move of a mixed run, folded into the scatter of the iteration
 // that makes it: over the superblock's columns c, xhat[c] += delta[c], and
 // the product's input is z[c] = colscale[c] * xhat[c] (value-free anchor
-// product; z is written for the gather) or xhat[c] itself (z == nullptr).
-// Each superblock leaves the compensated sum of its new xhat in
-// (psum, pcomp)[superblock].
+// product; z is written for the gather) or xhat[c] itself (z == nullptr),
+// and xhat32[c] = fl32(xhat[c]).  Each superblock leaves the compensated
+// sum of its new xhat in (psum, pcomp)[superblock].
 struct AnchorMove {
   double *xhat;
+  float *xhat32;  // fl32(xhat), for the scale term of the mixed iterations (or nullptr)
   const float *delta;
   double *z;
   const double *colscale;
@@ -540,6 +541,7 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
           const long c = c0 + i;
           const double v = am.xhat[c] + static_cast<double>(am.delta[c]);
           am.xhat[c] = v;
+          if (am.xhat32) am.xhat32[c] = static_cast<float>(v);
           csum_add(cs, v);
           double xv = v;
           if (am.z) {
@@ -816,7 +818,8 @@ template <typename VA, typename VD, typename XD, typename AD, int TM, bool PAT32
 __global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restrict__ lvala,
                                                      const double *__restrict__ xa, const VD *__restrict__ lvald,
                                                      const XD *__restrict__ xd, const void *prod, Teleport tp,
-                                                     const double *__restrict__ xhat, float *__restrict__ w,
+                                                     const double *__restrict__ xhat,
+                                                     const float *__restrict__ xhat32, float *__restrict__ w,
                                                      const float *__restrict__ dold, float *__restrict__ dnew,
                                                      float *__restrict__ zdnew, const float *__restrict__ colscale32,
                                                      const PrState *st, double *__restrict__ perr,
@@ -849,7 +852,10 @@ __global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restr
       w[row] = d32;
       err += fabs(static_cast<double>(d32));
     } else {
-      d32 = static_cast<float>(r * static_cast<double>(w[row]) + r1 * xhat[row] +
+      // (r - 1) is of the order of the change of the leak since the anchor,
+      // so the FP32 anchor copy carries this term to well below FP32 delta
+      // precision
+      d32 = static_cast<float>(r * static_cast<double>(w[row]) + r1 * static_cast<double>(xhat32[row]) +
                                c * add_teleport<TM>(beta * ybin[i], dt, tp, row));
       err += fabs(static_cast<double>(d32) - static_cast<double>(dold[row]));
     }
@@ -965,7 +971,7 @@ void launch_pb_mx_tm(const PbMatrix &pb, int va, int vd, const MxBufs &b, const 
   hipLaunchKernelGGL((pb_scatter_mx<VA, VD, XD, float>), pb_scatter_grid(pb), dim3(kPbScatterThreads), 0, s,
                      pb_args(pb), fa, fd, xd, pb.prod.get(), b.am, st);
   hipLaunchKernelGGL((pb_gather_mx<VA, VD, XD, float, TM, PAT32>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), la,
-                     xa, ld, xd, pb.prod.get(), tp, b.am.xhat, b.w, b.dold, b.dnew, b.zdnew, b.colscale32, st,
+                     xa, ld, xd, pb.prod.get(), tp, b.am.xhat, b.am.xhat32, b.w, b.dold, b.dnew, b.zdnew, b.colscale32, st,
                      sp.perr, sp.ps, sp.pc);
 }
 
@@ -1060,7 +1066,7 @@ struct Workspace {
 
 struct Scratch {
   WsBuf st, slots, perr, ps, pc, total, member, nodes, hsum, hcomp;
-  WsBuf X0, X1, Z, Z1, yhat, xhat, D0, D1, colscale, ZD0, ZD1, colscale32, W;
+  WsBuf X0, X1, Z, Z1, yhat, xhat, xhat32, D0, D1, colscale, ZD0, ZD1, colscale32, W;
 };
 
 // zd = fl32(colscale32 * d) of a constant FP32 delta d
@@ -1308,6 +1314,8 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
       // anchor is 0) and the scatter's per-superblock sums of a moved anchor
       ws.alloc(w.W, sizeof(float) * n);
       hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, w.W.as<float>(), n, 0.0f);
+      ws.alloc(w.xhat32, sizeof(float) * n);
+      hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, w.xhat32.as<float>(), n, 0.0f);
       ws.alloc(w.hsum, sizeof(double) * std::max(1, pbm->nsb));
       ws.alloc(w.hcomp, sizeof(double) * std::max(1, pbm->nsb));
     } else {
@@ -1373,8 +1381,8 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
         const SnPartials sp{w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>()};
         if (mixed) {
           MxBufs b;
-          b.am = AnchorMove{w.xhat.as<double>(), D[it & 1], mixed_pattern ? Z : nullptr, colscale,
-                            w.hsum.as<double>(), w.hcomp.as<double>()};
+          b.am = AnchorMove{w.xhat.as<double>(), w.xhat32.as<float>(), D[it & 1], mixed_pattern ? Z : nullptr,
+                            colscale, w.hsum.as<double>(), w.hcomp.as<double>()};
           b.xd = pat32 ? static_cast<const void *>(ZD[it & 1]) : static_cast<const void *>(D[it & 1]);
           b.w = w.W.as<float>();
           b.dold = D[it & 1];
