@@ -208,6 +208,7 @@ def leg_formats(torch, tb, dev, steps):
 
 
 PSTAR_ITERATIONS = 48   # the reference iteration is in its limit 2-cycle from ~40 iterations on
+REF_RULE_ITERATIONS = 26  # the drop-in (reference-rule) runs, timed per iteration
 CPU_PR_SAMPLE = 5       # reference page_rank iterations timed on the host
 
 
@@ -240,12 +241,28 @@ def leg_pagerank(torch, tb, dev, cpu=True):
         pr[prec]["speedup_vs_fp64"] = round(base / pr[prec]["seconds"], 3)
     # the value-free FP32 scheme against the value-free FP64 path as well
     pr["pattern_f32"]["speedup_vs_fp64_pattern"] = round(pr["pattern"]["seconds"] / pr["pattern_f32"]["seconds"], 3)
+    # the drop-in page_rank() path: the reference's stopping rule (which
+    # never fires at this size: 2-cycle at 2.5e-9), a fixed number of
+    # iterations, with propagation-blocked products (the default) and with
+    # fixed-order tiles (gather="tiles", the reference's summation order)
+    ref_rule = {}
+    for gather in ("auto", "tiles"):
+        o = torch.zeros(n, dtype=torch.float64, device=dev)
+        g.page_rank_device(o, epsilon=0.0, max_iteration=4, precision="f64", rule="reference", gather=gather)  # warm
+        rep = g.page_rank_device(o, epsilon=0.0, max_iteration=REF_RULE_ITERATIONS, precision="f64", rule="reference",
+                                 gather=gather)
+        ref_rule["propagation_blocked" if gather == "auto" else "tiles"] = {
+            "iterations": rep["iterations"], "seconds": round(rep["seconds"], 6),
+            "ms_per_iteration": round(rep["seconds"] / rep["iterations"] * 1e3, 3)}
+        log(f"pagerank reference rule ({gather}): {rep['seconds'] / rep['iterations'] * 1e3:.3f} ms per iteration")
+        del o
     # the GPU's own FP64 limit: the reference iteration on fixed-order tiles
     lim = torch.zeros(n, dtype=torch.float64, device=dev)
     g.page_rank_device(lim, epsilon=0.0, max_iteration=PSTAR_ITERATIONS, precision="f64", rule="reference",
                        gather="tiles")
     out = {"graph": "fill_cqmat_stochastic(2^24,2^24,8,0.5,7), beta 0.85",
-           "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr}
+           "stop": "sum-normalised L1 bound <= 1e-10*(1-beta)/beta", "variants": pr,
+           "reference_rule": ref_rule}
     if not cpu:  # (--no-cpu: no host legs; the GPU's own FP64 limit is the reference point)
         for prec, o in outs.items():
             pr[prec]["l1_vs_gpu_fp64_limit"] = float((o - lim).abs().sum().item())
@@ -322,7 +339,12 @@ def leg_batch(torch, tb, dev):
                      "bytes_per_iteration": rep2["bytes_per_iteration"],
                      "autotuned": rep["config"],
                      "first_run": {"seconds": round(rep["seconds"], 6), "tune_seconds": round(rep["tune_seconds"], 4),
-                                   "tune_iterations": rep["tune_iterations"]},
+                                   "tune_iterations": rep["tune_iterations"],
+                                   # what the trials cost beyond the iterations they ran: the first
+                                   # run against the later runs of the same batch in the chosen shape
+                                   "tune_overhead_seconds": round(rep["seconds"] - min(rep2["seconds"], rep3["seconds"]), 5),
+                                   "tune_overhead_fraction": round((rep["seconds"] - min(rep2["seconds"], rep3["seconds"]))
+                                                                   / rep["seconds"], 4)},
                      "second_batch": {"seconds": round(rep3["seconds"], 6), "tune_cached": rep3["tune_cached"],
                                       "tune_seconds": round(rep3["tune_seconds"], 5),
                                       "tune_iterations": rep3["tune_iterations"]},

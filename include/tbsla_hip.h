@@ -225,6 +225,7 @@ typedef struct {
   int tune_cached;             /* 1: the shape came from a batch of the same signature */
   double bytes_per_iteration;  /* algorithmic bytes of one iteration, all sets active */
   int wg_per_cu, unroll, xcd_map; /* launch shape in use */
+  int tile_nnz, spmm_rows;        /* entries per tile, rows per SpMM work item in use */
 } tbsla_batch_report;
 
 /* Sum-normalised vectors of every teleport set into out_dev (FP64, layout

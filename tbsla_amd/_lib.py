@@ -58,7 +58,7 @@ class BatchReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("tune_seconds", C.c_double),
                 ("tune_iterations", C.c_int), ("tune_cached", C.c_int),
                 ("bytes_per_iteration", C.c_double), ("wg_per_cu", C.c_int), ("unroll", C.c_int),
-                ("xcd_map", C.c_int)]
+                ("xcd_map", C.c_int), ("tile_nnz", C.c_int), ("spmm_rows", C.c_int)]
 
 
 # Every entry point the headers declare, with its ctypes signature.

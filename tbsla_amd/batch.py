@@ -132,5 +132,6 @@ class PageRankBatch:
         report = {"iterations": rep.iterations, "seconds": rep.seconds, "tune_seconds": rep.tune_seconds,
                   "tune_iterations": rep.tune_iterations, "tune_cached": bool(rep.tune_cached),
                   "bytes_per_iteration": rep.bytes_per_iteration,
-                  "config": {"wg_per_cu": rep.wg_per_cu, "unroll": rep.unroll, "xcd_map": rep.xcd_map}}
+                  "config": {"wg_per_cu": rep.wg_per_cu, "unroll": rep.unroll, "xcd_map": rep.xcd_map,
+                             "tile_nnz": rep.tile_nnz, "spmm_rows": rep.spmm_rows}}
         return out, report, its[: self.n_rhs()]

@@ -753,6 +753,8 @@ int tbsla_batch_run(tbsla_batch *b, const tbsla_batch_params *p, double *out_dev
       report->wg_per_cu = r.config.wg_per_cu;
       report->unroll = r.config.unroll;
       report->xcd_map = r.config.xcd_map;
+      report->tile_nnz = r.config.tile_nnz;
+      report->spmm_rows = r.config.spmm_rows;
     }
   });
 }

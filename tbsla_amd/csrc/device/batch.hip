@@ -24,17 +24,25 @@ namespace hip {
 namespace {
 
 constexpr int kLanes = 64;           // RHS per SpMM group (one per lane)
-constexpr int kSpmmRowsPerWave = 16;
-constexpr int kSpmmRows = 4 * kSpmmRowsPerWave;  // rows per SpMM work item
+// SpMM rows per wavefront (a work item is four wavefronts' rows): the
+// autotuner picks one of these
+constexpr int kSpmmRpw[3] = {8, 16, 32};
 constexpr int kSlots = 16;           // atomic-max slots per RHS
 constexpr int kChunkRows = 2048;     // rows per pass item, single-RHS graphs
-// entries per tile of the single-RHS graphs: smaller than the SpMV tiles, so
-// the persistent step (tile and SpMM items in one kernel) keeps its
-// occupancy
-constexpr int kBatchTileNnz = 2048;
+// entries per tile of the single-RHS graphs, picked by the autotuner (the
+// persistent step holds tile and SpMM items in one kernel, so the SpMV's
+// 3072 is not always best).  Rows longer than kBatchLongRow entries are long
+// rows at every tile size, summed in chunks of kBatchLongChunk, so every
+// tile size computes the same bits.
+constexpr int kBatchTiles[3] = {1024, 2048, 3072};
+constexpr int kBatchLongRow = 512;
+constexpr int kBatchLongChunk = 1024;
 constexpr int kSpmmChunkRows = 1024;  // rows per pass item, SpMM groups
 constexpr int kMaxGroups = 64;
 constexpr int kQueues = 8;           // one per XCD
+
+int tile_index(int tile_nnz) { return tile_nnz == 1024 ? 0 : (tile_nnz == 3072 ? 2 : 1); }
+int rpw_index(int spmm_rows) { return spmm_rows == 32 ? 0 : (spmm_rows == 128 ? 2 : 1); }
 
 struct RhsState {
   double tele_scale, tele, maxv, err, sum, sum_prev, bound, total;
@@ -80,6 +88,7 @@ struct BatchDev {
   const int *q_items;
   const int *q_off;           // kQueues + 1 queue boundaries in q_items
   int xcd_map;
+  int spmm_rpw;  // SpMM rows per wavefront
   double beta;
 };
 
@@ -203,10 +212,11 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
   const XT *X = xin + G.xoff;
   XT *Y = xout + G.xoff;
   if (__ballot(any) != 0) {
-    const int r0 = blk * kSpmmRows + wave * kSpmmRowsPerWave;
+    const int rpw = d.spmm_rpw;
+    const int r0 = (blk * 4 + wave) * rpw;
     // bounds of the wave's rows, loaded once: lane l <= 16 holds rowptr[r0 + l]
     // (clamped to rows, an index rowptr has)
-    const int rpl = lane <= kSpmmRowsPerWave ? rowptr[min(r0 + lane, rows)] - rp0 : 0;
+    const int rpl = lane <= rpw ? rowptr[min(r0 + lane, rows)] - rp0 : 0;
     // the next pass's row bounds, first 16 column indices and values, and
     // membership word, read while the current pass's gathers are in flight
     int a0n, a1n, cqn = 0;
@@ -228,14 +238,14 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
     };
     ahead(0);
 #pragma unroll 1
-    for (int pass = 0; pass < kSpmmRowsPerWave / 4; pass++) {
+    for (int pass = 0; pass < rpw / 4; pass++) {
       const int row = r0 + pass * 4 + sub;
       const bool rv = row < rows;
       const int a0 = a0n, a1 = a1n;
       int cq = cqn;
       AT vq = vqn;
       const unsigned long long mbrow = mbn;
-      if (pass + 1 < kSpmmRowsPerWave / 4) ahead(pass + 1);
+      if (pass + 1 < rpw / 4) ahead(pass + 1);
       AT acc[4] = {0, 0, 0, 0};
       for (int base = a0; __ballot(base < a1) != 0; base += 16) {
         const int cnt = min(16, a1 - base);  // <= 0 for a finished row
@@ -315,11 +325,11 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
 // (queue = XCD of the workgroup when xcd_map is set, else one shared list).
 // (sixteen gathers in flight fit two waves per SIMD in FP32 only with the
 // register budget held to 256)
-template <typename VT, typename XT, int UNR>
+template <typename VT, typename XT, int UNR, int TN>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(UNR == 16 ? 2 : 1)))
 void batch_step(BatchDev<VT, XT> d, int par) {
   using AT = typename dev::Val<VT>::A;
-  __shared__ dev::CsrTileLds<AT, kBatchTileNnz> lds;
+  __shared__ dev::CsrTileLds<AT, TN> lds;
   __shared__ double red[kTileThreads];
   const XT *xa_in = par ? d.xa1 : d.xa0, *xb_in = par ? d.xb1 : d.xb0;
   XT *xa_out = par ? d.xa0 : d.xa1, *xb_out = par ? d.xb0 : d.xb1;
@@ -352,9 +362,9 @@ void batch_step(BatchDev<VT, XT> d, int par) {
       TileEpi<XT> epi{xa_out, d.row_graph, d.rs, d.a_member, d.slots, d.beta, gA, gB, w % kSlots,
                       -INFINITY, -INFINITY};
       if (tile)
-        dev::csr_tile_rows_at<VT, XT, TileEpi<XT>, kBatchTileNnz>(d.a, xa_in, lds, epi, w);
+        dev::csr_tile_rows_at<VT, XT, TileEpi<XT>, TN, kBatchLongRow>(d.a, xa_in, lds, epi, w);
       else
-        dev::csr_long_row_at<VT, XT, TileEpi<XT>, kBatchTileNnz>(d.a, xa_in, lds, epi, w - d.a_ntiles);
+        dev::csr_long_row_at<VT, XT, TileEpi<XT>, TN, kBatchLongChunk>(d.a, xa_in, lds, epi, w - d.a_ntiles);
       const double bA = dev::block_tree_max<double>(epi.mA, red);
       const double bB = dev::block_tree_max<double>(epi.mB, red);
       if (threadIdx.x == 0) {
@@ -534,7 +544,8 @@ unsigned fill_grid(long n) { return static_cast<unsigned>(std::max<long>(1, std:
 
 std::string BatchConfig::str() const {
   std::ostringstream o;
-  o << "wg_per_cu=" << wg_per_cu << " unroll=" << unroll << " xcd_map=" << xcd_map;
+  o << "wg_per_cu=" << wg_per_cu << " unroll=" << unroll << " xcd_map=" << xcd_map << " tile_nnz=" << tile_nnz
+    << " spmm_rows=" << spmm_rows;
   return o.str();
 }
 
@@ -548,8 +559,10 @@ struct PageRankBatch::Impl {
   bool prepared = false;
   int cus = 256;
   // single-RHS block-diagonal matrix
-  DevBuf a_rowptr, a_colidx, a_vals, a_tiles, a_long, row_graph, row_base, a_member, xa[2];
-  int a_rows = 0, a_ntiles = 0, a_nlong = 0, n_a = 0;
+  DevBuf a_rowptr, a_colidx, a_vals, a_long, row_graph, row_base, a_member, xa[2];
+  DevBuf a_tiles[3];  // tile tables by tile size (kBatchTiles)
+  int a_ntiles[3] = {0, 0, 0};
+  int a_rows = 0, a_nlong = 0, n_a = 0;
   long a_nnz = 0;
   // SpMM graphs and groups
   struct SGraph {
@@ -559,7 +572,7 @@ struct PageRankBatch::Impl {
   };
   std::vector<std::unique_ptr<SGraph>> sgraphs;
   struct Group {
-    int sgraph, n_rhs, rhs_base, item_base, nitems;
+    int sgraph, n_rhs, rhs_base;
     long xoff;
     DevBuf member;
   };
@@ -575,11 +588,16 @@ struct PageRankBatch::Impl {
   DevBuf rs, slots, info, parts, items, out_off;
   int n_items = 0;
   long n_parts = 0;
-  DevBuf group_desc;  // SpmmGroup<VT> array
-  // work queues
-  DevBuf q_items[2], q_off_dev[2];
-  std::vector<int> q_off[2];
-  int n_work = 0;
+  // The work items of one step for one (tile size, SpMM rows per wavefront):
+  // the tiles, then the long rows, then each group's SpMM items; queued in
+  // item order (xcd_map 0) or pinned to XCD queues (xcd_map 1).
+  struct Work {
+    DevBuf group_desc;  // SpmmGroup<VT> array with this layout's item bases
+    DevBuf q_items[2], q_off_dev[2];
+    int n_work = 0;
+  };
+  Work work[3][3];
+  int n_work = 0;  // (of the default layout: zero means no work at all)
   BatchConfig cfg;
   bool tuned = false;
   double bytes_step = 0, bytes_pass = 0;
@@ -595,27 +613,29 @@ struct PageRankBatch::Impl {
     if (stream) (void)hipStreamDestroy(stream);
   }
   template <typename VT, typename XT>
-  BatchDev<VT, XT> dev_view(int xcd_map, double beta) const;
+  BatchDev<VT, XT> dev_view(const BatchConfig &c, double beta) const;
   template <typename VT, typename XT>
   void enqueue_iteration(const BatchDev<VT, XT> &d, int par, const BatchParams &p, hipStream_t s) const;
   template <typename VT, typename XT>
   void enqueue_rest(const BatchDev<VT, XT> &d, int par, const BatchParams &p, hipStream_t s) const;
   template <typename VT, typename XT>
-  void launch_step(const BatchDev<VT, XT> &d, int par, int unroll, int wg_per_cu, hipStream_t s) const;
+  void launch_step(const BatchDev<VT, XT> &d, int par, const BatchConfig &c, hipStream_t s) const;
   template <typename VT, typename XT>
   BatchReport run_typed(const BatchParams &p, double *out, hipStream_t s);
   void build();
   void init_state(const BatchParams &p, hipStream_t s);
+  void build_work(int ti, int ri, const std::vector<int> &rp_all, const std::vector<int> &rg);
   template <typename VT>
-  void upload_groups() {
+  void upload_groups(Work &w, const std::vector<int> &item_base, const std::vector<int> &nitems) {
     std::vector<SpmmGroup<VT>> gd;
-    for (const auto &gp : groups) {
-      const SGraph &sg = *sgraphs[static_cast<std::size_t>(gp->sgraph)];
+    for (std::size_t k = 0; k < groups.size(); k++) {
+      const Group &gp = *groups[k];
+      const SGraph &sg = *sgraphs[static_cast<std::size_t>(gp.sgraph)];
       gd.push_back(SpmmGroup<VT>{sg.rowptr.as<int>(), sg.colidx.as<int>(), sg.vals.as<VT>(),
-                                 gp->member.as<unsigned long long>(), sg.rows, sg.rp0, 0, gp->n_rhs, gp->rhs_base,
-                                 gp->item_base, gp->nitems, 0, gp->xoff});
+                                 gp.member.as<unsigned long long>(), sg.rows, sg.rp0, 0, gp.n_rhs, gp.rhs_base,
+                                 item_base[k], nitems[k], 0, gp.xoff});
     }
-    group_desc.upload(gd.data(), gd.size() * sizeof(SpmmGroup<VT>));
+    if (!gd.empty()) w.group_desc.upload(gd.data(), gd.size() * sizeof(SpmmGroup<VT>));
   }
 };
 
@@ -642,6 +662,100 @@ void upload_converted(DevBuf &dst, const double *src, std::size_t n, Dtype prec)
 }
 
 }  // namespace
+
+// The work items and queues of one layout.  Weights are the bytes an item
+// reads.  XCD-pinned queues: SpMM row ranges split evenly, single-RHS graphs
+// whole, balanced by weight.
+void PageRankBatch::Impl::build_work(int ti, int ri, const std::vector<int> &rp_all, const std::vector<int> &rg) {
+  Work &w = work[ti][ri];
+  const int tn = kBatchTiles[ti], rows_item = 4 * kSpmmRpw[ri];
+  const int ntiles = a_rows > 0 ? a_ntiles[ti] : 0;
+  const double vb = static_cast<double>(vsize());
+  std::vector<int> item_base(groups.size()), nitems(groups.size());
+  int nw = ntiles + a_nlong;
+  for (std::size_t k = 0; k < groups.size(); k++) {
+    const int rows = sgraphs[static_cast<std::size_t>(groups[k]->sgraph)]->rows;
+    item_base[k] = nw;
+    nitems[k] = (rows + rows_item - 1) / rows_item;
+    nw += nitems[k];
+  }
+  w.n_work = nw;
+  std::vector<double> wgt(static_cast<std::size_t>(nw), 0.0);
+  std::vector<int> item_graph(static_cast<std::size_t>(nw), -1);
+  if (a_rows > 0) {
+    std::vector<int> t(static_cast<std::size_t>(ntiles) + 1);
+    TBSLA_HIP_CHECK(hipMemcpy(t.data(), a_tiles[ti].get(), t.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < ntiles; k++) {
+      const long e0 = static_cast<long>(k) * tn;
+      wgt[static_cast<std::size_t>(k)] = static_cast<double>(std::min<long>(tn, a_nnz - e0)) * (vb + 4 + vb);
+      const int r = std::min(t[static_cast<std::size_t>(k)], a_rows - 1);
+      item_graph[static_cast<std::size_t>(k)] = rg[static_cast<std::size_t>(r)];
+    }
+    std::vector<int> lr(static_cast<std::size_t>(a_nlong));
+    if (a_nlong) TBSLA_HIP_CHECK(hipMemcpy(lr.data(), a_long.get(), lr.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < a_nlong; k++) {
+      const int r = lr[static_cast<std::size_t>(k)];
+      wgt[static_cast<std::size_t>(ntiles + k)] =
+          static_cast<double>(rp_all[static_cast<std::size_t>(r) + 1] - rp_all[static_cast<std::size_t>(r)]) * (vb + 4 + vb);
+      item_graph[static_cast<std::size_t>(ntiles + k)] = rg[static_cast<std::size_t>(r)];
+    }
+  }
+  for (std::size_t k = 0; k < groups.size(); k++) {
+    const SGraph &sg = *sgraphs[static_cast<std::size_t>(groups[k]->sgraph)];
+    const double per_row = static_cast<double>(sg.nnz) / std::max(1, sg.rows) * (vb + 4 + kLanes * vb) + 2.0 * kLanes * vb;
+    for (int b = 0; b < nitems[k]; b++) wgt[static_cast<std::size_t>(item_base[k] + b)] = per_row * rows_item;
+  }
+  std::vector<int> q_off[2];
+  // interleaved: one list in item order
+  {
+    std::vector<int> all(static_cast<std::size_t>(nw));
+    std::iota(all.begin(), all.end(), 0);
+    q_off[0].assign(kQueues + 1, nw);
+    q_off[0][0] = 0;
+    if (nw) w.q_items[0].upload(all.data(), all.size() * sizeof(int));
+  }
+  // XCD-pinned
+  {
+    std::vector<std::vector<int>> q(kQueues);
+    std::vector<double> load(kQueues, 0.0);
+    for (std::size_t k = 0; k < groups.size(); k++)
+      for (int x = 0; x < kQueues; x++) {
+        const int b0 = static_cast<int>(static_cast<long>(nitems[k]) * x / kQueues);
+        const int b1 = static_cast<int>(static_cast<long>(nitems[k]) * (x + 1) / kQueues);
+        for (int b = b0; b < b1; b++) {
+          q[static_cast<std::size_t>(x)].push_back(item_base[k] + b);
+          load[static_cast<std::size_t>(x)] += wgt[static_cast<std::size_t>(item_base[k] + b)];
+        }
+      }
+    std::vector<std::vector<int>> by_graph(static_cast<std::size_t>(std::max(1, n_a)));
+    std::vector<double> gw(static_cast<std::size_t>(std::max(1, n_a)), 0.0);
+    for (int k = 0; k < ntiles + a_nlong; k++) {
+      const int g = item_graph[static_cast<std::size_t>(k)];
+      by_graph[static_cast<std::size_t>(g)].push_back(k);
+      gw[static_cast<std::size_t>(g)] += wgt[static_cast<std::size_t>(k)];
+    }
+    std::vector<int> order(static_cast<std::size_t>(n_a));
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gw[static_cast<std::size_t>(x)] > gw[static_cast<std::size_t>(y)]; });
+    for (int g : order) {
+      const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      for (int k : by_graph[static_cast<std::size_t>(g)]) q[static_cast<std::size_t>(x)].push_back(k);
+      load[static_cast<std::size_t>(x)] += gw[static_cast<std::size_t>(g)];
+    }
+    std::vector<int> flat;
+    q_off[1].assign(1, 0);
+    for (int x = 0; x < kQueues; x++) {
+      flat.insert(flat.end(), q[static_cast<std::size_t>(x)].begin(), q[static_cast<std::size_t>(x)].end());
+      q_off[1].push_back(static_cast<int>(flat.size()));
+    }
+    if (!flat.empty()) w.q_items[1].upload(flat.data(), flat.size() * sizeof(int));
+  }
+  for (int k = 0; k < 2; k++) w.q_off_dev[k].upload(q_off[k].data(), q_off[k].size() * sizeof(int));
+  if (prec == Dtype::F64)
+    upload_groups<double>(w, item_base, nitems);
+  else
+    upload_groups<float>(w, item_base, nitems);
+}
 
 void PageRankBatch::Impl::build() {
   int dev = 0;
@@ -767,12 +881,14 @@ void PageRankBatch::Impl::build() {
       row_graph.upload(rg.data(), rg.size() * sizeof(int));
       row_base.upload(rb.data(), rb.size() * sizeof(int));
       a_member.upload(mem.data(), mem.size());
-      std::vector<int> t = csr_entry_tiles(rp_all.data(), a_rows, kBatchTileNnz);
-      a_ntiles = static_cast<int>(t.size()) - 1;
-      a_tiles.upload(t.data(), t.size() * sizeof(int));
+      for (int ti = 0; ti < 3; ti++) {
+        std::vector<int> t = csr_entry_tiles(rp_all.data(), a_rows, kBatchTiles[ti]);
+        a_ntiles[ti] = static_cast<int>(t.size()) - 1;
+        a_tiles[ti].upload(t.data(), t.size() * sizeof(int));
+      }
       std::vector<int> lr;
       for (int i = 0; i < a_rows; i++)
-        if (rp_all[static_cast<std::size_t>(i) + 1] - rp_all[static_cast<std::size_t>(i)] > kBatchTileNnz / 2) lr.push_back(i);
+        if (rp_all[static_cast<std::size_t>(i) + 1] - rp_all[static_cast<std::size_t>(i)] > kBatchLongRow) lr.push_back(i);
       a_nlong = static_cast<int>(lr.size());
       if (a_nlong) a_long.upload(lr.data(), lr.size() * sizeof(int));
       for (int k = 0; k < 2; k++) xa[k].alloc(vb * static_cast<std::size_t>(a_rows));
@@ -780,7 +896,6 @@ void PageRankBatch::Impl::build() {
   }
 
   // ---- multi-RHS graphs: one CSR each, groups of up to 64 RHS
-  int item_base = a_ntiles + a_nlong;
   xb_len = 0;
   for (int g : multi) {
     auto sg = std::unique_ptr<SGraph>(new SGraph());
@@ -806,9 +921,6 @@ void PageRankBatch::Impl::build() {
       gr->sgraph = static_cast<int>(sgraphs.size());
       gr->n_rhs = static_cast<int>(std::min<std::size_t>(kLanes, rhs.size() - k0));
       gr->rhs_base = user_rhs_dev[static_cast<std::size_t>(g)][k0];
-      gr->item_base = item_base;
-      gr->nitems = (sg->rows + kSpmmRows - 1) / kSpmmRows;
-      item_base += gr->nitems;
       gr->xoff = xb_len;
       xb_len += static_cast<long>(sg->rows) * kLanes;
       std::vector<unsigned long long> mb(static_cast<std::size_t>(sg->rows), 0ull);
@@ -820,7 +932,6 @@ void PageRankBatch::Impl::build() {
     }
     sgraphs.push_back(std::move(sg));
   }
-  n_work = item_base;
   if (xb_len > 0)
     for (int k = 0; k < 2; k++) xb[k].alloc(vb * static_cast<std::size_t>(xb_len));
 
@@ -863,88 +974,18 @@ void PageRankBatch::Impl::build() {
   rs.alloc(sizeof(RhsState) * static_cast<std::size_t>(std::max(1, n_rhs_total)));
   slots.alloc(sizeof(unsigned long long) * kSlots * static_cast<std::size_t>(std::max(1, n_rhs_total)));
 
-  // ---- work queues.  Weights are the bytes an item reads.
-  std::vector<double> wgt(static_cast<std::size_t>(n_work), 0.0);
-  std::vector<int> item_graph(static_cast<std::size_t>(n_work), -1);
+  // ---- work items and queues of every (tile size, SpMM rows) layout
   {
-    std::vector<int> rp_all;
+    std::vector<int> rp_all, rg;
     if (a_rows > 0) {
       rp_all.resize(static_cast<std::size_t>(a_rows) + 1);
+      rg.resize(static_cast<std::size_t>(a_rows));
       TBSLA_HIP_CHECK(hipMemcpy(rp_all.data(), a_rowptr.get(), rp_all.size() * sizeof(int), hipMemcpyDeviceToHost));
-      std::vector<int> t(static_cast<std::size_t>(a_ntiles) + 1), rg(static_cast<std::size_t>(a_rows));
-      TBSLA_HIP_CHECK(hipMemcpy(t.data(), a_tiles.get(), t.size() * sizeof(int), hipMemcpyDeviceToHost));
       TBSLA_HIP_CHECK(hipMemcpy(rg.data(), row_graph.get(), rg.size() * sizeof(int), hipMemcpyDeviceToHost));
-      for (int k = 0; k < a_ntiles; k++) {
-        const long e0 = static_cast<long>(k) * kBatchTileNnz;
-        wgt[static_cast<std::size_t>(k)] = static_cast<double>(std::min<long>(kBatchTileNnz, a_nnz - e0)) * (vb + 4 + vb);
-        const int r = std::min(t[static_cast<std::size_t>(k)], a_rows - 1);
-        item_graph[static_cast<std::size_t>(k)] = rg[static_cast<std::size_t>(r)];
-      }
-      std::vector<int> lr(static_cast<std::size_t>(a_nlong));
-      if (a_nlong) TBSLA_HIP_CHECK(hipMemcpy(lr.data(), a_long.get(), lr.size() * sizeof(int), hipMemcpyDeviceToHost));
-      for (int k = 0; k < a_nlong; k++) {
-        const int r = lr[static_cast<std::size_t>(k)];
-        wgt[static_cast<std::size_t>(a_ntiles + k)] =
-            static_cast<double>(rp_all[static_cast<std::size_t>(r) + 1] - rp_all[static_cast<std::size_t>(r)]) * (vb + 4 + vb);
-        item_graph[static_cast<std::size_t>(a_ntiles + k)] = rg[static_cast<std::size_t>(r)];
-      }
     }
-    for (const auto &gp : groups) {
-      const SGraph &sg = *sgraphs[static_cast<std::size_t>(gp->sgraph)];
-      const double per_row = static_cast<double>(sg.nnz) / std::max(1, sg.rows) * (vb + 4 + kLanes * vb) + 2.0 * kLanes * vb;
-      for (int b = 0; b < gp->nitems; b++) wgt[static_cast<std::size_t>(gp->item_base + b)] = per_row * kSpmmRows;
-    }
-  }
-  // interleaved: one list in item order
-  {
-    std::vector<int> all(static_cast<std::size_t>(n_work));
-    std::iota(all.begin(), all.end(), 0);
-    q_off[0].assign(kQueues + 1, n_work);
-    q_off[0][0] = 0;
-    if (n_work) q_items[0].upload(all.data(), all.size() * sizeof(int));
-  }
-  // XCD-pinned: SpMM row ranges split evenly, single-RHS graphs whole, by weight
-  {
-    std::vector<std::vector<int>> q(kQueues);
-    std::vector<double> load(kQueues, 0.0);
-    for (const auto &gp : groups)
-      for (int x = 0; x < kQueues; x++) {
-        const int b0 = static_cast<int>(static_cast<long>(gp->nitems) * x / kQueues);
-        const int b1 = static_cast<int>(static_cast<long>(gp->nitems) * (x + 1) / kQueues);
-        for (int b = b0; b < b1; b++) {
-          q[static_cast<std::size_t>(x)].push_back(gp->item_base + b);
-          load[static_cast<std::size_t>(x)] += wgt[static_cast<std::size_t>(gp->item_base + b)];
-        }
-      }
-    std::vector<std::vector<int>> by_graph(static_cast<std::size_t>(std::max(1, n_a)));
-    std::vector<double> gw(static_cast<std::size_t>(std::max(1, n_a)), 0.0);
-    for (int k = 0; k < a_ntiles + a_nlong; k++) {
-      const int g = item_graph[static_cast<std::size_t>(k)];
-      by_graph[static_cast<std::size_t>(g)].push_back(k);
-      gw[static_cast<std::size_t>(g)] += wgt[static_cast<std::size_t>(k)];
-    }
-    std::vector<int> order(static_cast<std::size_t>(n_a));
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gw[static_cast<std::size_t>(x)] > gw[static_cast<std::size_t>(y)]; });
-    for (int g : order) {
-      const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-      for (int k : by_graph[static_cast<std::size_t>(g)]) q[static_cast<std::size_t>(x)].push_back(k);
-      load[static_cast<std::size_t>(x)] += gw[static_cast<std::size_t>(g)];
-    }
-    std::vector<int> flat;
-    q_off[1].assign(1, 0);
-    for (int x = 0; x < kQueues; x++) {
-      flat.insert(flat.end(), q[static_cast<std::size_t>(x)].begin(), q[static_cast<std::size_t>(x)].end());
-      q_off[1].push_back(static_cast<int>(flat.size()));
-    }
-    if (!flat.empty()) q_items[1].upload(flat.data(), flat.size() * sizeof(int));
-  }
-  for (int k = 0; k < 2; k++) q_off_dev[k].upload(q_off[k].data(), q_off[k].size() * sizeof(int));
-  if (!groups.empty()) {
-    if (prec == Dtype::F64)
-      upload_groups<double>();
-    else
-      upload_groups<float>();
+    for (int ti = 0; ti < 3; ti++)
+      for (int ri = 0; ri < 3; ri++) build_work(ti, ri, rp_all, rg);
+    n_work = work[1][1].n_work;
   }
   // algorithmic bytes of one iteration with every RHS active
   bytes_step = static_cast<double>(a_nnz) * (vb + 4) + static_cast<double>(a_rows) * (4 + 1 + 2 * vb);
@@ -959,12 +1000,15 @@ void PageRankBatch::Impl::build() {
 }
 
 template <typename VT, typename XT>
-BatchDev<VT, XT> PageRankBatch::Impl::dev_view(int xcd_map, double beta) const {
+BatchDev<VT, XT> PageRankBatch::Impl::dev_view(const BatchConfig &c, double beta) const {
+  const int ti = tile_index(c.tile_nnz), ri = rpw_index(c.spmm_rows);
+  const Work &w = work[ti][ri];
+  const int xcd_map = c.xcd_map ? 1 : 0;
   BatchDev<VT, XT> d;
   std::memset(&d, 0, sizeof(d));
-  d.a = dev::CsrView<VT>{a_rowptr.as<int>(), a_colidx.as<int>(), a_vals.as<VT>(), a_tiles.as<int>(),
+  d.a = dev::CsrView<VT>{a_rowptr.as<int>(), a_colidx.as<int>(), a_vals.as<VT>(), a_tiles[ti].as<int>(),
                          a_long.as<int>(), a_nnz, 0, 0};
-  d.a_ntiles = a_ntiles;
+  d.a_ntiles = a_rows > 0 ? a_ntiles[ti] : 0;
   d.a_nlong = a_nlong;
   d.a_rows = a_rows;
   d.n_groups = static_cast<int>(groups.size());
@@ -975,33 +1019,47 @@ BatchDev<VT, XT> PageRankBatch::Impl::dev_view(int xcd_map, double beta) const {
   d.xa1 = xa[1].as<XT>();
   d.xb0 = xb[0].as<XT>();
   d.xb1 = xb[1].as<XT>();
-  d.g = group_desc.as<SpmmGroup<VT>>();
+  d.g = w.group_desc.as<SpmmGroup<VT>>();
   d.rs = rs.as<RhsState>();
   d.slots = slots.as<unsigned long long>();
-  d.q_items = q_items[xcd_map].as<int>();
-  d.q_off = q_off_dev[xcd_map].as<int>();
+  d.q_items = w.q_items[xcd_map].as<int>();
+  d.q_off = w.q_off_dev[xcd_map].as<int>();
   d.xcd_map = xcd_map;
+  d.spmm_rpw = kSpmmRpw[ri];
   d.beta = beta;
   return d;
 }
 
+namespace {
+template <typename VT, typename XT, int UNR>
+void launch_step_tn(const BatchDev<VT, XT> &d, int par, int tn, unsigned grid, hipStream_t s) {
+  if (tn == 1024)
+    hipLaunchKernelGGL((batch_step<VT, XT, UNR, 1024>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+  else if (tn == 3072)
+    hipLaunchKernelGGL((batch_step<VT, XT, UNR, 3072>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+  else
+    hipLaunchKernelGGL((batch_step<VT, XT, UNR, 2048>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+}
+}  // namespace
+
 template <typename VT, typename XT>
-void PageRankBatch::Impl::launch_step(const BatchDev<VT, XT> &d, int par, int unroll, int wg_per_cu,
+void PageRankBatch::Impl::launch_step(const BatchDev<VT, XT> &d, int par, const BatchConfig &c,
                                       hipStream_t s) const {
   if (n_work == 0) return;
-  const unsigned grid = static_cast<unsigned>(std::max(kQueues, (cus * wg_per_cu) / kQueues * kQueues));
-  if (unroll == 4)
-    hipLaunchKernelGGL((batch_step<VT, XT, 4>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
-  else if (unroll == 8)
-    hipLaunchKernelGGL((batch_step<VT, XT, 8>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+  const unsigned grid = static_cast<unsigned>(std::max(kQueues, (cus * c.wg_per_cu) / kQueues * kQueues));
+  const int tn = kBatchTiles[tile_index(c.tile_nnz)];
+  if (c.unroll == 4)
+    launch_step_tn<VT, XT, 4>(d, par, tn, grid, s);
+  else if (c.unroll == 8)
+    launch_step_tn<VT, XT, 8>(d, par, tn, grid, s);
   else
-    hipLaunchKernelGGL((batch_step<VT, XT, 16>), dim3(grid), dim3(kTileThreads), 0, s, d, par);
+    launch_step_tn<VT, XT, 16>(d, par, tn, grid, s);
 }
 
 template <typename VT, typename XT>
 void PageRankBatch::Impl::enqueue_iteration(const BatchDev<VT, XT> &d, int par, const BatchParams &p,
                                             hipStream_t s) const {
-  launch_step<VT, XT>(d, par, cfg.unroll, cfg.wg_per_cu, s);
+  launch_step<VT, XT>(d, par, cfg, s);
   enqueue_rest<VT, XT>(d, par, p, s);
 }
 
@@ -1074,15 +1132,18 @@ BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hi
       BatchConfig best = base;
       float best_ms = 1e30f;
       std::vector<std::pair<BatchConfig, float>> tried;
+      auto same = [](const BatchConfig &a, const BatchConfig &b) {
+        return a.unroll == b.unroll && a.wg_per_cu == b.wg_per_cu && a.xcd_map == b.xcd_map &&
+               a.tile_nnz == b.tile_nnz && a.spmm_rows == b.spmm_rows;
+      };
       auto trial = [&](const BatchConfig &c, bool again) {
         if (!again)
           for (const auto &t : tried)
-            if (t.first.unroll == c.unroll && t.first.wg_per_cu == c.wg_per_cu && t.first.xcd_map == c.xcd_map)
-              return;
-        const BatchDev<VT, XT> d = dev_view<VT, XT>(c.xcd_map, p.beta);
+            if (same(t.first, c)) return;
+        const BatchDev<VT, XT> d = dev_view<VT, XT>(c, p.beta);
         const int par = done_its & 1;
         TBSLA_HIP_CHECK(hipEventRecord(e0, s));
-        launch_step<VT, XT>(d, par, c.unroll, c.wg_per_cu, s);
+        launch_step<VT, XT>(d, par, c, s);
         TBSLA_HIP_CHECK(hipEventRecord(e1, s));
         enqueue_rest<VT, XT>(d, par, p, s);
         TBSLA_HIP_CHECK(hipEventSynchronize(e1));
@@ -1095,10 +1156,17 @@ BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hi
           best = c;
         }
       };
-      for (int un : {8, 4, 16}) trial(BatchConfig{base.wg_per_cu, un, base.xcd_map}, false);
-      const int un = best.unroll;
-      for (int wg : {2, 4, 8}) trial(BatchConfig{wg, un, base.xcd_map}, false);
-      trial(BatchConfig{best.wg_per_cu, un, 1 - base.xcd_map}, false);
+      auto with = [](BatchConfig c, int BatchConfig::*f, int v) {
+        c.*f = v;
+        return c;
+      };
+      for (int un : {8, 4, 16}) trial(with(base, &BatchConfig::unroll, un), false);
+      for (int wg : {2, 4, 8}) trial(with(best, &BatchConfig::wg_per_cu, wg), false);
+      trial(with(best, &BatchConfig::xcd_map, 1 - base.xcd_map), false);
+      if (a_rows > 0)
+        for (int tn : {2048, 1024, 3072}) trial(with(best, &BatchConfig::tile_nnz, tn), false);
+      if (!groups.empty())
+        for (int sr : {64, 32, 128}) trial(with(best, &BatchConfig::spmm_rows, sr), false);
       if (done_its & 1) trial(best, true);  // one more iteration, with the chosen shape
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -1110,7 +1178,7 @@ BatchReport PageRankBatch::Impl::run_typed(const BatchParams &p, double *out, hi
     rep.tune_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     rep.tune_iterations = done_its;
   }
-  const BatchDev<VT, XT> d = dev_view<VT, XT>(cfg.xcd_map, p.beta);
+  const BatchDev<VT, XT> d = dev_view<VT, XT>(cfg, p.beta);
   const int C = std::max(2, (p.check_every + 1) / 2 * 2);
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;

@@ -56,6 +56,8 @@ struct BatchConfig {
   int wg_per_cu = 4;  // persistent workgroups per compute unit
   int unroll = 8;     // gathers in flight per lane in the SpMM (4 or 8)
   int xcd_map = 1;    // 1: graphs / row ranges pinned to XCD queues; 0: interleaved
+  int tile_nnz = 2048;  // entries per tile of the single-set graphs (1024, 2048, 3072)
+  int spmm_rows = 64;   // rows per SpMM work item (32, 64, 128)
   std::string str() const;
 };
 

@@ -348,16 +348,26 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
     pb->nbins = (n + kPbRows - 1) / kPbRows;
     pb->nsb = (n + kPbCols - 1) / kPbCols;
     const int nb = pb->nbins, ns = pb->nsb;
-    // pass 1: counts per bin (far, local) and per column (far)
+    // pass 1: counts per bin (far; local groups) and per column (far).  The
+    // local groups of a bin are formed in row order: an entry opens a new
+    // group when the current one is full or its row is kPbLocalPad or more
+    // rows past the group's base row
     std::vector<int> colcnt(static_cast<std::size_t>(n) + 1, 0), bincnt(static_cast<std::size_t>(nb) + 1, 0),
         loccnt(static_cast<std::size_t>(nb) + 1, 0);
+    int gfill = kPbLocalGroup, gbase = 0;  // the current bin's open group
     for (int r = 0; r < n; r++) {
       const int b = r >> kPbShift;
+      if ((r & (kPbRows - 1)) == 0) gfill = kPbLocalGroup;  // a new bin
       for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
         const int col = ci[static_cast<std::size_t>(r0 + e)] - c.get_f_col();
         if (col < 0 || col >= n) bad("CSR column index outside the matrix");
         if ((col >> kPbShift) == b) {
-          loccnt[static_cast<std::size_t>(b) + 1]++;
+          if (gfill == kPbLocalGroup || r - gbase >= kPbLocalPad) {
+            loccnt[static_cast<std::size_t>(b) + 1] += kPbLocalGroup;
+            gfill = 0;
+            gbase = r;
+          }
+          gfill++;
         } else {
           colcnt[static_cast<std::size_t>(col) + 1]++;
           bincnt[static_cast<std::size_t>(b) + 1]++;
@@ -365,33 +375,44 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
       }
     }
     for (int j = 0; j < n; j++) colcnt[static_cast<std::size_t>(j) + 1] += colcnt[static_cast<std::size_t>(j)];
-    // each bin's local run is padded to a multiple of kPbLocalGroup entries
-    // (row kPbPadRow, column 0, value 0), so the gather reads groups with
-    // aligned vector loads
     for (int b = 0; b < nb; b++) {
       bincnt[static_cast<std::size_t>(b) + 1] += bincnt[static_cast<std::size_t>(b)];
-      const int lc = loccnt[static_cast<std::size_t>(b) + 1];
-      loccnt[static_cast<std::size_t>(b) + 1] =
-          loccnt[static_cast<std::size_t>(b)] + (lc + kPbLocalGroup - 1) / kPbLocalGroup * kPbLocalGroup;
+      loccnt[static_cast<std::size_t>(b) + 1] += loccnt[static_cast<std::size_t>(b)];
     }
     pb->n_far = colcnt[static_cast<std::size_t>(n)];
     pb->n_loc = loccnt[static_cast<std::size_t>(nb)];
     if (pb->n_far >= (1L << 31) || pb->n_loc >= (1L << 31)) bad("propagation-blocked layout limited to 2^31 entries");
     const std::size_t nf = static_cast<std::size_t>(pb->n_far), nl = static_cast<std::size_t>(pb->n_loc);
-    // pass 2: far entries in column order (rows ascending), local entries bin-grouped
+    // pass 2: far entries in column order (rows ascending), local entries
+    // bin-grouped (the groups of pass 1; unused slots stay padding)
     std::vector<int> csc_src(nf), csc_row(nf);
     pb->loc_src.assign(nl, -1);
-    std::vector<unsigned short> lrow(nl, static_cast<unsigned short>(kPbPadRow)), lcol(nl, 0);
-    std::vector<int> ccur(colcnt.begin(), colcnt.end() - 1), lcur(loccnt.begin(), loccnt.end() - 1);
+    std::vector<unsigned short> lkey(nl, static_cast<unsigned short>(kPbLocalPad << kPbShift));
+    std::vector<unsigned short> lbase(nl / kPbLocalGroup, 0);
+    std::vector<int> ccur(colcnt.begin(), colcnt.end() - 1);
+    long lnext = 0;  // first slot of the next local group
+    int lslot = 0;   // next slot in the open group (kPbLocalGroup: none open)
+    gfill = kPbLocalGroup;
     for (int r = 0; r < n; r++) {
       const int b = r >> kPbShift;
+      if ((r & (kPbRows - 1)) == 0) {
+        gfill = kPbLocalGroup;
+        lnext = loccnt[static_cast<std::size_t>(b)];
+      }
       for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
         const int col = ci[static_cast<std::size_t>(r0 + e)] - c.get_f_col();
         if ((col >> kPbShift) == b) {
-          const int k = lcur[static_cast<std::size_t>(b)]++;
-          pb->loc_src[static_cast<std::size_t>(k)] = static_cast<int>(e);
-          lrow[static_cast<std::size_t>(k)] = static_cast<unsigned short>(r & (kPbRows - 1));
-          lcol[static_cast<std::size_t>(k)] = static_cast<unsigned short>(col & (kPbRows - 1));
+          if (gfill == kPbLocalGroup || r - gbase >= kPbLocalPad) {
+            lslot = static_cast<int>(lnext);
+            lnext += kPbLocalGroup;
+            gfill = 0;
+            gbase = r;
+            lbase[static_cast<std::size_t>(lslot / kPbLocalGroup)] = static_cast<unsigned short>(r & (kPbRows - 1));
+          }
+          const std::size_t k = static_cast<std::size_t>(lslot + gfill);
+          gfill++;
+          pb->loc_src[k] = static_cast<int>(e);
+          lkey[k] = static_cast<unsigned short>((col & (kPbRows - 1)) | ((r - gbase) << kPbShift));
         } else {
           const int k = ccur[static_cast<std::size_t>(col)]++;
           csc_src[static_cast<std::size_t>(k)] = static_cast<int>(e);
@@ -483,8 +504,8 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
     pb->frow.upload(frow.data(), frow.size() * sizeof(unsigned short));
     pb->bp.upload(bincnt.data(), bincnt.size() * sizeof(int));
     pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));
-    pb->lrow.upload(lrow.data(), lrow.size() * sizeof(unsigned short));
-    pb->lcol.upload(lcol.data(), lcol.size() * sizeof(unsigned short));
+    pb->lkey.upload(lkey.data(), lkey.size() * sizeof(unsigned short));
+    pb->lbase.upload(lbase.data(), lbase.size() * sizeof(unsigned short));
     pb->prod.alloc(sizeof(double) * std::max<std::size_t>(1, nf));
     cache.pb = std::move(pb);
   }

@@ -78,20 +78,26 @@ struct DeviceMatrix {
 //          in bin-grouped order -- by (row bin, column) -- with the row offset
 //          in the bin (bp: bin boundaries).  Within a bin the products of one
 //          superblock are contiguous, so the scatter pass writes runs.
-//   local  bin-grouped (lp, lrow, lcol = offsets in the bin, values), summed
-//          by the gather pass with x read from the bin's own slice; each bin's
-//          run is padded to a multiple of kPbLocalGroup entries with row
-//          kPbPadRow (no such row), column 0 and value 0
+//   local  bin-grouped in row order (lp: bin boundaries), summed by the
+//          gather pass with x read from the bin's own slice, in groups of
+//          kPbLocalGroup entries: each group has a base row (lbase, offset in
+//          the bin) and each entry a 16-bit key lkey = column offset in the
+//          bin | (row - base) << kPbShift, with row - base < kPbLocalPad; a
+//          key whose top field is kPbLocalPad is padding (column 0, value 0).
+//          A group ends early, padded, where the next entry's row is
+//          kPbLocalPad or more rows past its base, and each bin's run is
+//          padded to a whole number of groups
 constexpr int kPbShift = 13;
 constexpr int kPbRows = 1 << kPbShift;
 constexpr int kPbColShift = 13;
 constexpr int kPbLocalGroup = 4;
-constexpr int kPbPadRow = 0xFFFF;
+constexpr int kPbLocalPad = 7;  // the row field of a padding key (3 bits above the column)
 constexpr int kPbCols = 1 << kPbColShift;
 // scatter keys: the column within the superblock, and this flag on the first
 // entry of each run; the run of an entry is found per wavefront-sized chunk
 constexpr unsigned kPbRunStart = 0x8000u;
 constexpr int kPbChunk = 64;
+static_assert(kPbShift + 3 <= 16, "local keys hold the column and a 3-bit row offset in 16 bits");
 static_assert(kPbCols <= 0x2000, "scatter keys hold the column in 13 bits");
 
 struct PbMatrix {
@@ -104,7 +110,7 @@ struct PbMatrix {
   // superblock s (entries sp[s] + 64j ...) begins in run cbase[cp[s] + j],
   // so the run of an entry is cbase plus the run starts after the chunk's
   // first entry up to the entry itself.
-  DevBuf sp, fkey, rp, rdelta, cbase, cp, frow, bp, lp, lrow, lcol;
+  DevBuf sp, fkey, rp, rdelta, cbase, cp, frow, bp, lp, lkey, lbase;
   DevBuf fval[3], lval[3];  // by Dtype F64, F32, BF16 (built on demand)
   DevBuf prod;              // 8-byte product slots, n_far
   std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry (-1: padding)

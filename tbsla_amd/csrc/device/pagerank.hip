@@ -569,37 +569,65 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
     }
   }
   __syncthreads();
-  // four entries per thread in flight: every global load first, then the
-  // run offsets and the LDS reads of x, then the products and the stores.
-  // The lanes of a wavefront hold consecutive entries of one 64-entry chunk
-  // (kPbScatterThreads is a multiple of 64), so an entry's run is the
-  // chunk's first run plus the run starts on lanes 1..lane, counted from a
-  // ballot; lanes past the end are the chunk's last and hold no run start.
-  static_assert(kPbScatterThreads % kPbChunk == 0 && kPbChunk == 64, "one chunk per wavefront");
-  const int lane = threadIdx.x & 63;
+  // Each wavefront walks chunks wv + 16u + 64it (u = 0..3) of the superblock,
+  // one entry per lane, as a software pipeline: the run of each chunk's
+  // first entry (cbase) is loaded two iterations ahead; the keys, the values
+  // and the slot offsets of the chunk's runs (rdelta[cb .. cb + 63], one
+  // coalesced load, each lane's offset picked with a lane shuffle) one
+  // iteration ahead, so no load of an iteration waits on another.  An
+  // entry's run is the chunk's first run plus the run starts on lanes
+  // 1..lane (a ballot); lanes past the end hold no run start.
+  static_assert(kPbScatterThreads == 16 * kPbChunk && kPbChunk == 64, "16 wavefronts of 64-entry chunks");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nent = k1 - k0, nck = (nent + kPbChunk - 1) / kPbChunk;
+  const int nr = a.rp[sb + 1] - a.rp[sb];  // runs of this superblock
+  const int nit = (nck + 63) >> 6;
   const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
-  for (int k = k0 + threadIdx.x; k < k1; k += 4 * kPbScatterThreads) {
+  struct Stage {
     unsigned key[4];
-    int sl[4], cb[4];
-    AT v1[4];
+    AT v[4];
+    int dl[4];
+  };
+  auto load_cb = [&](int it, int (&cb)[4]) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int kk = k + u * kPbScatterThreads;
-      const bool ok = kk < k1;
-      key[u] = ok ? a.fkey[kk] : 0u;
-      cb[u] = ok ? cbase[(kk - k0) >> 6] : 0;
-      v1[u] = ok ? PbVal<VT>::template prod<AT>(val, kk, AT(1)) : AT(0);
+      const int j = wv + 16 * u + 64 * it;
+      cb[u] = j < nck ? cbase[j] : 0;
     }
+  };
+  auto load_stage = [&](int it, const int (&cb)[4], Stage &g) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int kk = k + u * kPbScatterThreads;
-      const unsigned long long starts = __ballot((key[u] & kPbRunStart) != 0u);
-      const int run = cb[u] + __popcll(starts & upto & ~1ull);
-      sl[u] = kk < k1 ? kk + delta[run] : -1;
+      const int j = wv + 16 * u + 64 * it;
+      const int e = j * kPbChunk + lane;
+      const bool ok = e < nent;
+      g.key[u] = ok ? a.fkey[k0 + e] : 0u;
+      g.v[u] = ok ? PbVal<VT>::template prod<AT>(val, k0 + e, AT(1)) : AT(0);
+      g.dl[u] = j < nck ? delta[min(cb[u] + lane, nr - 1)] : 0;
     }
+  };
+  auto store = [&](int it, const Stage &g) {
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (sl[u] >= 0) prod[sl[u]] = v1[u] * static_cast<AT>(xs[key[u] & static_cast<unsigned>(kPbCols - 1)]);
+    for (int u = 0; u < 4; u++) {
+      const int e = (wv + 16 * u + 64 * it) * kPbChunk + lane;
+      const unsigned long long starts = __ballot((g.key[u] & kPbRunStart) != 0u);
+      const int d = __shfl(g.dl[u], __popcll(starts & upto & ~1ull), 64);
+      if (e < nent) prod[k0 + e + d] = g.v[u] * static_cast<AT>(xs[g.key[u] & static_cast<unsigned>(kPbCols - 1)]);
+    }
+  };
+  int cbA[4], cbB[4];
+  Stage sA, sB;
+  load_cb(0, cbA);
+  load_stage(0, cbA, sA);
+  load_cb(1, cbB);
+  for (int it = 0; it < nit; it++) {
+    int cbC[4];
+    load_stage(it + 1, cbB, sB);  // (chunks past the end load nothing)
+    load_cb(it + 2, cbC);
+    store(it, sA);
+    sA = sB;
+#pragma unroll
+    for (int u = 0; u < 4; u++) cbB[u] = cbC[u];
   }
 }
 
@@ -632,7 +660,7 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mx(PbScatterArgs
 
 struct PbView {
   const int *bp, *lp;
-  const unsigned short *frow, *lrow, *lcol;
+  const unsigned short *frow, *lkey, *lbase;
   int n;
 };
 
@@ -672,17 +700,40 @@ __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lva
   for (int i = tid; i < kPbRows; i += 1024) ybin[i] = 0.0;
   __syncthreads();
   const int f1 = v.bp[bin + 1], l1 = v.lp[bin + 1];
-  // far products: eight loads per thread issued before their LDS additions
-  // (the compiler does not move global loads across LDS atomics); rounds
-  // with all eight in range run unmasked, and one masked round ends the run
+  // far products in rounds of eight per thread, 1024 entries apart.  Rounds
+  // with all eight in range run unmasked as a software pipeline: the next
+  // round's loads are issued before this round's LDS additions (the
+  // compiler does not move global loads across LDS atomics); one masked
+  // round ends the run
   constexpr int U = 8;
   int kf = v.bp[bin] + tid;
-  for (; kf + (U - 1) * 1024 < f1; kf += U * 1024) pb_far_round<U, false>(v.frow, prod, kf, f1, ybin);
+  if (kf + (U - 1) * 1024 < f1) {
+    int rA[U];
+    AT pA[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) rA[u] = v.frow[kf + u * 1024], pA[u] = prod[kf + u * 1024];
+    for (;;) {
+      const int kn = kf + U * 1024;
+      const bool more = kn + (U - 1) * 1024 < f1;
+      int rB[U];
+      AT pB[U];
+      if (more)
+#pragma unroll
+        for (int u = 0; u < U; u++) rB[u] = v.frow[kn + u * 1024], pB[u] = prod[kn + u * 1024];
+#pragma unroll
+      for (int u = 0; u < U; u++) atomicAdd(&ybin[rA[u]], static_cast<double>(pA[u]));
+      kf = kn;
+      if (!more) break;
+#pragma unroll
+      for (int u = 0; u < U; u++) rA[u] = rB[u], pA[u] = pB[u];
+    }
+  }
   if (kf < f1) pb_far_round<U, true>(v.frow, prod, kf, f1, ybin);
-  // local entries are in row order and each bin's run is padded to groups
-  // of four: one group per thread (aligned vector loads), runs of one row
-  // summed and added once
+  // local entries: one group of four per thread (a 2-byte base row, one
+  // 8-byte key load, aligned value loads), rows decoded from the keys; runs
+  // of one row summed and added once
   constexpr int LU = 2;  // groups per thread in flight (4 measured slower)
+  constexpr int kPad = -1;
   for (int k = v.lp[bin] + kPbLocalGroup * tid; k < l1; k += LU * kPbLocalGroup * 1024) {
     int r[LU][4], c[LU][4];
     AT lv[LU][4], xv[LU][4];
@@ -690,14 +741,19 @@ __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lva
     for (int g = 0; g < LU; g++) {
       const int kg = k + g * kPbLocalGroup * 1024;
       if (g == 0 || kg < l1) {
-        const dev::v4us rr = *reinterpret_cast<const dev::v4us *>(v.lrow + kg);
-        const dev::v4us cc = *reinterpret_cast<const dev::v4us *>(v.lcol + kg);
+        const int rb = v.lbase[kg / kPbLocalGroup];
+        const dev::v4us kk = *reinterpret_cast<const dev::v4us *>(v.lkey + kg);
         PbVal<VT>::template get4<AT>(lval, kg, lv[g]);
-        r[g][0] = rr.x, r[g][1] = rr.y, r[g][2] = rr.z, r[g][3] = rr.w;
-        c[g][0] = cc.x, c[g][1] = cc.y, c[g][2] = cc.z, c[g][3] = cc.w;
+        const unsigned key[4] = {kk.x, kk.y, kk.z, kk.w};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int dr = static_cast<int>(key[u] >> kPbShift);
+          r[g][u] = dr == kPbLocalPad ? kPad : rb + dr;
+          c[g][u] = static_cast<int>(key[u] & static_cast<unsigned>(kPbRows - 1));
+        }
       } else {  // past the bin's run: a group of padding
 #pragma unroll
-        for (int u = 0; u < 4; u++) r[g][u] = kPbPadRow, c[g][u] = 0, lv[g][u] = AT(0);
+        for (int u = 0; u < 4; u++) r[g][u] = kPad, c[g][u] = 0, lv[g][u] = AT(0);
       }
     }
 #pragma unroll
@@ -710,7 +766,7 @@ __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lva
       int row = r[g][0];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        if (r[g][u] == kPbPadRow) break;  // padding closes a bin's run
+        if (r[g][u] == kPad) break;  // padding closes a group
         if (r[g][u] != row) {
           atomicAdd(&ybin[row], acc);
           acc = 0.0;
@@ -718,7 +774,7 @@ __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lva
         }
         acc += static_cast<double>(PbVal<VT>::template mul<AT>(lv[g][u], xv[g][u]));
       }
-      if (row != kPbPadRow) atomicAdd(&ybin[row], acc);
+      if (row != kPad) atomicAdd(&ybin[row], acc);
     }
   }
   __syncthreads();
@@ -874,8 +930,8 @@ __global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restr
 }
 
 PbView pb_view(const PbMatrix &pb) {
-  return PbView{pb.bp.as<int>(), pb.lp.as<int>(), pb.frow.as<unsigned short>(), pb.lrow.as<unsigned short>(),
-                pb.lcol.as<unsigned short>(), pb.n};
+  return PbView{pb.bp.as<int>(), pb.lp.as<int>(), pb.frow.as<unsigned short>(), pb.lkey.as<unsigned short>(),
+                pb.lbase.as<unsigned short>(), pb.n};
 }
 
 PbScatterArgs pb_args(const PbMatrix &pb) {

@@ -256,8 +256,9 @@ __device__ inline void csr_tile_stage(const CsrView<VT> &m, const XT *__restrict
 }
 
 // Rows of entry tile `tile`; epi(row, sum) once per owned row by the thread
-// that sums it.  The caller synchronises the block before reusing `lds`.
-template <typename VT, typename XT, typename Epi, int TN = kCsrTileNnz>
+// that sums it, rows longer than LT left to csr_long_row.  The caller
+// synchronises the block before reusing `lds`.
+template <typename VT, typename XT, typename Epi, int TN = kCsrTileNnz, int LT = TN / 2>
 __device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restrict__ x,
                                         CsrTileLds<typename Val<VT>::A, TN> &lds, Epi &epi, int tile) {
 #pragma clang fp contract(off)
@@ -278,7 +279,7 @@ __device__ inline void csr_tile_rows_at(const CsrView<VT> &m, const XT *__restri
       a = m.rowptr[r] - m.rp0;
       b = m.rowptr[r + 1] - m.rp0;
     }
-    if (b - a > TN / 2) continue;  // summed by csr_long_row
+    if (b - a > LT) continue;  // summed by csr_long_row
     AT s = epi_init<AT>(epi, r, 0);
     // a row owned by the tile starts in it: 0 <= a - e0 < TN, so the reads
     // below stay inside prod[TN + kSumRun]
@@ -311,17 +312,18 @@ __device__ inline void csr_tile_rows(const CsrView<VT> &m, const XT *__restrict_
   csr_tile_rows_at<VT, XT, Epi>(m, x, lds, epi, blockIdx.x);
 }
 
-// One long row (m.long_rows[idx]) per workgroup: chunk products, a
-// fixed-order tree per chunk, chunk totals added in order.  epi by thread 0.
-template <typename VT, typename XT, typename Epi, int TN = kCsrTileNnz>
+// One long row (m.long_rows[idx]) per workgroup: chunk products (LC
+// entries per chunk), a fixed-order tree per chunk, chunk totals added in
+// order.  epi by thread 0.
+template <typename VT, typename XT, typename Epi, int TN = kCsrTileNnz, int LC = TN>
 __device__ inline void csr_long_row_at(const CsrView<VT> &m, const XT *__restrict__ x,
                                        CsrTileLds<typename Val<VT>::A, TN> &lds, Epi &epi, int idx) {
   using AT = typename Val<VT>::A;
   const int r = m.long_rows[idx];
   const long a = m.rowptr[r] - m.rp0, b = m.rowptr[r + 1] - m.rp0;
   AT total = 0;
-  for (long base = a; base < b; base += TN) {
-    const long cnt = min(static_cast<long>(TN), b - base);
+  for (long base = a; base < b; base += LC) {
+    const long cnt = min(static_cast<long>(LC), b - base);
     AT acc = 0;
     for (long e = threadIdx.x; e < cnt; e += kTileThreads) acc += Prod<VT, XT>::one(m, x, base + e);
     total += block_tree_sum<AT>(acc, lds.prod);

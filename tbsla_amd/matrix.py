@@ -83,8 +83,8 @@ class Matrix:
     def fill_cqmat(self, n_row, n_col, c, q, seed_mult=1, pr=0, pc=0, NR=1, NC=1, value_seed=0, device=False):
         """``value_seed`` != 0 applies the project value overlay
         value = 1 - u01(value_seed, i), i the entry's global generation index.
-        ``device=True`` (CSR only) generates the entries on the GPU; the
-        arrays are the same."""
+        ``device=True`` generates the entries on the GPU (COO, SCOO, CSR and
+        ELL; DENSE is refused); the arrays are the same."""
         if device:
             check(lib.tbsla_matrix_fill_cqmat_gpu(self._h, n_row, n_col, c, float(q), int(seed_mult) & 0xFFFFFFFF,
                                                   pr, pc, NR, NC, int(value_seed), 0))
@@ -94,8 +94,9 @@ class Matrix:
         return self
 
     def fill_cqmat_stochastic(self, n_row, n_col, c, q, seed_mult=1, pr=0, pc=0, NR=1, NC=1, device=False):
-        """``device=True`` (CSR only) generates the entries and the column
-        counts on the GPU; the arrays are the same."""
+        """``device=True`` generates the entries and the column counts on the
+        GPU (COO, SCOO, CSR and ELL; DENSE is refused); the arrays are the
+        same."""
         if device:
             check(lib.tbsla_matrix_fill_cqmat_gpu(self._h, n_row, n_col, c, float(q), int(seed_mult) & 0xFFFFFFFF,
                                                   pr, pc, NR, NC, 0, 1))

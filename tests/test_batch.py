@@ -109,6 +109,41 @@ def test_batch_autotuner_trials_and_cache(tb, oracle, gpu):
     assert np.array_equal(o1.cpu().numpy(), o2.cpu().numpy())
 
 
+def test_batch_every_shape_same_bits(tb, gpu):
+    """The autotuner's trials cover every tile size (1024, 2048, 3072
+    entries) and SpMM rows per item (32, 64, 128) as iterations of the run:
+    rows longer than 512 entries are long rows at every tile size, so the
+    result equals a fixed-shape run's bit for bit.  Rows of up to 1600
+    entries (past half of every tile size) in the single-set graph, and an
+    SpMM group whose rows end inside and past each work item size."""
+    rng = np.random.default_rng(2027)
+    graphs = []
+    for n in (4000, 3300):
+        lens = rng.integers(0, 20, n)
+        if n == 4000:
+            lens[[3, 900, 1800, 3999]] = [513, 1025, 1537, 1600]
+        rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        col = rng.integers(0, n, rowptr[-1]).astype(np.int32)
+        cnt = np.bincount(col, minlength=n).astype(np.float64)
+        graphs.append((n, rowptr, col, 1.0 / cnt[col]))
+    sets = [[[123]], [[(29 * r + 3) % 3300] for r in range(5)]]
+
+    def run(autotune):
+        b = tb.PageRankBatch("f64")
+        for (n, rowptr, col, val), st in zip(graphs, sets):
+            b.add_graph(tb.MatrixCSR(n, n, val, rowptr, col), st)
+        out, rep, its = b.run(epsilon=1e-12, max_iteration=300, graph=True, autotune=autotune)
+        return out.cpu().numpy(), rep, its
+
+    a, ra, ia = run(True)
+    b, rb, ib = run(False)
+    if not ra["tune_cached"]:
+        assert ra["tune_iterations"] >= 10, ra
+    assert ra["config"]["tile_nnz"] in (1024, 2048, 3072) and ra["config"]["spmm_rows"] in (32, 64, 128)
+    assert rb["config"]["tile_nnz"] == 2048 and rb["config"]["spmm_rows"] == 64
+    assert np.array_equal(ia, ib) and np.array_equal(a, b)
+
+
 def test_batch_f32_within_tolerance(tb, oracle, gpu):
     """FP32 values, iterates and products (FP64 residual and sums), stopped by
     the sum-normalised bound at 1e-6: within 1e-5 L1 of the FP64 result."""
