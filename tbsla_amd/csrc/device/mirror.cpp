@@ -429,31 +429,34 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
       slot_csc[k] = sl;
       frow[static_cast<std::size_t>(sl)] = static_cast<unsigned short>(r & (kPbRows - 1));
     }
-    // scatter order: superblocks in order, within a superblock by (bin, column)
-    std::vector<int> spc(static_cast<std::size_t>(ns) + 1, 0);
-    for (int s2 = 0; s2 < ns; s2++)
-      spc[static_cast<std::size_t>(s2) + 1] = colcnt[static_cast<std::size_t>(std::min(n, (s2 + 1) * kPbCols))];
-    pb->far_src.resize(nf);
-    // per superblock, the runs of entries bound for one bin, in bin order:
-    // entry o of a run lands in slot o + delta, with delta the run's first
-    // slot minus its first entry
-    std::vector<unsigned short> fkey(nf);
-    std::vector<int> runp(static_cast<std::size_t>(ns) + 1, 0), rdelta;
-    std::vector<int> chp(static_cast<std::size_t>(ns) + 1, 0), cbase;
+    // scatter order over superblocks of `cols` columns: superblocks in
+    // order, within a superblock by (bin, column).  Per superblock, the runs
+    // of entries bound for one bin, in bin order: entry o of a run lands in
+    // slot o + delta, with delta the run's first slot minus its first entry.
     constexpr int kNoDelta = std::numeric_limits<int>::min();
-    {
-      std::vector<int> cnt(static_cast<std::size_t>(nb) + 1), erun;
-      for (int s2 = 0; s2 < ns; s2++) {
+    auto scatter_side = [&](int cols, int &nsb_out, std::vector<int> &far_src, DevBuf &sp_d, DevBuf &fkey_d,
+                            DevBuf &rp_d, DevBuf &rdelta_d, DevBuf &cbase_d, DevBuf &cp_d) {
+      const int ns2 = (n + cols - 1) / cols;
+      nsb_out = ns2;
+      std::vector<int> spc(static_cast<std::size_t>(ns2) + 1, 0);
+      for (int s2 = 0; s2 < ns2; s2++)
+        spc[static_cast<std::size_t>(s2) + 1] =
+            colcnt[static_cast<std::size_t>(std::min(static_cast<long>(n), static_cast<long>(s2 + 1) * cols))];
+      far_src.resize(nf);
+      std::vector<unsigned short> fkey(nf);
+      std::vector<int> runp(static_cast<std::size_t>(ns2) + 1, 0), rdelta;
+      std::vector<int> chp(static_cast<std::size_t>(ns2) + 1, 0), cbase;
+      std::vector<int> cnt(static_cast<std::size_t>(nb) + 1), erun, run_of(static_cast<std::size_t>(nb));
+      for (int s2 = 0; s2 < ns2; s2++) {
         const int k0 = spc[static_cast<std::size_t>(s2)], k1 = spc[static_cast<std::size_t>(s2) + 1];
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int k = k0; k < k1; k++) cnt[static_cast<std::size_t>(csc_row[static_cast<std::size_t>(k)] >> kPbShift) + 1]++;
         // run index of each non-empty bin, and its slot delta (the run's
         // entries are consecutive in CSC order within the bin's slots)
-        std::vector<int> run_of(static_cast<std::size_t>(nb), -1);
+        std::fill(run_of.begin(), run_of.end(), -1);
         int nr = 0;
         for (int b = 0; b < nb; b++)
           if (cnt[static_cast<std::size_t>(b) + 1] > 0) run_of[static_cast<std::size_t>(b)] = nr++;
-        if (nr > 65536) bad("propagation-blocked layout: more than 65536 bins reached from one superblock");
         for (int b = 0; b < nb; b++) cnt[static_cast<std::size_t>(b) + 1] += cnt[static_cast<std::size_t>(b)];
         const std::vector<int> first(cnt);  // first entry of each bin's run, from k0
         const std::size_t rbase = rdelta.size();
@@ -463,9 +466,9 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
           const int b = csc_row[static_cast<std::size_t>(k)] >> kPbShift;
           const int o = k0 + cnt[static_cast<std::size_t>(b)]++;
           const int r = run_of[static_cast<std::size_t>(b)];
-          pb->far_src[static_cast<std::size_t>(o)] = csc_src[static_cast<std::size_t>(k)];
+          far_src[static_cast<std::size_t>(o)] = csc_src[static_cast<std::size_t>(k)];
           const int col = (ci[static_cast<std::size_t>(r0 + csc_src[static_cast<std::size_t>(k)])] - c.get_f_col()) &
-                          (kPbCols - 1);
+                          (cols - 1);
           const bool starts = o - k0 == first[static_cast<std::size_t>(b)];
           fkey[static_cast<std::size_t>(o)] = static_cast<unsigned short>(col | (starts ? kPbRunStart : 0u));
           erun[static_cast<std::size_t>(o - k0)] = r;
@@ -492,15 +495,20 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
         }
         chp[static_cast<std::size_t>(s2) + 1] = chp[static_cast<std::size_t>(s2)] + nck;
       }
-    }
-    pb->sp.upload(spc.data(), spc.size() * sizeof(int));
-    pb->fkey.upload(fkey.data(), fkey.size() * sizeof(unsigned short));
-    pb->rp.upload(runp.data(), runp.size() * sizeof(int));
-    if (rdelta.empty()) rdelta.push_back(0);
-    pb->rdelta.upload(rdelta.data(), rdelta.size() * sizeof(int));
-    if (cbase.empty()) cbase.push_back(0);
-    pb->cbase.upload(cbase.data(), cbase.size() * sizeof(int));
-    pb->cp.upload(chp.data(), chp.size() * sizeof(int));
+      sp_d.upload(spc.data(), spc.size() * sizeof(int));
+      fkey_d.upload(fkey.data(), fkey.size() * sizeof(unsigned short));
+      rp_d.upload(runp.data(), runp.size() * sizeof(int));
+      if (rdelta.empty()) rdelta.push_back(0);
+      rdelta_d.upload(rdelta.data(), rdelta.size() * sizeof(int));
+      if (cbase.empty()) cbase.push_back(0);
+      cbase_d.upload(cbase.data(), cbase.size() * sizeof(int));
+      cp_d.upload(chp.data(), chp.size() * sizeof(int));
+    };
+    int ns_check = 0;
+    scatter_side(kPbCols, ns_check, pb->far_src, pb->sp, pb->fkey, pb->rp, pb->rdelta, pb->cbase, pb->cp);
+    if (ns_check != ns) bad("propagation-blocked layout: superblock count");
+    scatter_side(kPbColsWide, pb->nsb_w, pb->far_src_w, pb->sp_w, pb->fkey_w, pb->rp_w, pb->rdelta_w, pb->cbase_w,
+                 pb->cp_w);
     pb->frow.upload(frow.data(), frow.size() * sizeof(unsigned short));
     pb->bp.upload(bincnt.data(), bincnt.size() * sizeof(int));
     pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));
@@ -514,8 +522,11 @@ const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
     const auto &va = c.get_values();
     const long r0 = c.get_rowptr()[0];
     if (static_cast<long>(va.size()) < static_cast<long>(c.get_rowptr().back())) bad("CSR rowptr points past values");
-    std::vector<double> fv(pb.far_src.size()), lv(pb.loc_src.size());
-    for (std::size_t k = 0; k < fv.size(); k++) fv[k] = va[static_cast<std::size_t>(r0 + pb.far_src[k])];
+    // far values in the scatter order that reads them: F64 (anchor and FP64
+    // products) and F32 (delta products) narrow, BF16 (delta products) wide
+    const std::vector<int> &fsrc = vdt == Dtype::BF16 ? pb.far_src_w : pb.far_src;
+    std::vector<double> fv(fsrc.size()), lv(pb.loc_src.size());
+    for (std::size_t k = 0; k < fv.size(); k++) fv[k] = va[static_cast<std::size_t>(r0 + fsrc[k])];
     for (std::size_t k = 0; k < lv.size(); k++)
       lv[k] = pb.loc_src[k] >= 0 ? va[static_cast<std::size_t>(r0 + pb.loc_src[k])] : 0.0;  // padding: 0
     upload_values(pb.fval[static_cast<int>(vdt)], fv.data(), fv.size(), vdt);

@@ -93,12 +93,18 @@ constexpr int kPbColShift = 13;
 constexpr int kPbLocalGroup = 4;
 constexpr int kPbLocalPad = 7;  // the row field of a padding key (3 bits above the column)
 constexpr int kPbCols = 1 << kPbColShift;
+// the wide scatter order: superblocks of kPbColsWide columns, for the FP32
+// delta products of mixed runs (a 4-byte x slice of that width fills the
+// same 64 KB of LDS as an 8-byte slice of kPbCols)
+constexpr int kPbColShiftWide = 14;
+constexpr int kPbColsWide = 1 << kPbColShiftWide;
 // scatter keys: the column within the superblock, and this flag on the first
 // entry of each run; the run of an entry is found per wavefront-sized chunk
 constexpr unsigned kPbRunStart = 0x8000u;
 constexpr int kPbChunk = 64;
 static_assert(kPbShift + 3 <= 16, "local keys hold the column and a 3-bit row offset in 16 bits");
 static_assert(kPbCols <= 0x2000, "scatter keys hold the column in 13 bits");
+static_assert(kPbColsWide <= kPbRunStart, "wide scatter keys hold the column in 15 bits");
 
 struct PbMatrix {
   int n = 0, nbins = 0, nsb = 0;
@@ -111,7 +117,15 @@ struct PbMatrix {
   // so the run of an entry is cbase plus the run starts after the chunk's
   // first entry up to the entry itself.
   DevBuf sp, fkey, rp, rdelta, cbase, cp, frow, bp, lp, lkey, lbase;
-  DevBuf fval[3], lval[3];  // by Dtype F64, F32, BF16 (built on demand)
+  // values by Dtype F64, F32, BF16 (built on demand); the far values in the
+  // scatter order that reads them: F64 and F32 narrow, BF16 wide
+  DevBuf fval[3], lval[3];
+  // The far entries in the wide scatter order (superblocks of kPbColsWide
+  // columns, the same product slots), read by the delta products of BF16 and
+  // value-free mixed runs.
+  int nsb_w = 0;
+  DevBuf sp_w, fkey_w, rp_w, rdelta_w, cbase_w, cp_w;
+  std::vector<int> far_src_w;
   DevBuf prod;              // 8-byte product slots, n_far
   std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry (-1: padding)
 };

@@ -27,6 +27,8 @@
 
 #include "kernels.hpp"
 #include "mirror.hpp"
+#include <type_traits>
+
 #include "pagerank.hpp"
 #include "tile_csr.hpp"
 
@@ -514,23 +516,28 @@ struct AnchorMove {
 
 // The far products of one superblock (picked from blockIdx): its slice of x
 // staged in xs (LDS), then every far product v * x[c] written to its slot.
-// With MOVE the slice is the anchor move's result instead of x.
-template <typename VT, typename XT, typename AT, bool MOVE>
+// With MOVE the slice is the anchor move's result instead of x.  COLS: the
+// superblock width of the layout `a` (kPbCols, or kPbColsWide for the FP32
+// delta products).
+template <typename VT, typename XT, typename AT, bool MOVE, int COLS>
 __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restrict__ val, const XT *__restrict__ x,
                                      AT *__restrict__ prod, XT *xs, const AnchorMove &am, double *red) {
 #pragma clang fp contract(off)
   // superblocks of one XCD (blockIdx % 8) are contiguous, so the runs that
   // share a cache line at a run boundary are written through the same L2
-  const int per = (gridDim.x + 7) / 8;
+  // (the grid may be larger than the layout needs: the mixed kernel serves
+  // two layouts)
+  const int per = (a.nsb + 7) / 8;
+  if (static_cast<int>(blockIdx.x) >= 8 * per) return;
   const int sb = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (sb >= a.nsb) return;
   const int k0 = a.sp[sb], k1 = a.sp[sb + 1];
   const int *__restrict__ delta = a.rdelta + a.rp[sb];  // slot - entry of each run of this superblock
   const int *__restrict__ cbase = a.cbase + a.cp[sb];   // run of the first entry of each chunk
-  const long c0 = static_cast<long>(sb) << kPbColShift;
-  const int nc = static_cast<int>(min(static_cast<long>(kPbCols), static_cast<long>(a.n) - c0));
+  const long c0 = static_cast<long>(sb) * COLS;
+  const int nc = static_cast<int>(min(static_cast<long>(COLS), static_cast<long>(a.n) - c0));
   {
-    constexpr int S = kPbCols / kPbScatterThreads;  // x values per thread, all loads issued first
+    constexpr int S = COLS / kPbScatterThreads;  // x values per thread, all loads issued first
     XT t[S];
     CSum cs{0, 0};
 #pragma unroll
@@ -612,7 +619,7 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
       const int e = (wv + 16 * u + 64 * it) * kPbChunk + lane;
       const unsigned long long starts = __ballot((g.key[u] & kPbRunStart) != 0u);
       const int d = __shfl(g.dl[u], __popcll(starts & upto & ~1ull), 64);
-      if (e < nent) prod[k0 + e + d] = g.v[u] * static_cast<AT>(xs[g.key[u] & static_cast<unsigned>(kPbCols - 1)]);
+      if (e < nent) prod[k0 + e + d] = g.v[u] * static_cast<AT>(xs[g.key[u] & static_cast<unsigned>(COLS - 1)]);
     }
   };
   int cbA[4], cbB[4];
@@ -638,24 +645,29 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a,
                                                                 const PrState *st) {
   if (st->done) return;
   __shared__ XT xs[kPbCols];
-  pb_scatter_sb<VT, XT, AT, false>(a, val, x, prod, xs, AnchorMove{}, nullptr);
+  pb_scatter_sb<VT, XT, AT, false, kPbCols>(a, val, x, prod, xs, AnchorMove{}, nullptr);
 }
 
 // Mixed runs, one launch per iteration: in an iteration that moves the
 // anchor, the anchor move and the FP64 anchor product's far products (values
-// VA, FP64); otherwise the delta product's (values VD, delta XD, sums AD).
-template <typename VA, typename VD, typename XD, typename AD>
-__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mx(PbScatterArgs a, const VA *__restrict__ vala,
+// VA, FP64, layout a); otherwise the delta product's (values VD, delta XD,
+// sums AD) over layout ad of superblock width DCOLS (kPbColsWide: the 4-byte
+// x slice fills the same LDS).
+template <typename VA, typename VD, typename XD, typename AD, int DCOLS>
+__global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mx(PbScatterArgs a, PbScatterArgs ad,
+                                                                   const VA *__restrict__ vala,
                                                                    const VD *__restrict__ vald,
                                                                    const XD *__restrict__ xd, void *prod,
                                                                    AnchorMove am, const PrState *st) {
+  static_assert(sizeof(XD) * DCOLS <= sizeof(double) * kPbCols, "the delta slice fits the LDS");
   if (st->done) return;
   __shared__ double xs[kPbCols];
   __shared__ double red[48];
   if (st->need_anchor)
-    pb_scatter_sb<VA, double, double, true>(a, vala, nullptr, static_cast<double *>(prod), xs, am, red);
+    pb_scatter_sb<VA, double, double, true, kPbCols>(a, vala, nullptr, static_cast<double *>(prod), xs, am, red);
   else
-    pb_scatter_sb<VD, XD, AD, false>(a, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am, nullptr);
+    pb_scatter_sb<VD, XD, AD, false, DCOLS>(ad, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am,
+                                            nullptr);
 }
 
 struct PbView {
@@ -939,6 +951,18 @@ PbScatterArgs pb_args(const PbMatrix &pb) {
                        pb.cbase.as<int>(), pb.cp.as<int>(),              pb.nsb,          pb.n};
 }
 
+// The scatter order of the delta products with values VD (the far values of
+// dtype F32 are stored narrow, BF16 wide; the pb_mirror build follows this).
+template <typename VD>
+constexpr bool pb_delta_wide() {
+  return !std::is_same<VD, float>::value;
+}
+
+PbScatterArgs pb_args_wide(const PbMatrix &pb) {
+  return PbScatterArgs{pb.sp_w.as<int>(),    pb.fkey_w.as<unsigned short>(), pb.rp_w.as<int>(), pb.rdelta_w.as<int>(),
+                       pb.cbase_w.as<int>(), pb.cp_w.as<int>(),              pb.nsb_w,          pb.n};
+}
+
 dim3 pb_scatter_grid(const PbMatrix &pb) { return dim3((pb.nsb + 7) / 8 * 8); }
 
 // One propagation-blocked product with the reference-rule update (FP64
@@ -1024,8 +1048,13 @@ void launch_pb_mx_tm(const PbMatrix &pb, int va, int vd, const MxBufs &b, const 
   const VD *ld = vd >= 0 ? pb.lval[vd].as<VD>() : nullptr;
   const double *xa = b.am.z ? b.am.z : b.am.xhat;
   const XD *xd = static_cast<const XD *>(b.xd);
-  hipLaunchKernelGGL((pb_scatter_mx<VA, VD, XD, float>), pb_scatter_grid(pb), dim3(kPbScatterThreads), 0, s,
-                     pb_args(pb), fa, fd, xd, pb.prod.get(), b.am, st);
+  // the delta products of BF16 and value-free runs take the wide scatter
+  // order, FP32-valued ones the narrow one (measured: value-free FP32 delta
+  // scatter 154 -> 140 us wide, BF16 196 -> 192, FP32 191 -> 196)
+  constexpr bool wide = pb_delta_wide<VD>();
+  hipLaunchKernelGGL((pb_scatter_mx<VA, VD, XD, float, wide ? kPbColsWide : kPbCols>), pb_scatter_grid(pb),
+                     dim3(kPbScatterThreads), 0, s, pb_args(pb), wide ? pb_args_wide(pb) : pb_args(pb), fa, fd, xd,
+                     pb.prod.get(), b.am, st);
   hipLaunchKernelGGL((pb_gather_mx<VA, VD, XD, float, TM, PAT32>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), la,
                      xa, ld, xd, pb.prod.get(), tp, b.am.xhat, b.am.xhat32, b.w, b.dold, b.dnew, b.zdnew, b.colscale32, st,
                      sp.perr, sp.ps, sp.pc);
