@@ -676,29 +676,17 @@ struct PbView {
   int n;
 };
 
-// One round of a bin's far products: U loads of (row, product) per thread,
-// 1024 entries apart, then their LDS additions. MASKED checks each entry
-// against the end of the bin's run.
-template <int U, bool MASKED, typename AT>
-__device__ inline void pb_far_round(const unsigned short *__restrict__ frow, const AT *__restrict__ prod, int k,
-                                    int f1, double *ybin) {
-  int r[U];
-  double pv[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int kk = k + u * 1024;
-    if (MASKED) {
-      r[u] = kk < f1 ? frow[kk] : -1;
-      pv[u] = kk < f1 ? static_cast<double>(prod[kk]) : 0.0;
-    } else {
-      r[u] = frow[kk];
-      pv[u] = static_cast<double>(prod[kk]);
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < U; u++)
-    if (!MASKED || r[u] >= 0) atomicAdd(&ybin[r[u]], pv[u]);
-}
+// Two products of type T in one vector load.
+template <typename T>
+struct PbPair;
+template <>
+struct PbPair<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct PbPair<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
 
 // The row sums of one bin (blockIdx) into the FP64 LDS accumulator ybin
 // (kPbRows entries): the bin's far products and its local products, x read
@@ -712,35 +700,68 @@ __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lva
   for (int i = tid; i < kPbRows; i += 1024) ybin[i] = 0.0;
   __syncthreads();
   const int f1 = v.bp[bin + 1], l1 = v.lp[bin + 1];
-  // far products in rounds of eight per thread, 1024 entries apart.  Rounds
-  // with all eight in range run unmasked as a software pipeline: the next
-  // round's loads are issued before this round's LDS additions (the
-  // compiler does not move global loads across LDS atomics); one masked
-  // round ends the run
-  constexpr int U = 8;
-  int kf = v.bp[bin] + tid;
-  if (kf + (U - 1) * 1024 < f1) {
-    int rA[U];
-    AT pA[U];
+  // far products in pairs: one 4-byte load of two rows and one 8- or
+  // 16-byte load of two products, four pairs per thread per round, 1024
+  // pairs apart (an odd first or last entry of the bin's run goes alone).
+  // Rounds with all four pairs in range run unmasked as a software
+  // pipeline: the next round's loads are issued before this round's LDS
+  // additions (the compiler does not move global loads across LDS atomics);
+  // one masked round ends the run
+  int f0 = v.bp[bin], fe = f1;
+  if ((f0 & 1) && f0 < fe) {
+    if (tid == 0) atomicAdd(&ybin[v.frow[f0]], static_cast<double>(prod[f0]));
+    f0++;
+  }
+  if (((fe - f0) & 1) && fe > f0) {
+    if (tid == 1023) atomicAdd(&ybin[v.frow[fe - 1]], static_cast<double>(prod[fe - 1]));
+    fe--;
+  }
+  {
+    using AT2 = typename PbPair<AT>::type;
+    const unsigned *__restrict__ frow2 = reinterpret_cast<const unsigned *>(v.frow);
+    const AT2 *__restrict__ prod2 = reinterpret_cast<const AT2 *>(prod);
+    const int p1 = fe >> 1;
+    constexpr int U = 4;
+    int kp = (f0 >> 1) + tid;
+    auto add = [&](unsigned rr, AT2 pp) {
+      atomicAdd(&ybin[rr & 0xFFFFu], static_cast<double>(pp.x));
+      atomicAdd(&ybin[rr >> 16], static_cast<double>(pp.y));
+    };
+    if (kp + (U - 1) * 1024 < p1) {
+      unsigned rA[U];
+      AT2 pA[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) rA[u] = v.frow[kf + u * 1024], pA[u] = prod[kf + u * 1024];
-    for (;;) {
-      const int kn = kf + U * 1024;
-      const bool more = kn + (U - 1) * 1024 < f1;
-      int rB[U];
-      AT pB[U];
-      if (more)
+      for (int u = 0; u < U; u++) rA[u] = frow2[kp + u * 1024], pA[u] = prod2[kp + u * 1024];
+      for (;;) {
+        const int kn = kp + U * 1024;
+        const bool more = kn + (U - 1) * 1024 < p1;
+        unsigned rB[U];
+        AT2 pB[U];
+        if (more)
 #pragma unroll
-        for (int u = 0; u < U; u++) rB[u] = v.frow[kn + u * 1024], pB[u] = prod[kn + u * 1024];
+          for (int u = 0; u < U; u++) rB[u] = frow2[kn + u * 1024], pB[u] = prod2[kn + u * 1024];
 #pragma unroll
-      for (int u = 0; u < U; u++) atomicAdd(&ybin[rA[u]], static_cast<double>(pA[u]));
-      kf = kn;
-      if (!more) break;
+        for (int u = 0; u < U; u++) add(rA[u], pA[u]);
+        kp = kn;
+        if (!more) break;
 #pragma unroll
-      for (int u = 0; u < U; u++) rA[u] = rB[u], pA[u] = pB[u];
+        for (int u = 0; u < U; u++) rA[u] = rB[u], pA[u] = pB[u];
+      }
+    }
+    if (kp < p1) {
+      unsigned r[U];
+      AT2 pv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int kk = kp + u * 1024;
+        r[u] = kk < p1 ? frow2[kk] : 0u;
+        pv[u] = kk < p1 ? prod2[kk] : AT2{AT(0), AT(0)};
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (kp + u * 1024 < p1) add(r[u], pv[u]);
     }
   }
-  if (kf < f1) pb_far_round<U, true>(v.frow, prod, kf, f1, ybin);
   // local entries: one group of four per thread (a 2-byte base row, one
   // 8-byte key load, aligned value loads), rows decoded from the keys; runs
   // of one row summed and added once
