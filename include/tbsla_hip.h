@@ -113,7 +113,9 @@ typedef struct {
                         TBSLA_PR_PATTERN_F32 (value-free FP32 deltas on an FP64 anchor) */
   int rule;          /* 0: reference residual < epsilon; 1: sum-normalised bound */
   int check_every;   /* iterations between convergence reads */
-  double anchor_tau; /* 0: default */
+  double anchor_tau; /* mixed precisions: move the anchor when the bound falls below anchor_tau
+                        times its value at the last move; 0: the default (3e-6 for F32 and
+                        PATTERN_F32, 1e-2 for BF16) */
   const int *nodes;  /* personalised teleport set, or NULL */
   int n_nodes;
   int gather;        /* CSR products: 0 auto (propagation-blocked for n >= 2^16),
