@@ -162,6 +162,14 @@ int tbsla_hip_page_rank_ex(tbsla_matrix *m, const tbsla_pagerank_params *p, cons
 int tbsla_hip_page_rank_noalloc(tbsla_matrix *m, const tbsla_pagerank_params *p, double *out_dev, size_t n_out,
                                 void *stream, tbsla_pagerank_report *report);
 
+/* Self-check of the propagation-blocked layout that PageRank's CSR products
+ * use (gather 0/2): builds it on the GPU (device sorts, the build page_rank
+ * uses) and on the host cores, and compares every index array.  0 when they
+ * are equal; TBSLA_EINVAL naming the first array that differs, or when the
+ * matrix is not a square single-block CSR matrix.  No reference counterpart
+ * (a test hook of this library's layout). */
+int tbsla_hip_pb_layout_check(tbsla_matrix *m);
+
 typedef struct {
   double beta;
   double epsilon;    /* stop when |lambda_k - lambda_{k-1}|_1 <= epsilon * |lambda_k|_1 */

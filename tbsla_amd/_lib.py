@@ -91,6 +91,7 @@ SIGNATURES = {
     "tbsla_matrix_personalized_page_rank": (_i, [_vp, _d, _d, _i, _ip, _i, _dp, _sz, _ip]),
     "tbsla_hip_prepare": (_i, [_vp, _i]),
     "tbsla_hip_mirror_info": (_i, [_vp, _i, C.POINTER(_l)]),
+    "tbsla_hip_pb_layout_check": (_i, [_vp]),
     "tbsla_hip_Ax": (_i, [_vp, _i, _vp, _sz, _i, _vp, _sz, _i, _i, _vp]),
     "tbsla_hip_fill": (_i, [_vp, _i, _sz, _d, _vp]),
     "tbsla_hip_page_rank": (_i, [_vp, C.POINTER(PageRankParams), _dp, _sz, _vp, C.POINTER(PageRankReport)]),

@@ -548,6 +548,18 @@ int tbsla_hip_mirror_info(tbsla_matrix *m, int value_dtype, long *out) {
   });
 }
 
+int tbsla_hip_pb_layout_check(tbsla_matrix *m) {
+  if (!m) return fail(TBSLA_EINVAL, "tbsla: NULL argument");
+  return guarded([&] {
+    if (m->m->format() != cpp::FORMAT_CSR) throw std::invalid_argument("tbsla: the propagation-blocked layout needs a CSR matrix");
+    const auto &c = dynamic_cast<const cpp::MatrixCSR &>(*m->m);
+    auto host = hip::pb_build_host(c);  // refuses a matrix that is not square in one block
+    auto gpu = hip::pb_build_gpu(hip::mirror(*m->m, hip::Dtype::F64), c.get_n_col());
+    const std::string diff = hip::pb_layout_compare(*gpu, *host);
+    if (!diff.empty()) throw std::invalid_argument("tbsla: propagation-blocked layouts differ in " + diff);
+  });
+}
+
 int tbsla_hip_Ax(tbsla_matrix *m, int value_dtype, const void *x, size_t nx, int x_dtype, void *y, size_t ny,
                  int y_dtype, int vect_incr, void *stream) {
   if (!m || (!x && nx) || (!y && ny)) return fail(TBSLA_EINVAL, "tbsla: NULL argument");

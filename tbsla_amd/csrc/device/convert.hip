@@ -18,6 +18,7 @@
 #include "../host/gen.hpp"
 #include "common.hpp"
 #include "convert.hpp"
+#include "sort.hpp"
 
 namespace tbsla {
 namespace hip {
@@ -98,6 +99,8 @@ __global__ __launch_bounds__(kThreads) void add_chunk_offsets(int *__restrict__ 
   }
 }
 
+}  // namespace
+
 // In-place exclusive scan of a[0, m) on the device (chunk scans, then the
 // chunk totals scanned the same way, recursively).
 void exclusive_scan(int *a, long m, hipStream_t s) {
@@ -112,6 +115,8 @@ void exclusive_scan(int *a, long m, hipStream_t s) {
   }
   TBSLA_HIP_CHECK(hipGetLastError());
 }
+
+namespace {
 
 // Lanes of this wavefront whose 8-bit digit (and validity) equal this lane's.
 __device__ inline unsigned long long digit_peers(unsigned d, bool valid) {
@@ -175,6 +180,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter(const unsigned long lo
   }
 }
 
+}  // namespace
+
 // Sorts (key, val)[0, n) by key bits [0, bits), stable; the result is in
 // (key, val) again (the ping-pong buffers k2, v2 hold n entries each).
 void radix_sort_pairs(unsigned long long *key, int *val, unsigned long long *k2, int *v2, long n, int bits,
@@ -199,6 +206,8 @@ void radix_sort_pairs(unsigned long long *key, int *val, unsigned long long *k2,
     TBSLA_HIP_CHECK(hipMemcpyAsync(val, va, sizeof(int) * static_cast<std::size_t>(n), hipMemcpyDeviceToDevice, s));
   }
 }
+
+namespace {
 
 // CSR: entry k of the sorted order
 __global__ __launch_bounds__(kThreads) void coo_gather(long n, const int *__restrict__ idx, const int *__restrict__ col,

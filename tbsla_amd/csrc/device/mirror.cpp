@@ -330,211 +330,253 @@ const DeviceMatrix &transpose_mirror(const cpp::Matrix &m) {
   return *cache.transposed;
 }
 
+// The host build of the layout (pb_build_gpu builds the same arrays on the
+// device; this one is the check of that build, tbsla_hip_pb_layout_check).
+std::unique_ptr<PbMatrix> pb_build_host(const cpp::MatrixCSR &c) {
+  const auto &rp = c.get_rowptr();
+  const auto &ci = c.get_colidx();
+  const int n = c.get_n_col();
+  if (c.get_n_row() != n || static_cast<int>(rp.size()) != n + 1) bad("the propagation-blocked layout needs a square matrix in one block");
+  const long r0 = rp[0];
+  const long nnz = static_cast<long>(rp.back()) - r0;
+  // entry e of the used range is colidx[e], e = rowptr[i] - rowptr[0] + k (MatrixCSR.cpp:135)
+  if (static_cast<long>(ci.size()) < nnz) bad("CSR rowptr points past colidx");
+  auto pb = std::unique_ptr<PbMatrix>(new PbMatrix());
+  pb->n = n;
+  pb->nbins = (n + kPbRows - 1) / kPbRows;
+  pb->nsb = (n + kPbCols - 1) / kPbCols;
+  const int nb = pb->nbins, ns = pb->nsb;
+  // pass 1: counts per bin (far; local groups) and per column (far).  The
+  // local groups of a bin are formed in row order: an entry opens a new
+  // group when the current one is full or its row is kPbLocalPad or more
+  // rows past the group's base row
+  std::vector<int> colcnt(static_cast<std::size_t>(n) + 1, 0), bincnt(static_cast<std::size_t>(nb) + 1, 0),
+      loccnt(static_cast<std::size_t>(nb) + 1, 0);
+  int gfill = kPbLocalGroup, gbase = 0;  // the current bin's open group
+  for (int r = 0; r < n; r++) {
+    const int b = r >> kPbShift;
+    if ((r & (kPbRows - 1)) == 0) gfill = kPbLocalGroup;  // a new bin
+    for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
+      const int col = ci[static_cast<std::size_t>(e)] - c.get_f_col();
+      if (col < 0 || col >= n) bad("CSR column index outside the matrix");
+      if ((col >> kPbShift) == b) {
+        if (gfill == kPbLocalGroup || r - gbase >= kPbLocalPad) {
+          loccnt[static_cast<std::size_t>(b) + 1] += kPbLocalGroup;
+          gfill = 0;
+          gbase = r;
+        }
+        gfill++;
+      } else {
+        colcnt[static_cast<std::size_t>(col) + 1]++;
+        bincnt[static_cast<std::size_t>(b) + 1]++;
+      }
+    }
+  }
+  for (int j = 0; j < n; j++) colcnt[static_cast<std::size_t>(j) + 1] += colcnt[static_cast<std::size_t>(j)];
+  for (int b = 0; b < nb; b++) {
+    bincnt[static_cast<std::size_t>(b) + 1] += bincnt[static_cast<std::size_t>(b)];
+    loccnt[static_cast<std::size_t>(b) + 1] += loccnt[static_cast<std::size_t>(b)];
+  }
+  pb->n_far = colcnt[static_cast<std::size_t>(n)];
+  pb->n_loc = loccnt[static_cast<std::size_t>(nb)];
+  if (pb->n_far >= (1L << 31) || pb->n_loc >= (1L << 31)) bad("propagation-blocked layout limited to 2^31 entries");
+  const std::size_t nf = static_cast<std::size_t>(pb->n_far), nl = static_cast<std::size_t>(pb->n_loc);
+  // pass 2: far entries in column order (rows ascending), local entries
+  // bin-grouped (the groups of pass 1; unused slots stay padding)
+  std::vector<int> csc_src(nf), csc_row(nf);
+  std::vector<int> loc_src_h(nl, -1), far_src_h, far_src_w_h;
+  std::vector<unsigned short> lkey(nl, static_cast<unsigned short>(kPbLocalPad << kPbShift));
+  std::vector<unsigned short> lbase(nl / kPbLocalGroup, 0);
+  std::vector<int> ccur(colcnt.begin(), colcnt.end() - 1);
+  long lnext = 0;  // first slot of the next local group
+  int lslot = 0;   // next slot in the open group (kPbLocalGroup: none open)
+  gfill = kPbLocalGroup;
+  for (int r = 0; r < n; r++) {
+    const int b = r >> kPbShift;
+    if ((r & (kPbRows - 1)) == 0) {
+      gfill = kPbLocalGroup;
+      lnext = loccnt[static_cast<std::size_t>(b)];
+    }
+    for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
+      const int col = ci[static_cast<std::size_t>(e)] - c.get_f_col();
+      if ((col >> kPbShift) == b) {
+        if (gfill == kPbLocalGroup || r - gbase >= kPbLocalPad) {
+          lslot = static_cast<int>(lnext);
+          lnext += kPbLocalGroup;
+          gfill = 0;
+          gbase = r;
+          lbase[static_cast<std::size_t>(lslot / kPbLocalGroup)] = static_cast<unsigned short>(r & (kPbRows - 1));
+        }
+        const std::size_t k = static_cast<std::size_t>(lslot + gfill);
+        gfill++;
+        loc_src_h[k] = static_cast<int>(e);
+        lkey[k] = static_cast<unsigned short>((col & (kPbRows - 1)) | ((r - gbase) << kPbShift));
+      } else {
+        const int k = ccur[static_cast<std::size_t>(col)]++;
+        csc_src[static_cast<std::size_t>(k)] = static_cast<int>(e);
+        csc_row[static_cast<std::size_t>(k)] = r;
+      }
+    }
+  }
+  // bin-grouped slot of every far entry: bins in order, each in column order
+  std::vector<int> slot_csc(nf), bcur(bincnt.begin(), bincnt.end() - 1);
+  std::vector<unsigned short> frow(nf);
+  for (std::size_t k = 0; k < nf; k++) {
+    const int r = csc_row[k];
+    const int sl = bcur[static_cast<std::size_t>(r >> kPbShift)]++;
+    slot_csc[k] = sl;
+    frow[static_cast<std::size_t>(sl)] = static_cast<unsigned short>(r & (kPbRows - 1));
+  }
+  // scatter order over superblocks of `cols` columns: superblocks in
+  // order, within a superblock by (bin, column).  Per superblock, the runs
+  // of entries bound for one bin, in bin order: entry o of a run lands in
+  // slot o + delta, with delta the run's first slot minus its first entry.
+  constexpr int kNoDelta = std::numeric_limits<int>::min();
+  auto scatter_side = [&](int cols, int &nsb_out, std::vector<int> &far_src, DevBuf &sp_d, DevBuf &fkey_d,
+                          DevBuf &rp_d, DevBuf &rdelta_d, DevBuf &cbase_d, DevBuf &cp_d) {
+    const int ns2 = (n + cols - 1) / cols;
+    nsb_out = ns2;
+    std::vector<int> spc(static_cast<std::size_t>(ns2) + 1, 0);
+    for (int s2 = 0; s2 < ns2; s2++)
+      spc[static_cast<std::size_t>(s2) + 1] =
+          colcnt[static_cast<std::size_t>(std::min(static_cast<long>(n), static_cast<long>(s2 + 1) * cols))];
+    far_src.resize(nf);
+    std::vector<unsigned short> fkey(nf);
+    std::vector<int> runp(static_cast<std::size_t>(ns2) + 1, 0), rdelta;
+    std::vector<int> chp(static_cast<std::size_t>(ns2) + 1, 0), cbase;
+    std::vector<int> cnt(static_cast<std::size_t>(nb) + 1), erun, run_of(static_cast<std::size_t>(nb));
+    for (int s2 = 0; s2 < ns2; s2++) {
+      const int k0 = spc[static_cast<std::size_t>(s2)], k1 = spc[static_cast<std::size_t>(s2) + 1];
+      std::fill(cnt.begin(), cnt.end(), 0);
+      for (int k = k0; k < k1; k++) cnt[static_cast<std::size_t>(csc_row[static_cast<std::size_t>(k)] >> kPbShift) + 1]++;
+      // run index of each non-empty bin, and its slot delta (the run's
+      // entries are consecutive in CSC order within the bin's slots)
+      std::fill(run_of.begin(), run_of.end(), -1);
+      int nr = 0;
+      for (int b = 0; b < nb; b++)
+        if (cnt[static_cast<std::size_t>(b) + 1] > 0) run_of[static_cast<std::size_t>(b)] = nr++;
+      for (int b = 0; b < nb; b++) cnt[static_cast<std::size_t>(b) + 1] += cnt[static_cast<std::size_t>(b)];
+      const std::vector<int> first(cnt);  // first entry of each bin's run, from k0
+      const std::size_t rbase = rdelta.size();
+      rdelta.resize(rbase + static_cast<std::size_t>(nr), kNoDelta);
+      erun.assign(static_cast<std::size_t>(k1 - k0), -1);
+      for (int k = k0; k < k1; k++) {  // column order, so each bin's run stays in column order
+        const int b = csc_row[static_cast<std::size_t>(k)] >> kPbShift;
+        const int o = k0 + cnt[static_cast<std::size_t>(b)]++;
+        const int r = run_of[static_cast<std::size_t>(b)];
+        far_src[static_cast<std::size_t>(o)] = csc_src[static_cast<std::size_t>(k)];
+        const int col = (ci[static_cast<std::size_t>(csc_src[static_cast<std::size_t>(k)])] - c.get_f_col()) &
+                        (cols - 1);
+        const bool starts = o - k0 == first[static_cast<std::size_t>(b)];
+        fkey[static_cast<std::size_t>(o)] = static_cast<unsigned short>(col | (starts ? kPbRunStart : 0u));
+        erun[static_cast<std::size_t>(o - k0)] = r;
+        // a bin's slots follow CSC order and a run is one superblock's
+        // column range of that order, so slot - o is the same for the
+        // whole run
+        const int d = slot_csc[static_cast<std::size_t>(k)] - o;
+        int &rd = rdelta[rbase + static_cast<std::size_t>(r)];
+        if (rd != kNoDelta && rd != d) bad("propagation-blocked layout: run slots are not consecutive");
+        rd = d;
+      }
+      runp[static_cast<std::size_t>(s2) + 1] = static_cast<int>(rdelta.size());
+      // run of the first entry of each chunk, checked against the run
+      // every entry gets from it and the run-start flags
+      const int nck = (k1 - k0 + kPbChunk - 1) / kPbChunk;
+      for (int j = 0; j < nck; j++) {
+        const int e0 = j * kPbChunk, e1 = std::min(k1 - k0, e0 + kPbChunk);
+        int run = erun[static_cast<std::size_t>(e0)];
+        cbase.push_back(run);
+        for (int e = e0; e < e1; e++) {
+          if (e > e0 && (fkey[static_cast<std::size_t>(k0 + e)] & kPbRunStart)) run++;
+          if (run != erun[static_cast<std::size_t>(e)]) bad("propagation-blocked layout: run starts do not give the runs");
+        }
+      }
+      chp[static_cast<std::size_t>(s2) + 1] = chp[static_cast<std::size_t>(s2)] + nck;
+    }
+    sp_d.upload(spc.data(), spc.size() * sizeof(int));
+    fkey_d.upload(fkey.data(), fkey.size() * sizeof(unsigned short));
+    rp_d.upload(runp.data(), runp.size() * sizeof(int));
+    if (rdelta.empty()) rdelta.push_back(0);
+    rdelta_d.upload(rdelta.data(), rdelta.size() * sizeof(int));
+    if (cbase.empty()) cbase.push_back(0);
+    cbase_d.upload(cbase.data(), cbase.size() * sizeof(int));
+    cp_d.upload(chp.data(), chp.size() * sizeof(int));
+  };
+  int ns_check = 0;
+  scatter_side(kPbCols, ns_check, far_src_h, pb->sp, pb->fkey, pb->rp, pb->rdelta, pb->cbase, pb->cp);
+  if (ns_check != ns) bad("propagation-blocked layout: superblock count");
+  scatter_side(kPbColsWide, pb->nsb_w, far_src_w_h, pb->sp_w, pb->fkey_w, pb->rp_w, pb->rdelta_w, pb->cbase_w,
+               pb->cp_w);
+  pb->frow.upload(frow.data(), frow.size() * sizeof(unsigned short));
+  pb->bp.upload(bincnt.data(), bincnt.size() * sizeof(int));
+  pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));
+  pb->lkey.upload(lkey.data(), lkey.size() * sizeof(unsigned short));
+  pb->lbase.upload(lbase.data(), lbase.size() * sizeof(unsigned short));
+  pb->prod.alloc(sizeof(double) * std::max<std::size_t>(1, nf));
+  pb->far_src.upload(far_src_h.data(), far_src_h.size() * sizeof(int));
+  pb->far_src_w.upload(far_src_w_h.data(), far_src_w_h.size() * sizeof(int));
+  pb->loc_src.upload(loc_src_h.data(), loc_src_h.size() * sizeof(int));
+  TBSLA_HIP_CHECK(hipDeviceSynchronize());
+  return pb;
+}
+
+std::string pb_layout_compare(const PbMatrix &a, const PbMatrix &b) {
+  if (a.n != b.n || a.nbins != b.nbins || a.nsb != b.nsb || a.nsb_w != b.nsb_w || a.n_far != b.n_far ||
+      a.n_loc != b.n_loc)
+    return "sizes";
+  auto dl = [](const DevBuf &d, std::size_t bytes) {
+    std::vector<unsigned char> v(bytes);
+    if (bytes) TBSLA_HIP_CHECK(hipMemcpy(v.data(), d.get(), bytes, hipMemcpyDeviceToHost));
+    return v;
+  };
+  const std::size_t nf = static_cast<std::size_t>(a.n_far), nl = static_cast<std::size_t>(a.n_loc);
+  const std::size_t i4 = sizeof(int), u2 = sizeof(unsigned short);
+  auto last = [&](const DevBuf &d, int count) {  // the total an offset table ends with
+    int v = 0;
+    TBSLA_HIP_CHECK(hipMemcpy(&v, d.as<int>() + count, sizeof(int), hipMemcpyDeviceToHost));
+    return static_cast<std::size_t>(v);
+  };
+  struct Item {
+    const char *name;
+    const DevBuf &x, &y;
+    std::size_t bytes;
+  };
+  const std::size_t ra = last(a.rp, a.nsb), rb = last(b.rp, b.nsb), ca = last(a.cp, a.nsb), cb = last(b.cp, b.nsb);
+  const std::size_t raw = last(a.rp_w, a.nsb_w), rbw = last(b.rp_w, b.nsb_w);
+  const std::size_t caw = last(a.cp_w, a.nsb_w), cbw = last(b.cp_w, b.nsb_w);
+  if (ra != rb || ca != cb || raw != rbw || caw != cbw) return "run or chunk counts";
+  const std::size_t ns1 = static_cast<std::size_t>(a.nsb) + 1, nw1 = static_cast<std::size_t>(a.nsb_w) + 1;
+  const std::size_t nb1 = static_cast<std::size_t>(a.nbins) + 1;
+  const Item items[] = {
+      {"sp", a.sp, b.sp, ns1 * i4},           {"fkey", a.fkey, b.fkey, nf * u2},
+      {"rp", a.rp, b.rp, ns1 * i4},           {"rdelta", a.rdelta, b.rdelta, ra * i4},
+      {"cbase", a.cbase, b.cbase, ca * i4},   {"cp", a.cp, b.cp, ns1 * i4},
+      {"sp_w", a.sp_w, b.sp_w, nw1 * i4},     {"fkey_w", a.fkey_w, b.fkey_w, nf * u2},
+      {"rp_w", a.rp_w, b.rp_w, nw1 * i4},     {"rdelta_w", a.rdelta_w, b.rdelta_w, raw * i4},
+      {"cbase_w", a.cbase_w, b.cbase_w, caw * i4}, {"cp_w", a.cp_w, b.cp_w, nw1 * i4},
+      {"frow", a.frow, b.frow, nf * u2},      {"bp", a.bp, b.bp, nb1 * i4},
+      {"lp", a.lp, b.lp, nb1 * i4},           {"lkey", a.lkey, b.lkey, nl * u2},
+      {"lbase", a.lbase, b.lbase, nl / kPbLocalGroup * u2},
+      {"far_src", a.far_src, b.far_src, nf * i4}, {"far_src_w", a.far_src_w, b.far_src_w, nf * i4},
+      {"loc_src", a.loc_src, b.loc_src, nl * i4}};
+  for (const Item &it : items)
+    if (dl(it.x, it.bytes) != dl(it.y, it.bytes)) return it.name;
+  return "";
+}
+
 const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt) {
   if (m.format() != cpp::FORMAT_CSR) bad("the propagation-blocked layout needs a CSR matrix");
   const auto &c = dynamic_cast<const cpp::MatrixCSR &>(m);
+  const int n = c.get_n_col();
+  if (c.get_n_row() != n || static_cast<int>(c.get_rowptr().size()) != n + 1)
+    bad("the propagation-blocked layout needs a square matrix in one block");
+  // built from the FP64 mirror (taken before the cache lock: mirror() locks it)
+  const DeviceMatrix &d64 = mirror(m, Dtype::F64);
   DeviceCache &cache = m.device_cache();
   std::lock_guard<std::mutex> g(cache.mu);
-  if (!cache.pb) {
-    const auto &rp = c.get_rowptr();
-    const auto &ci = c.get_colidx();
-    const int n = c.get_n_col();
-    if (c.get_n_row() != n || static_cast<int>(rp.size()) != n + 1) bad("the propagation-blocked layout needs a square matrix in one block");
-    const long r0 = rp[0];
-    const long nnz = static_cast<long>(rp.back()) - r0;
-    if (static_cast<long>(ci.size()) < r0 + nnz) bad("CSR rowptr points past colidx");
-    auto pb = std::unique_ptr<PbMatrix>(new PbMatrix());
-    pb->n = n;
-    pb->nbins = (n + kPbRows - 1) / kPbRows;
-    pb->nsb = (n + kPbCols - 1) / kPbCols;
-    const int nb = pb->nbins, ns = pb->nsb;
-    // pass 1: counts per bin (far; local groups) and per column (far).  The
-    // local groups of a bin are formed in row order: an entry opens a new
-    // group when the current one is full or its row is kPbLocalPad or more
-    // rows past the group's base row
-    std::vector<int> colcnt(static_cast<std::size_t>(n) + 1, 0), bincnt(static_cast<std::size_t>(nb) + 1, 0),
-        loccnt(static_cast<std::size_t>(nb) + 1, 0);
-    int gfill = kPbLocalGroup, gbase = 0;  // the current bin's open group
-    for (int r = 0; r < n; r++) {
-      const int b = r >> kPbShift;
-      if ((r & (kPbRows - 1)) == 0) gfill = kPbLocalGroup;  // a new bin
-      for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
-        const int col = ci[static_cast<std::size_t>(r0 + e)] - c.get_f_col();
-        if (col < 0 || col >= n) bad("CSR column index outside the matrix");
-        if ((col >> kPbShift) == b) {
-          if (gfill == kPbLocalGroup || r - gbase >= kPbLocalPad) {
-            loccnt[static_cast<std::size_t>(b) + 1] += kPbLocalGroup;
-            gfill = 0;
-            gbase = r;
-          }
-          gfill++;
-        } else {
-          colcnt[static_cast<std::size_t>(col) + 1]++;
-          bincnt[static_cast<std::size_t>(b) + 1]++;
-        }
-      }
-    }
-    for (int j = 0; j < n; j++) colcnt[static_cast<std::size_t>(j) + 1] += colcnt[static_cast<std::size_t>(j)];
-    for (int b = 0; b < nb; b++) {
-      bincnt[static_cast<std::size_t>(b) + 1] += bincnt[static_cast<std::size_t>(b)];
-      loccnt[static_cast<std::size_t>(b) + 1] += loccnt[static_cast<std::size_t>(b)];
-    }
-    pb->n_far = colcnt[static_cast<std::size_t>(n)];
-    pb->n_loc = loccnt[static_cast<std::size_t>(nb)];
-    if (pb->n_far >= (1L << 31) || pb->n_loc >= (1L << 31)) bad("propagation-blocked layout limited to 2^31 entries");
-    const std::size_t nf = static_cast<std::size_t>(pb->n_far), nl = static_cast<std::size_t>(pb->n_loc);
-    // pass 2: far entries in column order (rows ascending), local entries
-    // bin-grouped (the groups of pass 1; unused slots stay padding)
-    std::vector<int> csc_src(nf), csc_row(nf);
-    pb->loc_src.assign(nl, -1);
-    std::vector<unsigned short> lkey(nl, static_cast<unsigned short>(kPbLocalPad << kPbShift));
-    std::vector<unsigned short> lbase(nl / kPbLocalGroup, 0);
-    std::vector<int> ccur(colcnt.begin(), colcnt.end() - 1);
-    long lnext = 0;  // first slot of the next local group
-    int lslot = 0;   // next slot in the open group (kPbLocalGroup: none open)
-    gfill = kPbLocalGroup;
-    for (int r = 0; r < n; r++) {
-      const int b = r >> kPbShift;
-      if ((r & (kPbRows - 1)) == 0) {
-        gfill = kPbLocalGroup;
-        lnext = loccnt[static_cast<std::size_t>(b)];
-      }
-      for (long e = rp[static_cast<std::size_t>(r)] - r0; e < rp[static_cast<std::size_t>(r) + 1] - r0; e++) {
-        const int col = ci[static_cast<std::size_t>(r0 + e)] - c.get_f_col();
-        if ((col >> kPbShift) == b) {
-          if (gfill == kPbLocalGroup || r - gbase >= kPbLocalPad) {
-            lslot = static_cast<int>(lnext);
-            lnext += kPbLocalGroup;
-            gfill = 0;
-            gbase = r;
-            lbase[static_cast<std::size_t>(lslot / kPbLocalGroup)] = static_cast<unsigned short>(r & (kPbRows - 1));
-          }
-          const std::size_t k = static_cast<std::size_t>(lslot + gfill);
-          gfill++;
-          pb->loc_src[k] = static_cast<int>(e);
-          lkey[k] = static_cast<unsigned short>((col & (kPbRows - 1)) | ((r - gbase) << kPbShift));
-        } else {
-          const int k = ccur[static_cast<std::size_t>(col)]++;
-          csc_src[static_cast<std::size_t>(k)] = static_cast<int>(e);
-          csc_row[static_cast<std::size_t>(k)] = r;
-        }
-      }
-    }
-    // bin-grouped slot of every far entry: bins in order, each in column order
-    std::vector<int> slot_csc(nf), bcur(bincnt.begin(), bincnt.end() - 1);
-    std::vector<unsigned short> frow(nf);
-    for (std::size_t k = 0; k < nf; k++) {
-      const int r = csc_row[k];
-      const int sl = bcur[static_cast<std::size_t>(r >> kPbShift)]++;
-      slot_csc[k] = sl;
-      frow[static_cast<std::size_t>(sl)] = static_cast<unsigned short>(r & (kPbRows - 1));
-    }
-    // scatter order over superblocks of `cols` columns: superblocks in
-    // order, within a superblock by (bin, column).  Per superblock, the runs
-    // of entries bound for one bin, in bin order: entry o of a run lands in
-    // slot o + delta, with delta the run's first slot minus its first entry.
-    constexpr int kNoDelta = std::numeric_limits<int>::min();
-    auto scatter_side = [&](int cols, int &nsb_out, std::vector<int> &far_src, DevBuf &sp_d, DevBuf &fkey_d,
-                            DevBuf &rp_d, DevBuf &rdelta_d, DevBuf &cbase_d, DevBuf &cp_d) {
-      const int ns2 = (n + cols - 1) / cols;
-      nsb_out = ns2;
-      std::vector<int> spc(static_cast<std::size_t>(ns2) + 1, 0);
-      for (int s2 = 0; s2 < ns2; s2++)
-        spc[static_cast<std::size_t>(s2) + 1] =
-            colcnt[static_cast<std::size_t>(std::min(static_cast<long>(n), static_cast<long>(s2 + 1) * cols))];
-      far_src.resize(nf);
-      std::vector<unsigned short> fkey(nf);
-      std::vector<int> runp(static_cast<std::size_t>(ns2) + 1, 0), rdelta;
-      std::vector<int> chp(static_cast<std::size_t>(ns2) + 1, 0), cbase;
-      std::vector<int> cnt(static_cast<std::size_t>(nb) + 1), erun, run_of(static_cast<std::size_t>(nb));
-      for (int s2 = 0; s2 < ns2; s2++) {
-        const int k0 = spc[static_cast<std::size_t>(s2)], k1 = spc[static_cast<std::size_t>(s2) + 1];
-        std::fill(cnt.begin(), cnt.end(), 0);
-        for (int k = k0; k < k1; k++) cnt[static_cast<std::size_t>(csc_row[static_cast<std::size_t>(k)] >> kPbShift) + 1]++;
-        // run index of each non-empty bin, and its slot delta (the run's
-        // entries are consecutive in CSC order within the bin's slots)
-        std::fill(run_of.begin(), run_of.end(), -1);
-        int nr = 0;
-        for (int b = 0; b < nb; b++)
-          if (cnt[static_cast<std::size_t>(b) + 1] > 0) run_of[static_cast<std::size_t>(b)] = nr++;
-        for (int b = 0; b < nb; b++) cnt[static_cast<std::size_t>(b) + 1] += cnt[static_cast<std::size_t>(b)];
-        const std::vector<int> first(cnt);  // first entry of each bin's run, from k0
-        const std::size_t rbase = rdelta.size();
-        rdelta.resize(rbase + static_cast<std::size_t>(nr), kNoDelta);
-        erun.assign(static_cast<std::size_t>(k1 - k0), -1);
-        for (int k = k0; k < k1; k++) {  // column order, so each bin's run stays in column order
-          const int b = csc_row[static_cast<std::size_t>(k)] >> kPbShift;
-          const int o = k0 + cnt[static_cast<std::size_t>(b)]++;
-          const int r = run_of[static_cast<std::size_t>(b)];
-          far_src[static_cast<std::size_t>(o)] = csc_src[static_cast<std::size_t>(k)];
-          const int col = (ci[static_cast<std::size_t>(r0 + csc_src[static_cast<std::size_t>(k)])] - c.get_f_col()) &
-                          (cols - 1);
-          const bool starts = o - k0 == first[static_cast<std::size_t>(b)];
-          fkey[static_cast<std::size_t>(o)] = static_cast<unsigned short>(col | (starts ? kPbRunStart : 0u));
-          erun[static_cast<std::size_t>(o - k0)] = r;
-          // a bin's slots follow CSC order and a run is one superblock's
-          // column range of that order, so slot - o is the same for the
-          // whole run
-          const int d = slot_csc[static_cast<std::size_t>(k)] - o;
-          int &rd = rdelta[rbase + static_cast<std::size_t>(r)];
-          if (rd != kNoDelta && rd != d) bad("propagation-blocked layout: run slots are not consecutive");
-          rd = d;
-        }
-        runp[static_cast<std::size_t>(s2) + 1] = static_cast<int>(rdelta.size());
-        // run of the first entry of each chunk, checked against the run
-        // every entry gets from it and the run-start flags
-        const int nck = (k1 - k0 + kPbChunk - 1) / kPbChunk;
-        for (int j = 0; j < nck; j++) {
-          const int e0 = j * kPbChunk, e1 = std::min(k1 - k0, e0 + kPbChunk);
-          int run = erun[static_cast<std::size_t>(e0)];
-          cbase.push_back(run);
-          for (int e = e0; e < e1; e++) {
-            if (e > e0 && (fkey[static_cast<std::size_t>(k0 + e)] & kPbRunStart)) run++;
-            if (run != erun[static_cast<std::size_t>(e)]) bad("propagation-blocked layout: run starts do not give the runs");
-          }
-        }
-        chp[static_cast<std::size_t>(s2) + 1] = chp[static_cast<std::size_t>(s2)] + nck;
-      }
-      sp_d.upload(spc.data(), spc.size() * sizeof(int));
-      fkey_d.upload(fkey.data(), fkey.size() * sizeof(unsigned short));
-      rp_d.upload(runp.data(), runp.size() * sizeof(int));
-      if (rdelta.empty()) rdelta.push_back(0);
-      rdelta_d.upload(rdelta.data(), rdelta.size() * sizeof(int));
-      if (cbase.empty()) cbase.push_back(0);
-      cbase_d.upload(cbase.data(), cbase.size() * sizeof(int));
-      cp_d.upload(chp.data(), chp.size() * sizeof(int));
-    };
-    int ns_check = 0;
-    scatter_side(kPbCols, ns_check, pb->far_src, pb->sp, pb->fkey, pb->rp, pb->rdelta, pb->cbase, pb->cp);
-    if (ns_check != ns) bad("propagation-blocked layout: superblock count");
-    scatter_side(kPbColsWide, pb->nsb_w, pb->far_src_w, pb->sp_w, pb->fkey_w, pb->rp_w, pb->rdelta_w, pb->cbase_w,
-                 pb->cp_w);
-    pb->frow.upload(frow.data(), frow.size() * sizeof(unsigned short));
-    pb->bp.upload(bincnt.data(), bincnt.size() * sizeof(int));
-    pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));
-    pb->lkey.upload(lkey.data(), lkey.size() * sizeof(unsigned short));
-    pb->lbase.upload(lbase.data(), lbase.size() * sizeof(unsigned short));
-    pb->prod.alloc(sizeof(double) * std::max<std::size_t>(1, nf));
-    cache.pb = std::move(pb);
-  }
+  if (!cache.pb) cache.pb = pb_build_gpu(d64, n);
   PbMatrix &pb = *cache.pb;
-  if (vdt != Dtype::PATTERN && !pb.fval[static_cast<int>(vdt)].get()) {
-    const auto &va = c.get_values();
-    const long r0 = c.get_rowptr()[0];
-    if (static_cast<long>(va.size()) < static_cast<long>(c.get_rowptr().back())) bad("CSR rowptr points past values");
-    // far values in the scatter order that reads them: F64 (anchor and FP64
-    // products) and F32 (delta products) narrow, BF16 (delta products) wide
-    const std::vector<int> &fsrc = vdt == Dtype::BF16 ? pb.far_src_w : pb.far_src;
-    std::vector<double> fv(fsrc.size()), lv(pb.loc_src.size());
-    for (std::size_t k = 0; k < fv.size(); k++) fv[k] = va[static_cast<std::size_t>(r0 + fsrc[k])];
-    for (std::size_t k = 0; k < lv.size(); k++)
-      lv[k] = pb.loc_src[k] >= 0 ? va[static_cast<std::size_t>(r0 + pb.loc_src[k])] : 0.0;  // padding: 0
-    upload_values(pb.fval[static_cast<int>(vdt)], fv.data(), fv.size(), vdt);
-    upload_values(pb.lval[static_cast<int>(vdt)], lv.data(), lv.size(), vdt);
-    if (fv.empty()) pb.fval[static_cast<int>(vdt)].alloc(8);
-    if (lv.empty()) pb.lval[static_cast<int>(vdt)].alloc(8);
-  }
-  TBSLA_HIP_CHECK(hipDeviceSynchronize());
+  if (vdt != Dtype::PATTERN && !pb.fval[static_cast<int>(vdt)].get()) pb_build_values(pb, d64, vdt);
   return pb;
 }
 

@@ -23,8 +23,10 @@
 #pragma once
 
 #include <tbsla/cpp/Matrix.hpp>
+#include <tbsla/cpp/MatrixCSR.hpp>
 
 #include <memory>
+#include <string>
 #include <mutex>
 #include <vector>
 
@@ -125,9 +127,10 @@ struct PbMatrix {
   // value-free mixed runs.
   int nsb_w = 0;
   DevBuf sp_w, fkey_w, rp_w, rdelta_w, cbase_w, cp_w;
-  std::vector<int> far_src_w;
   DevBuf prod;              // 8-byte product slots, n_far
-  std::vector<int> far_src, loc_src;  // CSR entry of each far (scatter order) / local entry (-1: padding)
+  // CSR entry of each far entry (narrow and wide scatter order) and of each
+  // local slot (-1: padding), int, on the device (the value builds read them)
+  DevBuf far_src, far_src_w, loc_src;
 };
 
 struct DeviceCache {
@@ -136,6 +139,9 @@ struct DeviceCache {
   std::unique_ptr<DeviceMatrix> transposed;
   std::unique_ptr<PbMatrix> pb;
   DevBuf x_stage, y_stage;
+  // count-stochastic CSR: the value of each column (fl(1/count)) in FP64 and
+  // FP32, 0 for a column with no entries (built by the PageRank engine)
+  DevBuf colscale64, colscale32;
 };
 
 // Build (or fetch) the mirror of `m` with values of type vdt.
@@ -150,8 +156,19 @@ const DeviceMatrix *mirror_if_built(const cpp::Matrix &m, Dtype vdt);
 const DeviceMatrix &transpose_mirror(const cpp::Matrix &m);
 
 // The propagation-blocked layout of a square single-block CSR matrix, with
-// values of type vdt (PATTERN: indices only).
+// values of type vdt (PATTERN: indices only), built on the GPU once per
+// matrix (values once per type).
 const PbMatrix &pb_mirror(const cpp::Matrix &m, Dtype vdt);
+
+// Builds of the layout's index arrays (no values): on the GPU from the FP64
+// mirror d64 of an n x n CSR matrix (device sorts), and on the host from
+// the matrix's arrays; both give the same arrays.
+std::unique_ptr<PbMatrix> pb_build_gpu(const DeviceMatrix &d64, int n);
+std::unique_ptr<PbMatrix> pb_build_host(const cpp::MatrixCSR &c);
+// The far and local values of type vdt, gathered on the GPU from d64.
+void pb_build_values(PbMatrix &pb, const DeviceMatrix &d64, Dtype vdt);
+// "" when the index arrays of a and b are equal, else the first that differs.
+std::string pb_layout_compare(const PbMatrix &a, const PbMatrix &b);
 
 // Host tile tables (exposed for tests and the autotuner).
 // tile_row of the CSR entry tiles (ntiles + 1 entries, ntiles = nnz / T + 1).

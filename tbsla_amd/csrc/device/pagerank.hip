@@ -418,6 +418,21 @@ __global__ __launch_bounds__(256) void pr_scale_kernel(const double *__restrict_
     out[i] = (d ? a[i] + static_cast<double>(d[i]) : a[i]) / S;
 }
 
+// cs[colidx[k] - f_col] = vals[k] over the stored entries: for a
+// count-stochastic matrix every entry of a column holds the same value, so
+// the order of the writes does not matter.
+__global__ __launch_bounds__(256) void colscale_kernel(const int *__restrict__ colidx, const double *__restrict__ vals,
+                                                       long nnz, int f_col, long n, double *__restrict__ cs) {
+  for (long k = blockIdx.x * 256L + threadIdx.x; k < nnz; k += static_cast<long>(gridDim.x) * 256) {
+    const long c = static_cast<long>(colidx[k]) - f_col;
+    if (c >= 0 && c < n) cs[c] = vals[k];
+  }
+}
+__global__ __launch_bounds__(256) void to_f32(const double *__restrict__ a, float *__restrict__ b, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256)
+    b[i] = static_cast<float>(a[i]);
+}
+
 __global__ __launch_bounds__(256) void fill_f64(double *y, long n, double v) {
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256) y[i] = v;
 }
@@ -1172,7 +1187,7 @@ struct Workspace {
 
 struct Scratch {
   WsBuf st, slots, perr, ps, pc, total, member, nodes, hsum, hcomp;
-  WsBuf X0, X1, Z, Z1, yhat, xhat, xhat32, D0, D1, colscale, ZD0, ZD1, colscale32, W;
+  WsBuf X0, X1, Z, Z1, yhat, xhat, xhat32, D0, D1, ZD0, ZD1, W;
 };
 
 // zd = fl32(colscale32 * d) of a constant FP32 delta d
@@ -1279,9 +1294,6 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
     throw std::invalid_argument("tbsla: explicit edge values need a CSR matrix and FP64 precision");
 
   const DeviceMatrix &d64 = mirror(m, Dtype::F64);
-  const DeviceMatrix *dlo = nullptr;
-  if (p.precision == PrPrecision::F32) dlo = &mirror(m, Dtype::F32);
-  if (p.precision == PrPrecision::BF16) dlo = &mirror(m, Dtype::BF16);
   const bool fused = csr && !d64.noop && d64.rows == n;
   // propagation-blocked products: explicit choice, or by default on graphs
   // of 2^16 nodes and more, whatever the stopping rule (their row sums are
@@ -1295,6 +1307,11 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
     if (p.precision == PrPrecision::F32) pbm = &pb_mirror(m, Dtype::F32);
     if (p.precision == PrPrecision::BF16) pbm = &pb_mirror(m, Dtype::BF16);
   }
+  // the tile products of a mixed run read the values in the low precision
+  // (the blocked ones read them from the blocked layout)
+  const DeviceMatrix *dlo = nullptr;
+  if (!use_pb && p.precision == PrPrecision::F32) dlo = &mirror(m, Dtype::F32);
+  if (!use_pb && p.precision == PrPrecision::BF16) dlo = &mirror(m, Dtype::BF16);
   if (csr && !d64.noop && d64.rows != n)
     throw std::invalid_argument("tbsla: CSR rowptr does not cover every row");
   if (d64.x_need > n) throw std::invalid_argument("tbsla: column index beyond n_col");
@@ -1342,28 +1359,31 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
     tp.member = w.member.as<unsigned char>();
   }
   const bool pattern = p.precision == PrPrecision::PATTERN;
-  if (pattern || mixed) {
-    // 1/count per column, read back from the stored values (every value of
-    // column j equals fl(1/count_j) after fill_cqmat_stochastic)
-    const auto &csrm = dynamic_cast<const cpp::MatrixCSR &>(m);
-    std::vector<double> cs(static_cast<std::size_t>(n), 0.0);
-    const auto &ci = csrm.get_colidx();
-    const auto &va = csrm.get_values();
-    const auto &rp = csrm.get_rowptr();
-    // the stored entries the products read: [0, rowptr[rows] - rowptr[0])
-    const std::size_t used = rp.size() < 2 ? 0 : static_cast<std::size_t>(rp.back() - rp.front());
-    if (used > ci.size() || used > va.size())
-      throw std::invalid_argument("tbsla: CSR colidx / values shorter than rowptr says");
-    for (std::size_t k = 0; k < used; k++) {
-      const int c = ci[k];
-      if (c < 0 || c >= n) throw std::invalid_argument("tbsla: column index outside [0, n_col)");
-      cs[static_cast<std::size_t>(c)] = va[k];
+  // 1/count per column (the value every entry of column j holds after
+  // fill_cqmat_stochastic), computed from the FP64 mirror on the GPU once
+  // per matrix and kept in its device cache
+  const double *colscale64 = nullptr;
+  const float *colscale32 = nullptr;
+  if (pattern || pat32 || (mixed && m.count_stochastic())) {
+    if (arena) throw std::invalid_argument("tbsla: the allocation-free entry point runs FP64 only");
+    DeviceCache &cache = m.device_cache();
+    std::lock_guard<std::mutex> g(cache.mu);
+    if (!cache.colscale64.get()) {
+      DevBuf c64, c32;
+      c64.alloc(sizeof(double) * static_cast<std::size_t>(n));
+      c32.alloc(sizeof(float) * static_cast<std::size_t>(n));
+      hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, c64.as<double>(), n, 0.0);
+      if (d64.entries > 0)
+        hipLaunchKernelGGL(colscale_kernel, dim3(grid_for(d64.entries)), dim3(256), 0, s, d64.colidx.as<int>(),
+                           d64.vals.as<double>(), d64.entries, d64.f_col, n, c64.as<double>());
+      hipLaunchKernelGGL(to_f32, dim3(gN), dim3(256), 0, s, c64.as<double>(), c32.as<float>(), n);
+      TBSLA_HIP_CHECK(hipGetLastError());
+      TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+      cache.colscale64 = std::move(c64);
+      cache.colscale32 = std::move(c32);
     }
-    ws.upload(w.colscale, cs.data(), cs.size() * sizeof(double), s);
-    if (pat32) {
-      std::vector<float> c32(cs.begin(), cs.end());
-      ws.upload(w.colscale32, c32.data(), c32.size() * sizeof(float), s);
-    }
+    colscale64 = cache.colscale64.as<double>();
+    colscale32 = cache.colscale32.as<float>();
   }
   PrState h{};
   h.beta = p.beta;
@@ -1439,15 +1459,15 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
     ZD[0] = w.ZD0.as<float>();
     ZD[1] = w.ZD1.as<float>();
     if (sn)
-      hipLaunchKernelGGL(scale_into_f32_32, dim3(gN), dim3(256), 0, s, w.colscale32.as<float>(),
+      hipLaunchKernelGGL(scale_into_f32_32, dim3(gN), dim3(256), 0, s, colscale32,
                          static_cast<float>(x0), ZD[0], n);
     else
-      hipLaunchKernelGGL(scale_into_f32, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), b0, ZD[0], n);
+      hipLaunchKernelGGL(scale_into_f32, dim3(gN), dim3(256), 0, s, colscale64, b0, ZD[0], n);
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, ZD[1], n, 0.0f);
   }
   if (pattern || (mixed && m.count_stochastic())) {
     ws.alloc(w.Z, sizeof(double) * n);
-    hipLaunchKernelGGL(scale_into, dim3(gN), dim3(256), 0, s, w.colscale.as<double>(), X[0], w.Z.as<double>(), n);
+    hipLaunchKernelGGL(scale_into, dim3(gN), dim3(256), 0, s, colscale64, X[0], w.Z.as<double>(), n);
   }
   // value-free FP64 sum-normalised runs write the next z beside the next
   // iterate, so z alternates between two buffers like the iterate
@@ -1480,7 +1500,7 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
                        w.pc.as<double>(), static_cast<int>(nb), st);
   };
   double *Z = w.Z.as<double>();
-  const double *colscale = w.colscale.as<double>();
+  const double *colscale = colscale64;
   while (launched < p.max_iteration) {
     const int batch = std::min(next, p.max_iteration - launched);
     for (int k = 0; k < batch; k++) {
@@ -1497,7 +1517,7 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
           b.dold = D[it & 1];
           b.dnew = D[(it + 1) & 1];
           b.zdnew = pat32 ? ZD[(it + 1) & 1] : nullptr;
-          b.colscale32 = w.colscale32.as<float>();
+          b.colscale32 = colscale32;
           if (pat32)
             launch_pb_mx<dev::Pattern, dev::Pattern, float, true>(tmode, *pbm, -1, -1, b, tp, st, sp, s);
           else if (mixed_pattern && p.precision == PrPrecision::F32)
@@ -1541,7 +1561,7 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
           fin_max();
           if (pat32)
             hipLaunchKernelGGL(pr_normalize<3>, dim3(gM), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,
-                               nullptr, nullptr, zdnext, w.colscale32.as<float>(), n, st, w.perr.as<double>(),
+                               nullptr, nullptr, zdnext, colscale32, n, st, w.perr.as<double>(),
                                w.ps.as<double>(), w.pc.as<double>());
           else
             hipLaunchKernelGGL(pr_normalize<1>, dim3(gM), dim3(kTileThreads), 0, s, xout, nullptr, xhat, dcur, dnext,

@@ -236,6 +236,37 @@ def test_blocked_products_small_and_odd_sizes(tb, gpu):
             assert float((a - b).abs().sum()) <= 1e-11, (m, prec, float((a - b).abs().sum()))
 
 
+def test_blocked_layout_gpu_build_matches_host_build(tb, graph_2_20, gpu):
+    """The propagation-blocked layout that page_rank builds on the GPU
+    (device radix sorts and scans) equals, array by array, the layout built
+    by a sequential pass on the host cores: on generated graphs, on 2^20
+    nodes, and on hand-made graphs with empty rows, a long row, a column
+    that every bin points to (runs across many bins), local rows more than
+    seven rows apart (groups ended early), a single node and no entries."""
+    from tbsla_amd._lib import lib
+    rng = np.random.default_rng(11)
+    mats = [tb.MatrixCSR().fill_cqmat_stochastic(3000, 3000, 5, 0.3, 4),
+            tb.MatrixCSR().fill_cqmat_stochastic(8195, 8195, 3, 0.9, 6),
+            tb.MatrixCSR().fill_cqmat_stochastic(40000, 40000, 16, 0.1, 2),
+            graph_2_20[0]]
+    n = 50000
+    lens = rng.integers(0, 9, n)
+    lens[::50] = 0
+    lens[4321] = 30000
+    lens[20000:28192:9] = 1  # one local entry every 9 rows
+    rowptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, n, rowptr[-1]).astype(np.int32)
+    col[col % 97 == 0] = 5  # a heavy column, seen from every bin
+    for r in range(20000, 28192, 9):
+        col[rowptr[r]:rowptr[r + 1]] = r
+    mats.append(tb.MatrixCSR(n, n, rng.random(rowptr[-1]), rowptr, col))
+    mats.append(tb.MatrixCSR(1, 1, np.array([1.0]), np.array([0, 1]), np.array([0])))
+    mats.append(tb.MatrixCSR(70000, 70000, np.ones(3), np.array([0] * 8 + [3] * 69993), np.array([69999, 3, 40000])))
+    mats.append(tb.MatrixCSR(100, 100, np.zeros(0), np.zeros(101), np.zeros(0)))
+    for m in mats:
+        assert lib.tbsla_hip_pb_layout_check(m._h) == 0, (m.get_n_col(), lib.tbsla_last_error())
+
+
 def test_value_free_precisions_need_count_stochastic_values(tb, gpu):
     """PATTERN and PATTERN_F32 read no values, so they are refused for a
     matrix whose values are not 1/count of their column; PATTERN_F32 also
