@@ -4,6 +4,8 @@ Workload (one GPU): rank (0,0)'s block of a 4x4 2D distribution of
 fill_cqmat(2^24, 2^24, c=16, q=0.1, seed 42) -- 4,194,304 x 4,194,304,
 62,075,592 entries -- with values 1 - u01(42, i) and x_j = u01(1, j).
 A step is one FP64 CSR SpMV of that block with the inputs resident in HBM.
+The W warm-up steps follow about 50 ms of untimed steps that bring the GPU's
+clocks up after the host work before the leg; the K timed steps follow.
 
 CSR matrices are generated on the GPU (fill_cqmat(..., device=True)), which
 gives the host generators' arrays; the other formats are generated on the
@@ -53,12 +55,31 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+SETTLE_S = 0.05  # device time of untimed calls that bring the clocks up before the warm-up steps
+
+
 def time_kernel(torch, fn, steps, warmup, batches=1):
     """Average device time per call of fn() over `steps` calls, measured with
     HIP events on torch's current stream (the stream the kernels run on).
     With batches > 1 the calls are split into that many consecutive groups
     with an event between groups, and the per-call time of each group is
-    returned as well: (mean over all steps, [per-group means])."""
+    returned as well: (mean over all steps, [per-group means]).
+
+    Before the `warmup` calls, about SETTLE_S seconds of untimed calls: after
+    the host work that precedes a leg the GPU is idle, and the first calls
+    after an idle gap of 0.5-2 s run 7-10% slower (measured on the CSR block:
+    5050-5200 GB/s in the first four calls, 5600-5650 after), which a few
+    warm-up calls of ~0.15 ms do not cover."""
+    for _ in range(4):
+        fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(4):
+        fn()
+    torch.cuda.synchronize()
+    per = max((time.perf_counter() - t) / 4, 1e-5)
+    for _ in range(min(2000, int(SETTLE_S / per))):
+        fn()
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()

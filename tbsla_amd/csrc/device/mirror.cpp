@@ -512,7 +512,7 @@ std::unique_ptr<PbMatrix> pb_build_host(const cpp::MatrixCSR &c) {
   pb->lp.upload(loccnt.data(), loccnt.size() * sizeof(int));
   pb->lkey.upload(lkey.data(), lkey.size() * sizeof(unsigned short));
   pb->lbase.upload(lbase.data(), lbase.size() * sizeof(unsigned short));
-  pb->prod.alloc(sizeof(double) * std::max<std::size_t>(1, nf));
+  pb->prod.alloc(sizeof(double) * (nf + kPbChunk));  // + spare slots
   pb->far_src.upload(far_src_h.data(), far_src_h.size() * sizeof(int));
   pb->far_src_w.upload(far_src_w_h.data(), far_src_w_h.size() * sizeof(int));
   pb->loc_src.upload(loc_src_h.data(), loc_src_h.size() * sizeof(int));

@@ -127,7 +127,8 @@ struct PbMatrix {
   // value-free mixed runs.
   int nsb_w = 0;
   DevBuf sp_w, fkey_w, rp_w, rdelta_w, cbase_w, cp_w;
-  DevBuf prod;              // 8-byte product slots, n_far
+  DevBuf prod;              // 8-byte product slots, n_far + kPbChunk (spare slots that
+                            // the scatter's lanes past a superblock's end write)
   // CSR entry of each far entry (narrow and wide scatter order) and of each
   // local slot (-1: padding), int, on the device (the value builds read them)
   DevBuf far_src, far_src_w, loc_src;

@@ -512,6 +512,7 @@ struct PbScatterArgs {
   const int *cbase;           // run of the first entry of each 64-entry chunk
   const int *cp;              // chunks of superblock s: [cp[s], cp[s+1])
   int nsb, n;
+  long nfar;                  // product slots; kPbChunk spare slots follow them
 };
 
 // The anchor move of a mixed run, folded into the scatter of the iteration
@@ -591,76 +592,97 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
     }
   }
   __syncthreads();
-  // Each wavefront walks chunks wv + 16u + 64it (u = 0..3) of the superblock,
-  // one entry per lane, as a software pipeline: the run of each chunk's
-  // first entry (cbase) is loaded two iterations ahead; the keys, the values
-  // and the slot offsets of the chunk's runs (rdelta[cb .. cb + 63], one
-  // coalesced load, each lane's offset picked with a lane shuffle) one
-  // iteration ahead, so no load of an iteration waits on another.  An
+  // Each wavefront walks chunks wv + 16u + 16*UC*it (u < UC) of the
+  // superblock, one entry per lane, as a software pipeline: the run of each
+  // chunk's first entry (cbase) is loaded two iterations ahead; the keys,
+  // the values and the slot offsets of the chunk's runs (rdelta[cb .. cb +
+  // 63], one coalesced load, each lane's offset picked with a lane shuffle)
+  // one iteration ahead, so no load of an iteration waits on another.  An
   // entry's run is the chunk's first run plus the run starts on lanes
-  // 1..lane (a ballot); lanes past the end hold no run start.
+  // 1..lane (a ballot).
   static_assert(kPbScatterThreads == 16 * kPbChunk && kPbChunk == 64, "16 wavefronts of 64-entry chunks");
+  // chunks per wavefront per stage (two stages in flight fit 64 VGPRs)
+  constexpr int UC = sizeof(AT) == 8 ? 2 : 4;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nent = k1 - k0, nck = (nent + kPbChunk - 1) / kPbChunk;
   const int nr = a.rp[sb + 1] - a.rp[sb];  // runs of this superblock
-  const int nit = (nck + 63) >> 6;
-  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+  const int nit = (nck + 16 * UC - 1) / (16 * UC);
   struct Stage {
-    unsigned key[4];
-    AT v[4];
-    int dl[4];
+    unsigned key[UC];
+    AT v[UC];
+    int dl[UC];
   };
-  auto load_cb = [&](int it, int (&cb)[4]) {
+  if (nent == 0) return;
+  // Every load is unconditional (indices clamped into the superblock) and
+  // every store too (lanes past the end write spare slots), so the loop is
+  // straight-line code and each wait covers only the loads it needs; the
+  // loop is unrolled twice with the two stages in fixed registers (no
+  // register copies, which would wait for the loads in flight).  On the
+  // 2^24-node graph this took the FP32 delta scatter from 193 to 177 us.
+  auto load_cb = [&](int it, int (&cb)[UC]) {
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int j = wv + 16 * u + 64 * it;
-      cb[u] = j < nck ? cbase[j] : 0;
-    }
+    for (int u = 0; u < UC; u++) cb[u] = cbase[min(wv + 16 * u + 16 * UC * it, nck - 1)];
   };
-  auto load_stage = [&](int it, const int (&cb)[4], Stage &g) {
+  // (lanes past the end keep what they load: their run starts only count
+  // for lanes past the end, and their products go to the spare slots)
+  auto load_stage = [&](int it, const int (&cb)[UC], Stage &g) {
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int j = wv + 16 * u + 64 * it;
-      const int e = j * kPbChunk + lane;
-      const bool ok = e < nent;
-      g.key[u] = ok ? a.fkey[k0 + e] : 0u;
-      g.v[u] = ok ? PbVal<VT>::template prod<AT>(val, k0 + e, AT(1)) : AT(0);
-      g.dl[u] = j < nck ? delta[min(cb[u] + lane, nr - 1)] : 0;
+    for (int u = 0; u < UC; u++) {
+      const int ec = k0 + min((wv + 16 * u + 16 * UC * it) * kPbChunk + lane, nent - 1);
+      g.key[u] = a.fkey[ec];
+      g.v[u] = PbVal<VT>::template prod<AT>(val, ec, AT(1));
+      g.dl[u] = delta[min(cb[u] + lane, nr - 1)];
     }
   };
   auto store = [&](int it, const Stage &g) {
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int e = (wv + 16 * u + 64 * it) * kPbChunk + lane;
+    for (int u = 0; u < UC; u++) {
+      const int e = (wv + 16 * u + 16 * UC * it) * kPbChunk + lane;
       const unsigned long long starts = __ballot((g.key[u] & kPbRunStart) != 0u);
-      const int d = __shfl(g.dl[u], __popcll(starts & upto & ~1ull), 64);
-      if (e < nent) prod[k0 + e + d] = g.v[u] * static_cast<AT>(xs[g.key[u] & static_cast<unsigned>(COLS - 1)]);
+      // run starts on lanes 1..lane: those below this lane (mbcnt) plus its own
+      const unsigned long long s1 = starts & ~1ull;
+      const int below = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+          static_cast<unsigned>(s1 >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(s1), 0u)));
+      const int d = __shfl(g.dl[u], below + static_cast<int>((s1 >> lane) & 1ull), 64);
+      // lanes past the end write a spare slot, so the store is unconditional
+      const long o = e < nent ? static_cast<long>(k0 + e + d) : a.nfar + lane;
+      prod[o] = g.v[u] * static_cast<AT>(xs[g.key[u] & static_cast<unsigned>(COLS - 1)]);
     }
   };
-  int cbA[4], cbB[4];
-  Stage sA, sB;
-  load_cb(0, cbA);
-  load_stage(0, cbA, sA);
-  load_cb(1, cbB);
-  for (int it = 0; it < nit; it++) {
-    int cbC[4];
-    load_stage(it + 1, cbB, sB);  // (chunks past the end load nothing)
-    load_cb(it + 2, cbC);
-    store(it, sA);
-    sA = sB;
-#pragma unroll
-    for (int u = 0; u < 4; u++) cbB[u] = cbC[u];
+  // iteration it: the loads of stage it + 1 (its chunk runs were loaded in
+  // iteration it - 1) and of the chunk runs of it + 2, then the stores of
+  // stage it (loaded in iteration it - 1); chunks past the end load the
+  // superblock's last entry and store to the spare slots
+  int cb0[UC], cb1[UC];
+  Stage s0, s1;
+  load_cb(0, cb0);
+  load_cb(1, cb1);
+  __builtin_amdgcn_sched_barrier(0);  // (the loop's order: chunk runs, then a stage)
+  load_stage(0, cb0, s0);
+  // (both halves of an iteration use what they load: the compiler would
+  // sink a load past a branch into the block that uses it)
+  int it = 0;
+  for (; it + 1 < nit; it += 2) {
+    load_cb(it + 2, cb0);  // (first: the next half waits for it, not for the stage loads)
+    load_stage(it + 1, cb1, s1);
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the stores
+    store(it, s0);
+    load_cb(it + 3, cb1);
+    load_stage(it + 2, cb0, s0);
+    __builtin_amdgcn_sched_barrier(0);
+    store(it + 1, s1);
   }
+  if (it < nit) store(it, s0);
 }
 
 // FP64 runs: the far products of x.
-template <typename VT, typename XT, typename AT>
+template <typename VT, typename XT, typename AT, int COLS>
 __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a, const VT *__restrict__ val,
                                                                 const XT *__restrict__ x, AT *__restrict__ prod,
                                                                 const PrState *st) {
   if (st->done) return;
-  __shared__ XT xs[kPbCols];
-  pb_scatter_sb<VT, XT, AT, false, kPbCols>(a, val, x, prod, xs, AnchorMove{}, nullptr);
+  __shared__ XT xs[COLS];
+  pb_scatter_sb<VT, XT, AT, false, COLS>(a, val, x, prod, xs, AnchorMove{}, nullptr);
 }
 
 // Mixed runs, one launch per iteration: in an iteration that moves the
@@ -984,7 +1006,8 @@ PbView pb_view(const PbMatrix &pb) {
 
 PbScatterArgs pb_args(const PbMatrix &pb) {
   return PbScatterArgs{pb.sp.as<int>(),    pb.fkey.as<unsigned short>(), pb.rp.as<int>(), pb.rdelta.as<int>(),
-                       pb.cbase.as<int>(), pb.cp.as<int>(),              pb.nsb,          pb.n};
+                       pb.cbase.as<int>(), pb.cp.as<int>(),              pb.nsb,          pb.n,
+                       pb.n_far};
 }
 
 // The scatter order of the delta products with values VD (the far values of
@@ -996,10 +1019,34 @@ constexpr bool pb_delta_wide() {
 
 PbScatterArgs pb_args_wide(const PbMatrix &pb) {
   return PbScatterArgs{pb.sp_w.as<int>(),    pb.fkey_w.as<unsigned short>(), pb.rp_w.as<int>(), pb.rdelta_w.as<int>(),
-                       pb.cbase_w.as<int>(), pb.cp_w.as<int>(),              pb.nsb_w,          pb.n};
+                       pb.cbase_w.as<int>(), pb.cp_w.as<int>(),              pb.nsb_w,          pb.n,
+                       pb.n_far};
 }
 
 dim3 pb_scatter_grid(const PbMatrix &pb) { return dim3((pb.nsb + 7) / 8 * 8); }
+dim3 pb_scatter_grid_wide(const PbMatrix &pb) { return dim3((pb.nsb_w + 7) / 8 * 8); }
+
+// Dynamic LDS added to the scatter launches whose static LDS (a 64 KB x
+// slice) would let two blocks share a CU: one scatter block per CU writes
+// its products faster (2^24-node graph: FP64 scatter 345 -> 315 us, and the
+// gather after it 374 -> 362 us; two blocks per CU for the mixed scatters:
+// 199 -> 207 us).  The value-free FP64 scatter takes the wide order, whose
+// 128 KB slice already holds a CU alone (234 -> 210 us; with values, FP64
+// and FP32, the wide order measured slower).
+constexpr unsigned kPbScatterLdsPad = 24 * 1024;
+
+// The far products of one FP64-sum iteration (values VT, x of type XT):
+// value-free ones over the wide order, the others over the narrow one.
+template <typename VT, typename XT, typename AT>
+void launch_pb_scatter(const PbMatrix &pb, const VT *fval, const XT *x, AT *prod, const PrState *st, hipStream_t s) {
+  if (pb.n_far == 0) return;
+  if constexpr (std::is_same<VT, dev::Pattern>::value && sizeof(XT) == 8)
+    hipLaunchKernelGGL((pb_scatter<VT, XT, AT, kPbColsWide>), pb_scatter_grid_wide(pb), dim3(kPbScatterThreads), 0, s,
+                       pb_args_wide(pb), fval, x, prod, st);
+  else
+    hipLaunchKernelGGL((pb_scatter<VT, XT, AT, kPbCols>), pb_scatter_grid(pb), dim3(kPbScatterThreads),
+                       kPbScatterLdsPad, s, pb_args(pb), fval, x, prod, st);
+}
 
 // One propagation-blocked product with the reference-rule update (FP64
 // runs): values of type VT from `pb` (dev::Pattern: value-free), x of type
@@ -1010,9 +1057,7 @@ void launch_pb(const PbMatrix &pb, int vslot, const XT *x, const Teleport &tp, d
   const VT *fval = vslot >= 0 ? pb.fval[vslot].as<VT>() : nullptr;
   const VT *lval = vslot >= 0 ? pb.lval[vslot].as<VT>() : nullptr;
   AT *prod = pb.prod.as<AT>();
-  if (pb.n_far > 0)
-    hipLaunchKernelGGL((pb_scatter<VT, XT, AT>), pb_scatter_grid(pb), dim3(kPbScatterThreads), 0, s, pb_args(pb), fval,
-                       x, prod, st);
+  launch_pb_scatter<VT, XT, AT>(pb, fval, x, prod, st, s);
   hipLaunchKernelGGL((pb_gather<VT, XT, AT, TM>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval, x, prod, tp,
                      out, st, slots);
 }
@@ -1043,9 +1088,7 @@ void launch_pb_sn_tm(const PbMatrix &pb, int vslot, const double *xin, const dou
   const VT *lval = vslot >= 0 ? pb.lval[vslot].as<VT>() : nullptr;
   const double *x = PAT ? z : xin;
   double *prod = pb.prod.as<double>();
-  if (pb.n_far > 0)
-    hipLaunchKernelGGL((pb_scatter<VT, double, double>), pb_scatter_grid(pb), dim3(kPbScatterThreads), 0, s,
-                       pb_args(pb), fval, x, prod, st);
+  launch_pb_scatter<VT, double, double>(pb, fval, x, prod, st, s);
   hipLaunchKernelGGL((pb_gather_sn<VT, double, double, TM, PAT>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), lval,
                      x, prod, tp, xin, xout, zout, colscale, st, sp.perr, sp.ps, sp.pc);
 }
@@ -1089,8 +1132,8 @@ void launch_pb_mx_tm(const PbMatrix &pb, int va, int vd, const MxBufs &b, const 
   // scatter 154 -> 140 us wide, BF16 196 -> 192, FP32 191 -> 196)
   constexpr bool wide = pb_delta_wide<VD>();
   hipLaunchKernelGGL((pb_scatter_mx<VA, VD, XD, float, wide ? kPbColsWide : kPbCols>), pb_scatter_grid(pb),
-                     dim3(kPbScatterThreads), 0, s, pb_args(pb), wide ? pb_args_wide(pb) : pb_args(pb), fa, fd, xd,
-                     pb.prod.get(), b.am, st);
+                     dim3(kPbScatterThreads), kPbScatterLdsPad, s, pb_args(pb), wide ? pb_args_wide(pb) : pb_args(pb),
+                     fa, fd, xd, pb.prod.get(), b.am, st);
   hipLaunchKernelGGL((pb_gather_mx<VA, VD, XD, float, TM, PAT32>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), la,
                      xa, ld, xd, pb.prod.get(), tp, b.am.xhat, b.am.xhat32, b.w, b.dold, b.dnew, b.zdnew, b.colscale32, st,
                      sp.perr, sp.ps, sp.pc);

@@ -469,7 +469,7 @@ std::unique_ptr<PbMatrix> pb_build_gpu(const DeviceMatrix &d, int n) {
                        lcol.as<int>(), le.as<int>(), pb->lp.as<int>(), nb, pb->lkey.as<unsigned short>(),
                        pb->lbase.as<unsigned short>(), pb->loc_src.as<int>());
   }
-  pb->prod.alloc(sizeof(double) * static_cast<std::size_t>(std::max(1L, nf)));
+  pb->prod.alloc(sizeof(double) * (static_cast<std::size_t>(nf) + kPbChunk));  // + spare slots
   TBSLA_HIP_CHECK(hipGetLastError());
   TBSLA_HIP_CHECK(hipDeviceSynchronize());
   return pb;
