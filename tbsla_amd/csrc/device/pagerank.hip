@@ -322,6 +322,31 @@ __global__ __launch_bounds__(64) void pr_finalize_max(unsigned long long *slots,
   if (threadIdx.x == 0) st->maxv = ord_decode(u);
 }
 
+// Sum of the per-block partials (err, compensated sum) over nb blocks by one
+// kTileThreads block, in the fixed order thread t: t, t + kTileThreads, ...
+// The loads of 8 partials per thread are issued together (a loop that loads
+// one partial per step waits for each load in turn).
+__device__ inline void sum_partials(const double *__restrict__ perr, const double *__restrict__ ps,
+                                    const double *__restrict__ pc, int nb, double &e, CSum &t) {
+  constexpr int U = 8;
+  for (int i0 = threadIdx.x; i0 < nb; i0 += U * kTileThreads) {
+    double pe[U], pv[U], pw[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = min(i0 + u * kTileThreads, nb - 1);
+      pe[u] = perr ? perr[i] : 0.0;
+      pv[u] = ps[i];
+      pw[u] = pc[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i0 + u * kTileThreads < nb) {
+        e += pe[u];
+        t = csum_merge(t, CSum{pv[u], pw[u]});
+      }
+  }
+}
+
 // One block: residual and compensated sum of the normalise pass, then the
 // step decision.
 __global__ __launch_bounds__(kTileThreads) void pr_finalize_norm(const double *__restrict__ perr,
@@ -331,10 +356,7 @@ __global__ __launch_bounds__(kTileThreads) void pr_finalize_norm(const double *_
   __shared__ double red_s[kTileThreads], red_c[kTileThreads];
   double e = 0;
   CSum t{0, 0};
-  for (int i = threadIdx.x; i < nb; i += kTileThreads) {
-    e += perr[i];
-    t = csum_merge(t, CSum{ps[i], pc[i]});
-  }
+  sum_partials(perr, ps, pc, nb, e, t);
   e = dev::block_tree_sum<double>(e, red_s);
   t = block_csum(t, red_s, red_c);
   if (threadIdx.x == 0) finalize_step(st, e, t.s + t.c);
@@ -1255,16 +1277,14 @@ __global__ __launch_bounds__(kTileThreads) void pr_finalize_sn(const double *__r
   __shared__ double red_s[kTileThreads], red_c[kTileThreads];
   double e = 0;
   CSum t{0, 0};
-  for (int i = threadIdx.x; i < nb; i += kTileThreads) {
-    e += perr[i];
-    t = csum_merge(t, CSum{ps[i], pc[i]});
-  }
+  sum_partials(perr, ps, pc, nb, e, t);
   e = dev::block_tree_sum<double>(e, red_s);
   t = block_csum(t, red_s, red_c);
   const bool moved = st->mixed && st->need_anchor;
   CSum h{0, 0};
   if (moved) {
-    for (int i = threadIdx.x; i < nsb; i += kTileThreads) h = csum_merge(h, CSum{hs[i], hc[i]});
+    double unused = 0;
+    sum_partials(nullptr, hs, hc, nsb, unused, h);
     h = block_csum(h, red_s, red_c);
   }
   if (threadIdx.x == 0) {

@@ -8,10 +8,12 @@
 //                 gives the bin-grouped slots, and stable sorts by
 //                 (superblock, bin) give the two scatter orders, each with
 //                 its run starts, run offsets (slot - entry) and chunk runs.
-//   local entries compacted in CSR order (row order), then one thread per
-//                 bin forms the bin's groups of four (a group ends early
-//                 where the next row is kPbLocalPad or more rows past its
-//                 base) -- a sequential rule, cheap per bin.
+//   local entries compacted in CSR order (row order), then one wavefront
+//                 per bin follows the bin's chain of groups of four (a group
+//                 ends early where the next row is kPbLocalPad or more rows
+//                 past its base), and one thread per entry places it.
+// Range boundaries (superblocks, bins) come from binary searches over the
+// sorted keys.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -82,15 +84,31 @@ __global__ __launch_bounds__(kT) void pbb_compact(const int *__restrict__ far, c
   }
 }
 
-// key = bin of the entry's row, val = its CSC position; far count per bin
+// key = bin of the entry's row, val = its CSC position
 __global__ __launch_bounds__(kT) void pbb_bin_keys(const int *__restrict__ csc_e, const int *__restrict__ row_of,
-                                                   long nf, unsigned long long *__restrict__ key, int *__restrict__ val,
-                                                   int *__restrict__ bincnt) {
+                                                   long nf, unsigned long long *__restrict__ key, int *__restrict__ val) {
   for (long k = blockIdx.x * static_cast<long>(kT) + threadIdx.x; k < nf; k += static_cast<long>(gridDim.x) * kT) {
-    const int b = row_of[csc_e[k]] >> kPbShift;
-    key[k] = static_cast<unsigned long long>(b);
+    key[k] = static_cast<unsigned long long>(row_of[csc_e[k]] >> kPbShift);
     val[k] = static_cast<int>(k);
-    atomicAdd(bincnt + b, 1);
+  }
+}
+
+// out[p] = the first k with key[k] >= p << shift (n if none), p = 0..np, for
+// ascending keys: the boundaries of the key ranges (a binary search per p;
+// counting with atomics contends on one address per range)
+__global__ __launch_bounds__(kT) void pbb_bounds(const unsigned long long *__restrict__ key, long n, int np, int shift,
+                                                 int *__restrict__ out) {
+  for (long p = blockIdx.x * static_cast<long>(kT) + threadIdx.x; p <= np; p += static_cast<long>(gridDim.x) * kT) {
+    const unsigned long long target = static_cast<unsigned long long>(p) << shift;
+    long lo = 0, hi = n;
+    while (lo < hi) {
+      const long mid = (lo + hi) >> 1;
+      if (key[mid] < target)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    out[p] = static_cast<int>(lo);
   }
 }
 
@@ -105,17 +123,16 @@ __global__ __launch_bounds__(kT) void pbb_slots(const int *__restrict__ slot_lis
   }
 }
 
-// key = superblock << binbits | bin, val = CSC position; far count per superblock
+// key = superblock << binbits | bin, val = CSC position
 __global__ __launch_bounds__(kT) void pbb_sc_keys(const unsigned long long *__restrict__ csc_col,
                                                   const int *__restrict__ csc_e, const int *__restrict__ row_of, long nf,
                                                   int colshift, int binbits, unsigned long long *__restrict__ key,
-                                                  int *__restrict__ val, int *__restrict__ sbcnt) {
+                                                  int *__restrict__ val) {
   for (long k = blockIdx.x * static_cast<long>(kT) + threadIdx.x; k < nf; k += static_cast<long>(gridDim.x) * kT) {
     const int sb = static_cast<int>(csc_col[k] >> colshift);
     const int b = row_of[csc_e[k]] >> kPbShift;
     key[k] = (static_cast<unsigned long long>(sb) << binbits) | static_cast<unsigned long long>(b);
     val[k] = static_cast<int>(k);
-    atomicAdd(sbcnt + sb, 1);
   }
 }
 
@@ -187,45 +204,61 @@ __global__ __launch_bounds__(kT) void pbb_local_begin(const int *__restrict__ ro
     lbeg[b] = v;
   }
 }
-// groups of each bin: an entry opens a group when the open one is full or
-// its row is kPbLocalPad or more rows past the group's base
-__global__ __launch_bounds__(kT) void pbb_local_groups(const int *__restrict__ lbeg, const int *__restrict__ lrow, int nb,
-                                                       int *__restrict__ ngroups) {
-  for (long b = blockIdx.x * static_cast<long>(kT) + threadIdx.x; b < nb; b += static_cast<long>(gridDim.x) * kT) {
-    int g = 0, fill = kPbLocalGroup, base = 0;
-    for (int li = lbeg[b]; li < lbeg[b + 1]; li++) {
-      const int r = lrow[li];
-      if (fill == kPbLocalGroup || r - base >= kPbLocalPad) {
-        g++;
-        fill = 0;
-        base = r;
-      }
-      fill++;
+// Groups of each bin, one wavefront per bin: an entry opens a group when the
+// open one is full or its row is kPbLocalPad or more rows past the group's
+// base.  So a group that opens at entry s ends before the first entry j > s
+// with j = s + kPbLocalGroup or row(j) - row(s) >= kPbLocalPad: each lane
+// finds that end for a group opening at its own entry, and the wavefront
+// follows the chain of groups through each 64-entry window (a lane shuffle
+// per group).  rel[li] = the slot of local entry li relative to its bin's
+// first slot (kPbLocalGroup * group + position in the group).
+__global__ __launch_bounds__(kT) void pbb_local_walk(const int *__restrict__ lbeg, const int *__restrict__ lrow, int nb,
+                                                     int *__restrict__ rel, int *__restrict__ ngroups) {
+  const long wave = (blockIdx.x * static_cast<long>(kT) + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= nb) return;  // (uniform over the wavefront)
+  const int b = static_cast<int>(wave);
+  const int l0 = lbeg[b], l1 = lbeg[b + 1];
+  int g = 0;              // groups opened so far
+  int s = l0, sp = l0;    // the next group's first entry, the last opened group's
+  for (int w = l0; w < l1; w += 64) {
+    const int i = w + lane;
+    int nx = i + kPbLocalGroup;  // end of a group opening at entry i
+    if (i < l1) {
+      const int r = lrow[i];
+#pragma unroll
+      for (int k = kPbLocalGroup - 1; k >= 1; k--)
+        if (i + k < l1 && lrow[i + k] - r >= kPbLocalPad) nx = i + k;
     }
-    ngroups[b] = g;
+    nx = min(nx, l1);
+    // entries before s in this window belong to the group opened at sp
+    int mine = i < s ? kPbLocalGroup * (g - 1) + (i - sp) : -1;
+    while (s < w + 64 && s < l1) {
+      const int e = __shfl(nx, s - w, 64);
+      if (i >= s && i < e) mine = kPbLocalGroup * g + (i - s);
+      sp = s;
+      s = e;
+      g++;
+    }
+    if (i < l1) rel[i] = mine;
   }
+  if (lane == 0) ngroups[b] = g;
 }
-// the same walk, writing the keys, group bases and CSR entries (padding
-// slots keep their fill)
-__global__ __launch_bounds__(kT) void pbb_local_fill(const int *__restrict__ lbeg, const int *__restrict__ lrow,
-                                                     const int *__restrict__ lcol, const int *__restrict__ le,
-                                                     const int *__restrict__ lp, int nb, unsigned short *__restrict__ lkey,
-                                                     unsigned short *__restrict__ lbase, int *__restrict__ loc_src) {
-  for (long b = blockIdx.x * static_cast<long>(kT) + threadIdx.x; b < nb; b += static_cast<long>(gridDim.x) * kT) {
-    int fill = kPbLocalGroup, base = 0, slot = lp[b] - kPbLocalGroup;
-    for (int li = lbeg[b]; li < lbeg[b + 1]; li++) {
-      const int r = lrow[li];
-      if (fill == kPbLocalGroup || r - base >= kPbLocalPad) {
-        slot += kPbLocalGroup;
-        fill = 0;
-        base = r;
-        lbase[slot / kPbLocalGroup] = static_cast<unsigned short>(r & (kPbRows - 1));
-      }
-      const int k = slot + fill;
-      fill++;
-      loc_src[k] = le[li];
-      lkey[k] = static_cast<unsigned short>(lcol[li] | ((r - base) << kPbShift));
-    }
+// Local entry li into its slot: key, CSR entry, and the group's base row at
+// the group's first entry (padding slots keep their fill)
+__global__ __launch_bounds__(kT) void pbb_local_place(const int *__restrict__ rel, const int *__restrict__ lrow,
+                                                      const int *__restrict__ lcol, const int *__restrict__ le,
+                                                      const int *__restrict__ lp, long nloc,
+                                                      unsigned short *__restrict__ lkey, unsigned short *__restrict__ lbase,
+                                                      int *__restrict__ loc_src) {
+  for (long li = blockIdx.x * static_cast<long>(kT) + threadIdx.x; li < nloc; li += static_cast<long>(gridDim.x) * kT) {
+    const int r = lrow[li], q = rel[li];
+    const int pos = q % kPbLocalGroup;
+    const int base = lrow[li - pos];  // the group's first entry
+    const int k = lp[r >> kPbShift] + q;
+    loc_src[k] = le[li];
+    lkey[k] = static_cast<unsigned short>(lcol[li] | ((r - base) << kPbShift));
+    if (pos == 0) lbase[k / kPbLocalGroup] = static_cast<unsigned short>(base & (kPbRows - 1));
   }
 }
 
@@ -291,8 +324,8 @@ void scatter_order(int cols, int n, int nb, long nf, const DevBuf &csc_col, cons
   const int ns = (n + cols - 1) / cols;
   nsb = ns;
   const int binbits = bits_for(nb - 1), sbbits = bits_for(ns - 1);
-  DevBuf key, val, k2, v2, sbcnt;
-  sbcnt.alloc(sizeof(int) * (static_cast<std::size_t>(ns) + 1));
+  DevBuf key, val, k2, v2, spb;
+  spb.alloc(sizeof(int) * (static_cast<std::size_t>(ns) + 1));
   fkey_d.alloc(sizeof(unsigned short) * static_cast<std::size_t>(std::max(1L, nf)));
   far_src_d.alloc(sizeof(int) * static_cast<std::size_t>(std::max(1L, nf)));
   DevBuf start, rpos, dd;
@@ -303,9 +336,11 @@ void scatter_order(int cols, int n, int nb, long nf, const DevBuf &csc_col, cons
     v2.alloc(sizeof(int) * static_cast<std::size_t>(nf));
     hipLaunchKernelGGL(pbb_sc_keys, dim3(blocks_for(nf)), dim3(kT), 0, s, csc_col.as<unsigned long long>(),
                        csc_e.as<int>(), row_of.as<int>(), nf, colshift, binbits, key.as<unsigned long long>(),
-                       val.as<int>(), sbcnt.as<int>());
+                       val.as<int>());
     radix_sort_pairs(key.as<unsigned long long>(), val.as<int>(), k2.as<unsigned long long>(), v2.as<int>(), nf,
                      binbits + sbbits, s);
+    hipLaunchKernelGGL(pbb_bounds, dim3(blocks_for(ns + 1L)), dim3(kT), 0, s, key.as<unsigned long long>(), nf, ns,
+                       binbits, spb.as<int>());
     start.alloc(sizeof(int) * static_cast<std::size_t>(nf));
     rpos.alloc(sizeof(int) * static_cast<std::size_t>(nf));
     dd.alloc(sizeof(int) * static_cast<std::size_t>(nf));
@@ -318,13 +353,12 @@ void scatter_order(int cols, int n, int nb, long nf, const DevBuf &csc_col, cons
   }
   const int runs = nf > 0 ? read_int(rpos.as<int>() + nf - 1) + read_int(start.as<int>() + nf - 1) : 0;
   // superblock boundaries, first runs and chunk offsets (small, on the host)
-  std::vector<int> cnt = download<int>(sbcnt, static_cast<std::size_t>(ns));
   std::vector<int> sp(static_cast<std::size_t>(ns) + 1, 0), cp(static_cast<std::size_t>(ns) + 1, 0);
-  for (int b = 0; b < ns; b++) {
-    sp[static_cast<std::size_t>(b) + 1] = sp[static_cast<std::size_t>(b)] + cnt[static_cast<std::size_t>(b)];
+  if (nf > 0) sp = download<int>(spb, static_cast<std::size_t>(ns) + 1);
+  for (int b = 0; b < ns; b++)
     cp[static_cast<std::size_t>(b) + 1] =
-        cp[static_cast<std::size_t>(b)] + (cnt[static_cast<std::size_t>(b)] + kPbChunk - 1) / kPbChunk;
-  }
+        cp[static_cast<std::size_t>(b)] +
+        (sp[static_cast<std::size_t>(b) + 1] - sp[static_cast<std::size_t>(b)] + kPbChunk - 1) / kPbChunk;
   sp_d.upload(sp.data(), sp.size() * sizeof(int));
   cp_d.upload(cp.data(), cp.size() * sizeof(int));
   std::vector<int> rp(static_cast<std::size_t>(ns) + 1, runs);
@@ -405,24 +439,26 @@ std::unique_ptr<PbMatrix> pb_build_gpu(const DeviceMatrix &d, int n) {
                      bits_for(n - 1), s);
   }
   // bin-grouped slots
-  DevBuf bkey, slot_list, bincnt, slot_of;
-  bincnt.alloc(sizeof(int) * (static_cast<std::size_t>(nb) + 1));
+  DevBuf bkey, slot_list, bpb, slot_of;
+  bpb.alloc(sizeof(int) * (static_cast<std::size_t>(nb) + 1));
   pb->frow.alloc(sizeof(unsigned short) * static_cast<std::size_t>(std::max(1L, nf)));
   slot_of.alloc(sizeof(int) * static_cast<std::size_t>(std::max(1L, nf)));
   if (nf > 0) {
     bkey.alloc(sizeof(unsigned long long) * static_cast<std::size_t>(nf));
     slot_list.alloc(sizeof(int) * static_cast<std::size_t>(nf));
     hipLaunchKernelGGL(pbb_bin_keys, dim3(blocks_for(nf)), dim3(kT), 0, s, csc_e.as<int>(), row_of.as<int>(), nf,
-                       bkey.as<unsigned long long>(), slot_list.as<int>(), bincnt.as<int>());
+                       bkey.as<unsigned long long>(), slot_list.as<int>());
     radix_sort_pairs(bkey.as<unsigned long long>(), slot_list.as<int>(), k2.as<unsigned long long>(), v2.as<int>(), nf,
                      bits_for(nb - 1), s);
+    hipLaunchKernelGGL(pbb_bounds, dim3(blocks_for(nb + 1L)), dim3(kT), 0, s, bkey.as<unsigned long long>(), nf, nb, 0,
+                       bpb.as<int>());
     hipLaunchKernelGGL(pbb_slots, dim3(blocks_for(nf)), dim3(kT), 0, s, slot_list.as<int>(), csc_e.as<int>(),
                        row_of.as<int>(), nf, slot_of.as<int>(), pb->frow.as<unsigned short>());
   }
-  {
-    std::vector<int> cnt = download<int>(bincnt, static_cast<std::size_t>(nb));
+  if (nf > 0) {
+    pb->bp = std::move(bpb);
+  } else {
     std::vector<int> bp(static_cast<std::size_t>(nb) + 1, 0);
-    for (int b = 0; b < nb; b++) bp[static_cast<std::size_t>(b) + 1] = bp[static_cast<std::size_t>(b)] + cnt[static_cast<std::size_t>(b)];
     pb->bp.upload(bp.data(), bp.size() * sizeof(int));
   }
   k2.release();
@@ -438,14 +474,15 @@ std::unique_ptr<PbMatrix> pb_build_gpu(const DeviceMatrix &d, int n) {
                 pb->rdelta_w, pb->cbase_w, pb->cp_w, pb->far_src_w, err);
   if (read_int(err.as<int>() + 1)) bad("propagation-blocked layout: run slots are not consecutive");
   // local groups
-  DevBuf lbeg, ngroups;
+  DevBuf lbeg, ngroups, rel;
   lbeg.alloc(sizeof(int) * (static_cast<std::size_t>(nb) + 1));
   ngroups.alloc(sizeof(int) * static_cast<std::size_t>(nb));
+  rel.alloc(sizeof(int) * static_cast<std::size_t>(std::max(1L, nloc)));
   if (nloc > 0) {
     hipLaunchKernelGGL(pbb_local_begin, dim3(blocks_for(nb + 1)), dim3(kT), 0, s, d.rowptr.as<int>(), d.rp0,
                        fpos.as<int>(), nnz, n, nb, static_cast<int>(nloc), lbeg.as<int>());
-    hipLaunchKernelGGL(pbb_local_groups, dim3(blocks_for(nb)), dim3(kT), 0, s, lbeg.as<int>(), lrow.as<int>(), nb,
-                       ngroups.as<int>());
+    hipLaunchKernelGGL(pbb_local_walk, dim3((nb + kT / 64 - 1) / (kT / 64)), dim3(kT), 0, s, lbeg.as<int>(),
+                       lrow.as<int>(), nb, rel.as<int>(), ngroups.as<int>());
   }
   std::vector<int> g = download<int>(ngroups, static_cast<std::size_t>(nb));
   std::vector<int> lp(static_cast<std::size_t>(nb) + 1, 0);
@@ -465,8 +502,8 @@ std::unique_ptr<PbMatrix> pb_build_gpu(const DeviceMatrix &d, int n) {
     hipLaunchKernelGGL(pbb_fill_u16, dim3(blocks_for(nl)), dim3(kT), 0, s, pb->lkey.as<unsigned short>(), nl,
                        static_cast<unsigned short>(kPbLocalPad << kPbShift));
     hipLaunchKernelGGL(pbb_fill_i32, dim3(blocks_for(nl)), dim3(kT), 0, s, pb->loc_src.as<int>(), nl, -1);
-    hipLaunchKernelGGL(pbb_local_fill, dim3(blocks_for(nb)), dim3(kT), 0, s, lbeg.as<int>(), lrow.as<int>(),
-                       lcol.as<int>(), le.as<int>(), pb->lp.as<int>(), nb, pb->lkey.as<unsigned short>(),
+    hipLaunchKernelGGL(pbb_local_place, dim3(blocks_for(nloc)), dim3(kT), 0, s, rel.as<int>(), lrow.as<int>(),
+                       lcol.as<int>(), le.as<int>(), pb->lp.as<int>(), nloc, pb->lkey.as<unsigned short>(),
                        pb->lbase.as<unsigned short>(), pb->loc_src.as<int>());
   }
   pb->prod.alloc(sizeof(double) * (static_cast<std::size_t>(nf) + kPbChunk));  // + spare slots
