@@ -49,8 +49,12 @@ struct PrState {
   // (the next iterate's scale), and for mixed runs the sum of the anchor,
   // the teleport term and the scale of the iteration that moved it
   double inv_sum, sum_hat, t_anchor, c_anchor;
+  // reference-rule runs with propagation-blocked products keep the update's
+  // unnormalised b (raw = 1): the iterate is b * xscale, xscale = 1 / max of
+  // the update that wrote it (1 for the start and in every other run)
+  double xscale;
   int it, max_it, done, rule;
-  int need_anchor, anchors, anchor_it, mixed;
+  int need_anchor, anchors, anchor_it, mixed, raw;
 };
 
 constexpr int kNormBlocks = 1024;
@@ -260,6 +264,8 @@ __device__ inline void finalize_step(PrState *st, double err, double S) {
 }
 
 // MODE 0: FP64, b normalised in place, previous iterate `prev`.
+// MODE 4: MODE 0 with b and prev left unnormalised (raw runs): b * (1/max)
+//         and prev * xscale are the iterates, nothing is written.
 // MODE 1: mixed, previous iterate xhat + dold, writes dnew = b - xhat (FP32).
 // MODE 2: FP64 + z = colscale .* b for the next value-free product.
 // MODE 3: MODE 1 + zd = colscale .* (b - xhat) in FP32 (colscale32) for the
@@ -278,10 +284,32 @@ __global__ __launch_bounds__(kTileThreads) void pr_normalize(double *__restrict_
   if (st->done) return;
   __shared__ double red_s[kTileThreads], red_c[kTileThreads];
   const double m = st->maxv;
+  const double inv_m = 1.0 / m, pscale = st->xscale;  // (MODE 4)
   const bool moved = st->need_anchor != 0;
   double err = 0;
   CSum cs{0, 0};
-  for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
+  if (MODE == 4) {
+    // four elements of the thread's sequence per step, loads first (the
+    // same order of additions as one element per step)
+    const long stride = static_cast<long>(gridDim.x) * kTileThreads;
+    for (long i0 = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i0 < n; i0 += 4 * stride) {
+      double bv[4], pv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const long i = min(i0 + u * stride, n - 1);
+        bv[u] = b[i];
+        pv[u] = prev[i];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (i0 + u * stride < n) {
+          const double bi = bv[u] * inv_m;
+          err += fabs(bi - pv[u] * pscale);
+          csum_add(cs, bi);
+        }
+    }
+  }
+  for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; MODE != 4 && i < n;
        i += static_cast<long>(gridDim.x) * kTileThreads) {
     const double bi = b[i] / m;
     const bool mixed = MODE == 1 || MODE == 3;
@@ -359,7 +387,10 @@ __global__ __launch_bounds__(kTileThreads) void pr_finalize_norm(const double *_
   sum_partials(perr, ps, pc, nb, e, t);
   e = dev::block_tree_sum<double>(e, red_s);
   t = block_csum(t, red_s, red_c);
-  if (threadIdx.x == 0) finalize_step(st, e, t.s + t.c);
+  if (threadIdx.x == 0) {
+    finalize_step(st, e, t.s + t.c);
+    if (st->raw) st->xscale = 1.0 / st->maxv;  // the scale of the b just written
+  }
 }
 
 // Mixed precision with tile products: move the anchor to the current iterate
@@ -405,15 +436,18 @@ __global__ __launch_bounds__(kTileThreads) void pr_anchor_spmv(dev::CsrView<VT> 
 }
 
 // v = a (+ d) ; out = v / sum(v) with a compensated sum (two passes).
+// (the element is (a + d) * xscale: xscale is 1 except in raw runs)
 __global__ __launch_bounds__(kTileThreads) void pr_sum_kernel(const double *__restrict__ a,
                                                               const float *__restrict__ d, long n,
-                                                              double *__restrict__ ps, double *__restrict__ pc) {
+                                                              const PrState *st, double *__restrict__ ps,
+                                                              double *__restrict__ pc) {
 #pragma clang fp contract(off)
   __shared__ double red_s[kTileThreads], red_c[kTileThreads];
+  const double sc = st->xscale;
   CSum cs{0, 0};
   for (long i = blockIdx.x * static_cast<long>(kTileThreads) + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * kTileThreads)
-    csum_add(cs, d ? a[i] + static_cast<double>(d[i]) : a[i]);
+    csum_add(cs, (d ? a[i] + static_cast<double>(d[i]) : a[i]) * sc);
   const CSum bs = block_csum(cs, red_s, red_c);
   if (threadIdx.x == 0) {
     ps[blockIdx.x] = bs.s;
@@ -432,12 +466,12 @@ __global__ __launch_bounds__(kTileThreads) void pr_sum_finalize(const double *__
 }
 
 __global__ __launch_bounds__(256) void pr_scale_kernel(const double *__restrict__ a, const float *__restrict__ d,
-                                                       long n, const double *__restrict__ total,
+                                                       long n, const PrState *st, const double *__restrict__ total,
                                                        double *__restrict__ out) {
 #pragma clang fp contract(off)
-  const double S = *total;
+  const double S = *total, sc = st->xscale;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += static_cast<long>(gridDim.x) * 256)
-    out[i] = (d ? a[i] + static_cast<double>(d[i]) : a[i]) / S;
+    out[i] = ((d ? a[i] + static_cast<double>(d[i]) : a[i]) * sc) / S;
 }
 
 // cs[colidx[k] - f_col] = vals[k] over the stored entries: for a
@@ -559,7 +593,8 @@ struct AnchorMove {
 // delta products).
 template <typename VT, typename XT, typename AT, bool MOVE, int COLS>
 __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restrict__ val, const XT *__restrict__ x,
-                                     AT *__restrict__ prod, XT *xs, const AnchorMove &am, double *red) {
+                                     AT *__restrict__ prod, XT *xs, const AnchorMove &am, double *red,
+                                     XT xsc = XT(1)) {
 #pragma clang fp contract(off)
   // superblocks of one XCD (blockIdx % 8) are contiguous, so the runs that
   // share a cache line at a run boundary are written through the same L2
@@ -598,7 +633,7 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
           t[u] = XT(0);
         }
       } else {
-        t[u] = i < nc ? x[c0 + i] : XT(0);
+        t[u] = i < nc ? static_cast<XT>(x[c0 + i] * xsc) : XT(0);
       }
     }
 #pragma unroll
@@ -704,7 +739,9 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a,
                                                                 const PrState *st) {
   if (st->done) return;
   __shared__ XT xs[COLS];
-  pb_scatter_sb<VT, XT, AT, false, COLS>(a, val, x, prod, xs, AnchorMove{}, nullptr);
+  // (raw reference-rule runs: x is the unnormalised b, the iterate x * xscale)
+  pb_scatter_sb<VT, XT, AT, false, COLS>(a, val, x, prod, xs, AnchorMove{}, nullptr,
+                                         static_cast<XT>(st->raw ? st->xscale : 1.0));
 }
 
 // Mixed runs, one launch per iteration: in an iteration that moves the
@@ -752,7 +789,7 @@ struct PbPair<double> {
 // from the bin's own slice.  Ends with a barrier.
 template <typename VT, typename XT, typename AT>
 __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lval, const XT *__restrict__ x,
-                                     const AT *__restrict__ prod, double *ybin) {
+                                     const AT *__restrict__ prod, double *ybin, XT xsc = XT(1)) {
 #pragma clang fp contract(off)
   const int bin = blockIdx.x, tid = threadIdx.x;
   const int base = bin << kPbShift;
@@ -851,7 +888,7 @@ __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lva
 #pragma unroll
     for (int g = 0; g < LU; g++)
 #pragma unroll
-      for (int u = 0; u < 4; u++) xv[g][u] = static_cast<AT>(x[base + c[g][u]]);
+      for (int u = 0; u < 4; u++) xv[g][u] = static_cast<AT>(x[base + c[g][u]] * xsc);
 #pragma unroll
     for (int g = 0; g < LU; g++) {
       double acc = 0.0;
@@ -882,7 +919,7 @@ __global__ __launch_bounds__(1024) void pb_gather(PbView v, const VT *__restrict
   if (st->done) return;
   __shared__ double ybin[kPbRows];  // FP64 sums (products may be FP32)
   __shared__ double red[16];
-  pb_accumulate<VT, XT, AT>(v, lval, x, prod, ybin);
+  pb_accumulate<VT, XT, AT>(v, lval, x, prod, ybin, static_cast<XT>(st->raw ? st->xscale : 1.0));
   const int bin = blockIdx.x, tid = threadIdx.x;
   const int base = bin << kPbShift;
   const int rows = min(kPbRows, v.n - base);
@@ -1472,6 +1509,12 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
   h.max_it = p.max_iteration;
   h.rule = static_cast<int>(p.rule);
   h.mixed = mixed ? 1 : 0;
+  // FP64 reference-rule runs on propagation-blocked products keep each
+  // update's b unnormalised and carry its scale (1/max) to the readers of
+  // the iterate, so the normalise pass writes nothing (a pass over the
+  // iterate less per iteration)
+  h.raw = (fused && pbm && !sn && !mixed && !pattern) ? 1 : 0;
+  h.xscale = 1.0;
   // mixed runs start from the anchor xhat = 0 (so yhat = A xhat = 0 with no
   // product) and delta = b0: the first phase carries the whole iterate in the
   // low precision, and the first anchor move comes when it has converged to
@@ -1647,9 +1690,14 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
           else
             launch_update<double, double, false>(tmode, d64, p.values, xin, nullptr, tp, xout, st, slots, s);
           fin_max();
-          hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
-                             nullptr, nullptr, nullptr, nullptr, nullptr, n, st, w.perr.as<double>(),
-                             w.ps.as<double>(), w.pc.as<double>());
+          if (pbm)
+            hipLaunchKernelGGL(pr_normalize<4>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
+                               nullptr, nullptr, nullptr, nullptr, nullptr, n, st, w.perr.as<double>(),
+                               w.ps.as<double>(), w.pc.as<double>());
+          else
+            hipLaunchKernelGGL(pr_normalize<0>, dim3(gN), dim3(kTileThreads), 0, s, xout, xin, nullptr, nullptr,
+                               nullptr, nullptr, nullptr, nullptr, nullptr, n, st, w.perr.as<double>(),
+                               w.ps.as<double>(), w.pc.as<double>());
           fin_norm(gN);
         }
       } else {
@@ -1682,11 +1730,11 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
   // final sum normalisation (Matrix.cpp:52-59), compensated
   const double *a = mixed ? w.xhat.as<double>() : X[it & 1];
   const float *dd = mixed ? D[it & 1] : nullptr;
-  hipLaunchKernelGGL(pr_sum_kernel, dim3(gN), dim3(kTileThreads), 0, s, a, dd, n, w.ps.as<double>(),
+  hipLaunchKernelGGL(pr_sum_kernel, dim3(gN), dim3(kTileThreads), 0, s, a, dd, n, st, w.ps.as<double>(),
                      w.pc.as<double>());
   hipLaunchKernelGGL(pr_sum_finalize, dim3(1), dim3(kTileThreads), 0, s, w.ps.as<double>(), w.pc.as<double>(),
                      static_cast<int>(gN), w.total.as<double>());
-  hipLaunchKernelGGL(pr_scale_kernel, dim3(gN), dim3(256), 0, s, a, dd, n, w.total.as<double>(), out_dev);
+  hipLaunchKernelGGL(pr_scale_kernel, dim3(gN), dim3(256), 0, s, a, dd, n, st, w.total.as<double>(), out_dev);
   TBSLA_HIP_CHECK(hipGetLastError());
   TBSLA_HIP_CHECK(hipStreamSynchronize(s));
   float ms = 0;
