@@ -174,6 +174,14 @@ def test_blocked_products_with_teleport_vector_and_bins(tb, oracle, gpu):
         # makes their stopping bound tighter: they may stop earlier)
         assert rb["iterations"] <= ra["iterations"] + 1
         assert float((a - b).abs().sum()) <= 1e-12, prec
+        # the reference rule (FP64 blocked runs keep b unnormalised and carry
+        # 1/max to its readers): the same iterations within one, the same vector
+        ra = p.page_rank_device(a, epsilon=1e-9, max_iteration=400, rule="reference", teleport=vd,
+                                precision=prec, gather="tiles")
+        rb = p.page_rank_device(b, epsilon=1e-9, max_iteration=400, rule="reference", teleport=vd,
+                                precision=prec, gather="blocked")
+        assert abs(rb["iterations"] - ra["iterations"]) <= 1, (prec, ra, rb)
+        assert float((a - b).abs().sum()) <= 1e-12, prec
 
 
 def test_mixed_precisions_with_teleport_vector_and_node_set(tb, oracle, gpu):
