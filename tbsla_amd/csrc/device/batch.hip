@@ -14,6 +14,8 @@
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
+#include <type_traits>
+#include <utility>
 
 #include "batch.hpp"
 #include "mirror.hpp"
@@ -182,6 +184,32 @@ struct Quad<double> {
   }
 };
 
+// Lane L of each 16-lane row of the wavefront, broadcast to the whole row
+// (a DPP row broadcast: a VALU move, no LDS round trip).
+template <int L>
+__device__ inline int row_bcast(int v) {
+  static_assert(L >= 0 && L < 16, "lane within a 16-lane row");
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + L, 0xF, 0xF, false);
+}
+template <int L>
+__device__ inline float row_bcast(float v) {
+  return __int_as_float(row_bcast<L>(__float_as_int(v)));
+}
+template <int L>
+__device__ inline double row_bcast(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = row_bcast<L>(static_cast<int>(b)), hi = row_bcast<L>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+// f(integral_constant<int, I>) for I = B .. E-1
+template <int B, int E, typename F>
+__device__ inline void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 // SpMM rows of one work item (64 rows): each wavefront keeps four rows in
 // flight, 16 lanes per row, lane quad q holding RHS 4q..4q+3.  A row's
 // column indices and values are loaded 16 at a time (one entry per lane) and
@@ -257,27 +285,29 @@ __device__ void spmm_item(const BatchDev<VT, XT> &d, const SpmmGroup<VT> &G, int
             vq = dev::Val<VT>::get(vals, base + q);
           }
         }
-#pragma unroll 1
-        for (int k0 = 0; k0 < 16; k0 += UNR) {
+        // entry e of the chunk sits on lane e of the row's 16 lanes: its
+        // column and value are row broadcasts; the UNR gathers of a group
+        // are unconditional (entries past the row's end, and lanes whose
+        // sets are all finished, read row 0) so they issue back to back
+        static_for<0, 16 / UNR>([&](auto G) {
           AT xv[UNR][4], vv[UNR];
+          static_for<0, UNR>([&](auto U) {
+            constexpr int e = decltype(G)::value * UNR + decltype(U)::value;
+            const int c = row_bcast<e>(cq);
+            vv[U] = row_bcast<e>(vq);
+            const int ce = e < cnt && any ? c : 0;
+            XT t4[4];
+            Quad<XT>::load(X + static_cast<long>(ce) * kLanes + 4 * q, t4);
 #pragma unroll
-          for (int u = 0; u < UNR; u++) {
-            const int src = (sub << 4) + k0 + u;
-            const int c = __shfl(cq, src, 64);
-            vv[u] = __shfl(vq, src, 64);
-            if (k0 + u < cnt && any) {
-              XT t4[4];
-              Quad<XT>::load(X + static_cast<long>(c) * kLanes + 4 * q, t4);
-#pragma unroll
-              for (int t = 0; t < 4; t++) xv[u][t] = static_cast<AT>(t4[t]);
-            }
-          }
+            for (int t = 0; t < 4; t++) xv[U][t] = static_cast<AT>(t4[t]);
+          });
+          // (an entry past the row's end has value 0 -- its lane loaded no
+          // value -- and adds a zero: the sums are unchanged)
 #pragma unroll
           for (int u = 0; u < UNR; u++)
-            if (k0 + u < cnt)
 #pragma unroll
-              for (int t = 0; t < 4; t++) acc[t] += vv[u] * xv[u][t];
-        }
+            for (int t = 0; t < 4; t++) acc[t] += vv[u] * xv[u][t];
+        });
       }
       if (rv && any) {
         const unsigned long long mb = mbrow >> (4 * q);
