@@ -173,8 +173,23 @@ __global__ __launch_bounds__(kTileThreads) void adj_finalize(const double *__res
   if (st->done) return;
   __shared__ double red[4][kTileThreads];
   double q[4] = {0, 0, 0, 0};
-  for (int i = threadIdx.x; i < nparts; i += kTileThreads)
-    for (int k = 0; k < 4; k++) q[k] += part[4 * i + k];
+  // four partials of U blocks per thread per step, the loads issued before
+  // the additions (in the order of one block per step)
+  constexpr int U = 4;
+  for (int i0 = threadIdx.x; i0 < nparts; i0 += U * kTileThreads) {
+    double t[U][4];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = min(i0 + u * kTileThreads, nparts - 1);
+#pragma unroll
+      for (int k = 0; k < 4; k++) t[u][k] = part[4 * i + k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i0 + u * kTileThreads < nparts)
+#pragma unroll
+        for (int k = 0; k < 4; k++) q[k] += t[u][k];
+  }
   block_sums<4>(q, red);
   if (threadIdx.x == 0) {
     st->a = q[0];
