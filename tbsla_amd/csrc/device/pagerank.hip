@@ -1369,6 +1369,24 @@ int next_batch(const PrState &h, int check) {
 
 namespace {
 
+// Iterations the state h predicts are left (as next_batch, unclamped): 1
+// once the bound is met (a certification step of a mixed run may follow),
+// `check` when the last two residuals give no decay to extrapolate.
+int iterations_left(const PrState &h, int check) {
+  const bool sumnorm = h.rule != 0;
+  const double cur = sumnorm ? h.bound : h.err, prev = sumnorm ? h.bound_prev : h.err_prev;
+  const double target = sumnorm ? h.bound_target : h.eps;
+  if (cur <= target) return 1;
+  if (!(cur > 0.0) || !(prev > cur) || !(target > 0.0)) return check;
+  const double left = std::log(target / cur) / std::log(cur / prev);
+  if (!(left > 0.0) || !(left < 1e6)) return check;
+  return static_cast<int>(std::ceil(left));
+}
+
+}  // namespace
+
+namespace {
+
 // The engine.  arena != nullptr: every buffer of the run is a slice of the
 // arena (cap bytes) and nothing is allocated (the caller has checked that
 // the mirror exists and that no lazily built layout is needed).
@@ -1607,8 +1625,7 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
   };
   double *Z = w.Z.as<double>();
   const double *colscale = colscale64;
-  while (launched < p.max_iteration) {
-    const int batch = std::min(next, p.max_iteration - launched);
+  auto launch_batch = [&](int batch) {
     for (int k = 0; k < batch; k++) {
       const int it = launched + k;
       double *xin = X[it & 1], *xout = X[(it + 1) & 1];
@@ -1719,10 +1736,62 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
     }
     TBSLA_HIP_CHECK(hipGetLastError());
     launched += batch;
-    TBSLA_HIP_CHECK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
-    TBSLA_HIP_CHECK(hipStreamSynchronize(s));
-    if (hs.done) break;
-    next = next_batch(hs, check);
+  };
+  // pinned state copies of the two batches in flight and their events
+  // (released on every exit; the stream has drained by then or the run failed)
+  struct ReadBack {
+    PrState *hbuf = nullptr;
+    hipEvent_t cev[2] = {nullptr, nullptr};
+    ~ReadBack() {
+      if (hbuf) (void)hipHostFree(hbuf);
+      for (hipEvent_t e : cev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } rb;
+  PrState *&hbuf = rb.hbuf;
+  hipEvent_t *cev = rb.cev;
+  if (arena) {
+    // allocation-free runs: the state is read after every batch
+    while (launched < p.max_iteration) {
+      launch_batch(std::min(next, p.max_iteration - launched));
+      TBSLA_HIP_CHECK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+      TBSLA_HIP_CHECK(hipStreamSynchronize(s));
+      if (hs.done) break;
+      next = next_batch(hs, check);
+    }
+  } else {
+    // Up to two batches in flight, each followed by a copy of the state to
+    // pinned memory and an event: while the host waits for one batch's
+    // state, the next batch is already queued when the state read before
+    // predicts more iterations than are in flight, so the GPU does not idle
+    // at the reads (kernels launched past convergence return at once).
+    TBSLA_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&hbuf), 2 * sizeof(PrState), hipHostMallocDefault));
+    for (int k = 0; k < 2; k++) TBSLA_HIP_CHECK(hipEventCreateWithFlags(&cev[k], hipEventDisableTiming));
+    int head = 0, npend = 0;  // slot of the oldest batch in flight, batches in flight
+    auto enqueue = [&](int batch) {
+      launch_batch(batch);
+      const int sl = (head + npend) & 1;
+      TBSLA_HIP_CHECK(hipMemcpyAsync(hbuf + sl, st, sizeof(PrState), hipMemcpyDeviceToHost, s));
+      TBSLA_HIP_CHECK(hipEventRecord(cev[sl], s));
+      npend++;
+    };
+    bool have = false;  // hs holds a state read back
+    if (launched < p.max_iteration) enqueue(std::min(check, p.max_iteration - launched));
+    while (npend > 0) {
+      if (npend == 1 && launched < p.max_iteration) {
+        int nb = check;
+        if (have) nb = std::min(check, iterations_left(hs, check) - (launched - hs.it));
+        nb = std::min(nb, p.max_iteration - launched);
+        if (nb > 0) enqueue(nb);
+      }
+      TBSLA_HIP_CHECK(hipEventSynchronize(cev[head]));
+      hs = hbuf[head];
+      head ^= 1;
+      npend--;
+      have = true;
+      if (hs.done) break;
+      if (npend == 0 && launched < p.max_iteration) enqueue(std::min(next_batch(hs, check), p.max_iteration - launched));
+    }
   }
   if (!arena) TBSLA_HIP_CHECK(hipEventRecord(e1, s));
   const double host_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
