@@ -581,9 +581,11 @@ struct AnchorMove {
   double *xhat;
   float *xhat32;  // fl32(xhat), for the scale term of the mixed iterations (or nullptr)
   const float *delta;
-  double *z;
+  // value-free anchor products: 1/count of each column (the product's input
+  // is colscale .* xhat, formed where it is read), else nullptr
   const double *colscale;
   double *psum, *pcomp;
+  int first;  // set on the device: the run's first move, whose anchor is still 0
 };
 
 // The far products of one superblock (picked from blockIdx): its slice of x
@@ -619,16 +621,12 @@ __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restric
       if (MOVE) {
         if (i < nc) {
           const long c = c0 + i;
-          const double v = am.xhat[c] + static_cast<double>(am.delta[c]);
+          // (0 + d is d: the first move need not read the zero anchor)
+          const double v = am.first ? static_cast<double>(am.delta[c]) : am.xhat[c] + static_cast<double>(am.delta[c]);
           am.xhat[c] = v;
           if (am.xhat32) am.xhat32[c] = static_cast<float>(v);
           csum_add(cs, v);
-          double xv = v;
-          if (am.z) {
-            xv = am.colscale[c] * v;
-            am.z[c] = xv;
-          }
-          t[u] = static_cast<XT>(xv);
+          t[u] = static_cast<XT>(am.colscale ? am.colscale[c] * v : v);
         } else {
           t[u] = XT(0);
         }
@@ -744,23 +742,40 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a,
                                          static_cast<XT>(st->raw ? st->xscale : 1.0));
 }
 
+// The value-free FP64 anchor product takes the wide scatter order (a 128 KB
+// x slice, runs of about 32 products), as the value-free FP64 runs do; with
+// FP64 values the narrow order was faster.
+template <typename VT>
+constexpr bool pb_value_free() {
+  return std::is_same<VT, dev::Pattern>::value;
+}
+template <typename VA>
+constexpr bool pb_anchor_wide() {
+  return pb_value_free<VA>();
+}
+
 // Mixed runs, one launch per iteration: in an iteration that moves the
 // anchor, the anchor move and the FP64 anchor product's far products (values
-// VA, FP64, layout a); otherwise the delta product's (values VD, delta XD,
-// sums AD) over layout ad of superblock width DCOLS (kPbColsWide: the 4-byte
-// x slice fills the same LDS).
+// VA, FP64, layout a of superblock width kPbColsWide for value-free anchors,
+// else kPbCols); otherwise the delta product's (values VD, delta XD, sums AD)
+// over layout ad of superblock width DCOLS (kPbColsWide: the 4-byte x slice
+// fills the LDS of a narrow FP64 slice).
 template <typename VA, typename VD, typename XD, typename AD, int DCOLS>
 __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mx(PbScatterArgs a, PbScatterArgs ad,
                                                                    const VA *__restrict__ vala,
                                                                    const VD *__restrict__ vald,
                                                                    const XD *__restrict__ xd, void *prod,
                                                                    AnchorMove am, const PrState *st) {
-  static_assert(sizeof(XD) * DCOLS <= sizeof(double) * kPbCols, "the delta slice fits the LDS");
+  constexpr int ACOLS = pb_anchor_wide<VA>() ? kPbColsWide : kPbCols;
+  static_assert(sizeof(XD) * DCOLS <= sizeof(double) * ACOLS, "the delta slice fits the LDS");
   if (st->done) return;
-  __shared__ double xs[kPbCols];
+  __shared__ double xs[ACOLS];
   __shared__ double red[48];
-  if (st->need_anchor)
-    pb_scatter_sb<VA, double, double, true, kPbCols>(a, vala, nullptr, static_cast<double *>(prod), xs, am, red);
+  if (st->need_anchor) {
+    AnchorMove m = am;
+    m.first = st->anchors == 1;  // (the count includes the move this iteration makes)
+    pb_scatter_sb<VA, double, double, true, ACOLS>(a, vala, nullptr, static_cast<double *>(prod), xs, m, red);
+  }
   else
     pb_scatter_sb<VD, XD, AD, false, DCOLS>(ad, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am,
                                             nullptr);
@@ -786,10 +801,12 @@ struct PbPair<double> {
 
 // The row sums of one bin (blockIdx) into the FP64 LDS accumulator ybin
 // (kPbRows entries): the bin's far products and its local products, x read
-// from the bin's own slice.  Ends with a barrier.
-template <typename VT, typename XT, typename AT>
+// from the bin's own slice (XCS: x[j] scaled by xcs[j], the value-free anchor
+// product's input colscale .* xhat; else by xsc).  Ends with a barrier.
+template <typename VT, typename XT, typename AT, bool XCS = false>
 __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lval, const XT *__restrict__ x,
-                                     const AT *__restrict__ prod, double *ybin, XT xsc = XT(1)) {
+                                     const AT *__restrict__ prod, double *ybin, XT xsc = XT(1),
+                                     const double *__restrict__ xcs = nullptr) {
 #pragma clang fp contract(off)
   const int bin = blockIdx.x, tid = threadIdx.x;
   const int base = bin << kPbShift;
@@ -888,7 +905,10 @@ __device__ inline void pb_accumulate(const PbView &v, const VT *__restrict__ lva
 #pragma unroll
     for (int g = 0; g < LU; g++)
 #pragma unroll
-      for (int u = 0; u < 4; u++) xv[g][u] = static_cast<AT>(x[base + c[g][u]] * xsc);
+      for (int u = 0; u < 4; u++) {
+        const int j = base + c[g][u];
+        xv[g][u] = static_cast<AT>(XCS ? xcs[j] * x[j] : x[j] * xsc);
+      }
 #pragma unroll
     for (int g = 0; g < LU; g++) {
       double acc = 0.0;
@@ -988,7 +1008,8 @@ __global__ __launch_bounds__(1024) void pb_gather_sn(PbView v, const VT *__restr
 // (FP64 anchor, FP32 delta) and is scaled by c = 1 / sum each iteration, as
 // the FP64 runs are: x' = c*(beta*A x + t*e), e the teleport pattern.  The
 // iteration that moves the anchor (xhat += delta, then delta = 0) sums the
-// FP64 anchor product (values VA, input xa = z or xhat) and stores the
+// FP64 anchor product (values VA, input xhat; value-free anchors: acs .*
+// xhat, acs = 1/count of each column, as the scatter forms it) and stores the
 // anchor's scaled one-step change w = c_a*(beta*A xhat + t_a*e) - xhat in
 // FP32 (c_a, t_a: that iteration's scale and teleport term); the next delta
 // is d' = w.  The other iterations sum the delta product (values VD, input
@@ -1001,7 +1022,7 @@ __global__ __launch_bounds__(1024) void pb_gather_sn(PbView v, const VT *__restr
 // the compensated sum of d' as per-bin partials.
 template <typename VA, typename VD, typename XD, typename AD, int TM, bool PAT32>
 __global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restrict__ lvala,
-                                                     const double *__restrict__ xa, const VD *__restrict__ lvald,
+                                                     const double *__restrict__ acs, const VD *__restrict__ lvald,
                                                      const XD *__restrict__ xd, const void *prod, Teleport tp,
                                                      const double *__restrict__ xhat,
                                                      const float *__restrict__ xhat32, float *__restrict__ w,
@@ -1015,7 +1036,8 @@ __global__ __launch_bounds__(1024) void pb_gather_mx(PbView v, const VA *__restr
   __shared__ double red[48];
   const bool anchor = st->need_anchor != 0;
   if (anchor)
-    pb_accumulate<VA, double, double>(v, lvala, xa, static_cast<const double *>(prod), ybin);
+    pb_accumulate<VA, double, double, pb_value_free<VA>()>(v, lvala, xhat, static_cast<const double *>(prod), ybin,
+                                                          1.0, acs);
   else
     pb_accumulate<VD, XD, AD>(v, lvald, xd, static_cast<const AD *>(prod), ybin);
   const int bin = blockIdx.x, tid = threadIdx.x;
@@ -1166,7 +1188,7 @@ void launch_pb_sn(int tmode, const PbMatrix &pb, int vslot, const double *xin, c
 
 // Buffers of a mixed sum-normalised iteration.
 struct MxBufs {
-  AnchorMove am;           // anchor xhat (and z) with the scatter's sum partials
+  AnchorMove am;           // anchor xhat (and 1/count, value-free) with the scatter's sum partials
   const void *xd;          // the delta product's input: delta (F32/BF16) or zd (PATTERN_F32)
   float *w;                // FP32 one-step change of the anchor
   const float *dold;       // delta of the current iterate
@@ -1184,17 +1206,19 @@ void launch_pb_mx_tm(const PbMatrix &pb, int va, int vd, const MxBufs &b, const 
   const VA *la = va >= 0 ? pb.lval[va].as<VA>() : nullptr;
   const VD *fd = vd >= 0 ? pb.fval[vd].as<VD>() : nullptr;
   const VD *ld = vd >= 0 ? pb.lval[vd].as<VD>() : nullptr;
-  const double *xa = b.am.z ? b.am.z : b.am.xhat;
   const XD *xd = static_cast<const XD *>(b.xd);
   // the delta products of BF16 and value-free runs take the wide scatter
   // order, FP32-valued ones the narrow one (measured: value-free FP32 delta
   // scatter 154 -> 140 us wide, BF16 196 -> 192, FP32 191 -> 196)
   constexpr bool wide = pb_delta_wide<VD>();
+  // (a wide anchor's 128 KB slice leaves room for one block per CU without
+  // the LDS pad)
+  constexpr bool awide = pb_anchor_wide<VA>();
   hipLaunchKernelGGL((pb_scatter_mx<VA, VD, XD, float, wide ? kPbColsWide : kPbCols>), pb_scatter_grid(pb),
-                     dim3(kPbScatterThreads), kPbScatterLdsPad, s, pb_args(pb), wide ? pb_args_wide(pb) : pb_args(pb),
-                     fa, fd, xd, pb.prod.get(), b.am, st);
+                     dim3(kPbScatterThreads), awide ? 0 : kPbScatterLdsPad, s, awide ? pb_args_wide(pb) : pb_args(pb),
+                     wide ? pb_args_wide(pb) : pb_args(pb), fa, fd, xd, pb.prod.get(), b.am, st);
   hipLaunchKernelGGL((pb_gather_mx<VA, VD, XD, float, TM, PAT32>), dim3(pb.nbins), dim3(1024), 0, s, pb_view(pb), la,
-                     xa, ld, xd, pb.prod.get(), tp, b.am.xhat, b.am.xhat32, b.w, b.dold, b.dnew, b.zdnew, b.colscale32, st,
+                     b.am.colscale, ld, xd, pb.prod.get(), tp, b.am.xhat, b.am.xhat32, b.w, b.dold, b.dnew, b.zdnew, b.colscale32, st,
                      sp.perr, sp.ps, sp.pc);
 }
 
@@ -1633,8 +1657,8 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
         const SnPartials sp{w.perr.as<double>(), w.ps.as<double>(), w.pc.as<double>()};
         if (mixed) {
           MxBufs b;
-          b.am = AnchorMove{w.xhat.as<double>(), w.xhat32.as<float>(), D[it & 1], mixed_pattern ? Z : nullptr,
-                            colscale, w.hsum.as<double>(), w.hcomp.as<double>()};
+          b.am = AnchorMove{w.xhat.as<double>(), w.xhat32.as<float>(), D[it & 1], mixed_pattern ? colscale : nullptr,
+                            w.hsum.as<double>(), w.hcomp.as<double>(), 0};
           b.xd = pat32 ? static_cast<const void *>(ZD[it & 1]) : static_cast<const void *>(D[it & 1]);
           b.w = w.W.as<float>();
           b.dold = D[it & 1];
@@ -1657,8 +1681,11 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
         } else {
           launch_pb_sn<double, false>(tmode, *pbm, 0, xin, nullptr, tp, xout, nullptr, nullptr, st, sp, s);
         }
+        // (the anchor move's sum partials: one per superblock of the anchor
+        // product's scatter order)
+        const int nsb_anchor = (pat32 || mixed_pattern) ? pbm->nsb_w : pbm->nsb;
         hipLaunchKernelGGL(pr_finalize_sn, dim3(1), dim3(kTileThreads), 0, s, sp.perr, sp.ps, sp.pc, pbm->nbins,
-                           mixed ? w.hsum.as<double>() : nullptr, mixed ? w.hcomp.as<double>() : nullptr, pbm->nsb,
+                           mixed ? w.hsum.as<double>() : nullptr, mixed ? w.hcomp.as<double>() : nullptr, nsb_anchor,
                            st);
       } else if (fused) {
         if (mixed) {
