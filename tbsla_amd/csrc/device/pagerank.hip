@@ -573,10 +573,10 @@ struct PbScatterArgs {
 
 // The anchor move of a mixed run, folded into the scatter of the iteration
 // that makes it: over the superblock's columns c, xhat[c] += delta[c], and
-// the product's input is z[c] = colscale[c] * xhat[c] (value-free anchor
-// product; z is written for the gather) or xhat[c] itself (z == nullptr),
-// and xhat32[c] = fl32(xhat[c]).  Each superblock leaves the compensated
-// sum of its new xhat in (psum, pcomp)[superblock].
+// the product's input is colscale[c] * xhat[c] (value-free anchor product;
+// the gather forms the same product for its local entries) or xhat[c] itself
+// (colscale == nullptr), and xhat32[c] = fl32(xhat[c]).  Each superblock
+// leaves the compensated sum of its new xhat in (psum, pcomp)[superblock].
 struct AnchorMove {
   double *xhat;
   float *xhat32;  // fl32(xhat), for the scale term of the mixed iterations (or nullptr)
@@ -591,8 +591,7 @@ struct AnchorMove {
 // The far products of one superblock (picked from blockIdx): its slice of x
 // staged in xs (LDS), then every far product v * x[c] written to its slot.
 // With MOVE the slice is the anchor move's result instead of x.  COLS: the
-// superblock width of the layout `a` (kPbCols, or kPbColsWide for the FP32
-// delta products).
+// superblock width of the layout `a` (kPbCols or kPbColsWide).
 template <typename VT, typename XT, typename AT, bool MOVE, int COLS>
 __device__ inline void pb_scatter_sb(const PbScatterArgs &a, const VT *__restrict__ val, const XT *__restrict__ x,
                                      AT *__restrict__ prod, XT *xs, const AnchorMove &am, double *red,
@@ -742,13 +741,13 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter(PbScatterArgs a,
                                          static_cast<XT>(st->raw ? st->xscale : 1.0));
 }
 
-// The value-free FP64 anchor product takes the wide scatter order (a 128 KB
-// x slice, runs of about 32 products), as the value-free FP64 runs do; with
-// FP64 values the narrow order was faster.
 template <typename VT>
 constexpr bool pb_value_free() {
   return std::is_same<VT, dev::Pattern>::value;
 }
+// The value-free FP64 anchor product takes the wide scatter order (a 128 KB
+// x slice, runs of about 32 products), as the value-free FP64 runs do; with
+// FP64 values the narrow order was faster.
 template <typename VA>
 constexpr bool pb_anchor_wide() {
   return pb_value_free<VA>();
@@ -775,10 +774,10 @@ __global__ __launch_bounds__(kPbScatterThreads) void pb_scatter_mx(PbScatterArgs
     AnchorMove m = am;
     m.first = st->anchors == 1;  // (the count includes the move this iteration makes)
     pb_scatter_sb<VA, double, double, true, ACOLS>(a, vala, nullptr, static_cast<double *>(prod), xs, m, red);
-  }
-  else
+  } else {
     pb_scatter_sb<VD, XD, AD, false, DCOLS>(ad, vald, xd, static_cast<AD *>(prod), reinterpret_cast<XD *>(xs), am,
                                             nullptr);
+  }
 }
 
 struct PbView {
