@@ -1571,8 +1571,13 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
   TBSLA_HIP_CHECK(hipMemcpyAsync(w.st.get(), &h, sizeof(h), hipMemcpyHostToDevice, s));
   PrState *st = w.st.as<PrState>();
   double *X[2] = {w.X0.as<double>(), w.X1.as<double>()};
-  hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[0], n, x0);
-  hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[1], n, 0.0);
+  // (mixed runs on propagation-blocked products keep the iterate as the
+  // anchor plus the delta and never read X)
+  const bool pb_mixed = mixed && sn;
+  if (!pb_mixed) {
+    hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[0], n, x0);
+    hipLaunchKernelGGL(fill_f64, dim3(gN), dim3(256), 0, s, X[1], n, 0.0);
+  }
   unsigned long long *slots = w.slots.as<unsigned long long>();
   hipLaunchKernelGGL(zero_u64, dim3(1), dim3(kMaxSlots), 0, s, slots, kMaxSlots);
   float *D[2] = {nullptr, nullptr};
@@ -1612,7 +1617,9 @@ PageRankReport run_page_rank(const cpp::Matrix &m, const PageRankParams &p, doub
       hipLaunchKernelGGL(scale_into_f32, dim3(gN), dim3(256), 0, s, colscale64, b0, ZD[0], n);
     hipLaunchKernelGGL(fill_f32, dim3(gN), dim3(256), 0, s, ZD[1], n, 0.0f);
   }
-  if (pattern || (mixed && m.count_stochastic())) {
+  // (propagation-blocked mixed runs form z = colscale .* xhat where they
+  // read it)
+  if (pattern || (mixed && m.count_stochastic() && !pb_mixed)) {
     ws.alloc(w.Z, sizeof(double) * n);
     hipLaunchKernelGGL(scale_into, dim3(gN), dim3(256), 0, s, colscale64, X[0], w.Z.as<double>(), n);
   }
