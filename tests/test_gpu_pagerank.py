@@ -188,7 +188,10 @@ def test_mixed_precisions_with_teleport_vector_and_node_set(tb, oracle, gpu):
     """The mixed schemes (FP32 and BF16 values, value-free FP32) with an
     explicit teleport distribution and with a personalised node set, on both
     product structures: within 1e-10 L1 of the FP64 run of the same problem
-    (the anchor products are FP64 in every teleport mode)."""
+    (the anchor products are FP64 in every teleport mode).  The graph spans
+    three 16384-column superblocks of the wide scatter order, the last one
+    partial, which the value-free anchor products and their moves (the
+    first from a zero anchor, the others from a moved one) run over."""
     import torch
     n = 5 * 8192 + 1234
     p = tb.MatrixCSR().fill_cqmat_stochastic(n, n, 12, 0.4, 77)
